@@ -1,0 +1,188 @@
+/*
+ * porescale.h -- C-ABI of the MI355X (gfx950) porescale library.
+ *
+ * The library replaces the two compute hot paths of the Fortran reference
+ * (Eng-RSMY/porescale):
+ *
+ *   LBM  the D3Q19 colour-gradient two-phase lattice Boltzmann step of
+ *        LBM-LSM/lsm_lbm_static_alpha_beta_multipore.f ("L1"):
+ *        majorityrule time loop L1:3634-3947 (collision L1:2407, calcFs
+ *        L1:2626, recolouring L1:3784-3816, streaming L1:3833-3839,
+ *        bounce-back L1:3890-3915, densities L1:3934-3945) and its
+ *        diagnostics check_state L1:2898.
+ *   LSM  the lattice spring model of LSM2S/Cxxxx/XX_P.f ("XX") and
+ *        LSM2S/Cyzyz/YZ_P.f ("YZ"): the force loops XX:1404-1500, the damped
+ *        Verlet loop XX:1377-1545 and calc_stress XX:1628.
+ *
+ * Conventions
+ *   - Plain pointers and sizes; every array is host memory owned by the
+ *     caller unless the name says "dev".  Voxel arrays are Fortran order
+ *     ind(i,j,k), i fastest.  Population arrays are Fortran (19,nl).
+ *   - Every entry point returns 0 on success and a negative PSX_E* code on
+ *     failure; psx_last_error() returns the calling thread's last message.
+ *     Nothing inside the library stops the process (the reference's
+ *     write+stop, e.g. L1:474-477, becomes an error code).
+ *   - A handle owns its device buffers and one HIP stream; calls on one
+ *     handle are serialised on that stream.  Use one handle per host thread.
+ *   - Fortran drivers bind these with ISO_C_BINDING (porescale_amd/fortran).
+ */
+#ifndef PORESCALE_H
+#define PORESCALE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PSX_OK = 0,
+    PSX_EINVAL = -1,  /* bad argument                       */
+    PSX_EHIP = -2,    /* HIP runtime error / no GPU         */
+    PSX_ENOMEM = -3,  /* allocation failure                 */
+    PSX_ESTATE = -4,  /* call not valid in this state       */
+    PSX_ECONV = -5    /* solver did not reach the tolerance */
+};
+
+const char *psx_last_error(void);
+/* Library version string and the device it runs on (name, CU count). */
+int psx_device_info(char *name, int32_t name_len, int32_t *num_cu, double *hbm_gb);
+
+/* =============================================================== LBM ===== */
+enum { PSX_COLL_REF_TRT = 0, PSX_COLL_TRT = 1, PSX_COLL_MRT = 2 };
+enum { PSX_PREC_F64 = 0, PSX_PREC_F32 = 1 };
+
+/* Run parameters: lines 1-8, 9, 17, 19, 21, 22 of the LBM INPUT_PARAMETERS
+   (L1:435-458) plus dt (L1:465).  sig is the value as read (the library
+   applies the factor 1/32 of L1:1020); lsize only scales the Reynolds number. */
+typedef struct {
+    double cr2, rr0, rb0, B, sig, vir, vib, wet;
+    int32_t idir, iipore;
+    double ee, lsize;
+    double bfr[3], bfb[3];
+    double dt;
+    int32_t collision;    /* PSX_COLL_REF_TRT: collmatrTRT (L1:2858, Lambda=3/16)
+                             PSX_COLL_TRT:     TRT with magic parameter `lambda`
+                             PSX_COLL_MRT:     relaxation vectors Sr, Sb as given */
+    double lambda;
+    double Sr[19], Sb[19];
+    int32_t precision;    /* PSX_PREC_F64: FP64 populations, bit-identical to L1
+                             PSX_PREC_F32: FP32 storage (total deviation from
+                             tr*rr0 and red fraction), FP64 arithmetic         */
+    int32_t reserved[7];
+} psx_lbm_params;
+
+/* check_state outputs (L1:2898-3296); outm[] is one output.m row (L1:3274). */
+typedef struct {
+    double permr, permb;
+    double Qr[3], Qb[3];
+    double um[3], umr[3], umb[3];
+    double tfr[3], tfb[3];
+    double nr, nb;
+    double tmr, tmb, tm;
+    double Cpr, Cpb;
+    double Re;
+    int32_t isneg, negdens_r, negdens_b, pad_;
+    double outm[12];
+} psx_lbm_diag;
+
+typedef struct psx_lbm psx_lbm;
+
+/* Builds the fluid node list and the propagation tables (replaces
+   L1:1026-1340) and allocates device state.  ind: nx*ny*nz voxel codes. */
+int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, int32_t nz,
+                   const int32_t *ind, psx_lbm **h);
+void psx_lbm_destroy(psx_lbm *h);
+/* counts[0..4] = nl, nps, npb2, npb4, ns (settings_19 sizes, L1:1442-1514) */
+int psx_lbm_counts(psx_lbm *h, int64_t *counts);
+/* Reference-format tables (1-based ids): icor(3,nl), ips(3,nps), ipb2(4,npb2),
+   ipb4(6,npb4).  Any pointer may be NULL. */
+int psx_lbm_tables(psx_lbm *h, int32_t *icor, int32_t *ips, int32_t *ipb2, int32_t *ipb4);
+/* The relaxation vectors in use (after collmatrTRT or the TRT formula). */
+int psx_lbm_rates(psx_lbm *h, double *Sr, double *Sb);
+/* initdistr with icon=0 (L1:2800-2842): Park-Miller all-red/all-blue start. */
+int psx_lbm_init_random(psx_lbm *h, double sat);
+/* State in / out: fr, fb are (19,nl) in the reference's node order; this is
+   the content of iteration2 (L1:3951-3961, read L1:2845-2850). */
+int psx_lbm_set_state(psx_lbm *h, const double *fr, const double *fb);
+int psx_lbm_get_state(psx_lbm *h, double *fr, double *fb);
+int psx_lbm_get_fields(psx_lbm *h, double *rr, double *rb, double *cf);
+/* nsteps time steps of L1:3634-3947 (pre-steps included: the caller adds 2). */
+int psx_lbm_step(psx_lbm *h, int64_t nsteps);
+/* check_state for iteration it1 with itw steps per iteration. */
+int psx_lbm_check_state(psx_lbm *h, int64_t it1, int64_t itw, psx_lbm_diag *out);
+int psx_lbm_sync(psx_lbm *h);
+/* Runs nsteps and times each kernel with HIP events on the handle's stream.
+   ms[0] = total wall ms (event to event), ms[1] = mean collide kernel ms,
+   ms[2] = mean density kernel ms. */
+int psx_lbm_profile(psx_lbm *h, int64_t nsteps, double *ms);
+/* Device bytes per fluid node held by the state (for reporting). */
+int psx_lbm_footprint(psx_lbm *h, int64_t *bytes);
+
+/* =============================================================== LSM ===== */
+/* Load cases: 0 xx, 1 yy, 2 zz, 3 yz, 4 zx, 5 xy.  Case (p,d) imposes the
+   jump 2*gstrain*e_d across the period of axis p: xx is XX_P.f (p=d=x),
+   yz is YZ_P.f (p=y, d=z); yy, zz, zx, xy are the same construction. */
+enum { PSX_XX = 0, PSX_YY = 1, PSX_ZZ = 2, PSX_YZ = 3, PSX_ZX = 4, PSX_XY = 5 };
+
+typedef struct {
+    double alf1, bet1, alf2, bet2; /* springs of component 1 (code 0), 2 (code 3) */
+    double rs, dt, gstrain;        /* XX INPUT_PARAMETERS 8, 9, 13                */
+    int32_t precision;             /* PSX_PREC_F64 or PSX_PREC_F32 (FP32 matvec,
+                                      FP64 residual and refinement)               */
+    int32_t reserved[7];
+} psx_lsm_params;
+
+typedef struct psx_lsm psx_lsm;
+
+/* ind: nx*ny*nz voxel codes (0 = solid 1, 3 = solid 2, else pore); stiff:
+   optional per-voxel stiffness multiplier (NULL = 1).  cube_*: the element
+   tables of the cube_bon/cube_ang/cube_sbond/cube_abond files (NULL = the
+   standard lattice); they must describe the D3Q19 cube lattice. */
+int psx_lsm_create(const psx_lsm_params *p, int32_t nx, int32_t ny, int32_t nz,
+                   const int32_t *ind, const double *stiff, const int32_t *cube_bon,
+                   const int32_t *cube_ang, const double *cube_sbond,
+                   const double *cube_abond, psx_lsm **h);
+void psx_lsm_destroy(psx_lsm *h);
+/* counts[0..3] = solid nodes (periodic grid), solid cubes, nx*ny*nz, reserved */
+int psx_lsm_counts(psx_lsm *h, int64_t *counts);
+/* Displacement field of a load case on the periodic node grid, (3,nx*ny*nz). */
+int psx_lsm_set_u(psx_lsm *h, int32_t loadcase, const double *u);
+int psx_lsm_get_u(psx_lsm *h, int32_t loadcase, double *u);
+/* Spring forces F = -K(u + jump) on the node grid (XX:1404-1500, glued pairs
+   summed as XX:1511-1520 does). */
+int psx_lsm_forces(psx_lsm *h, int32_t loadcase, const double *u, double *f);
+/* Reference solver: nsteps of the damped velocity-Verlet loop (XX:1377-1545)
+   from the reference's initial state (XX:1366-1375) or the current one. */
+int psx_lsm_verlet(psx_lsm *h, int32_t loadcase, int64_t nsteps, int32_t restart);
+/* Equilibrium solve K u = f_jump to ||r||/||b|| <= rtol (preconditioned CG;
+   FP32 inner solves with FP64 refinement when precision = PSX_PREC_F32). */
+int psx_lsm_solve(psx_lsm *h, int32_t loadcase, double rtol, int32_t maxit,
+                  int32_t *iters, double *relres);
+/* calc_stress (XX:1628-1997): sum over solid cubes of the cube stress,
+   asig(3,3) column-major (force component, bond direction) and the
+   normalised moduli asig/(V*2*gstrain/L_p). */
+int psx_lsm_stress(psx_lsm *h, int32_t loadcase, double *asig, double *moduli);
+/* Solves the six load cases and returns the 6x6 Voigt stiffness C (row =
+   stress component xx,yy,zz,yz,zx,xy; column = load case) and the Voigt
+   bulk and shear moduli. */
+int psx_lsm_moduli(psx_lsm *h, double rtol, int32_t maxit, double *C, double *K, double *G);
+/* Adjoint of (K, G) after psx_lsm_moduli: wK, wG weight the two moduli;
+   dab[4] = d/d(alf1, bet1, alf2, bet2); dstiff (nx*ny*nz, may be NULL) =
+   d/d(per-voxel stiffness multiplier).  No counterpart in the reference. */
+int psx_lsm_adjoint(psx_lsm *h, double wK, double wG, double rtol, int32_t maxit,
+                    double *dab, double *dstiff);
+/* Matvec timing: mean ms of the FP64 and FP32 operator kernels over n calls. */
+int psx_lsm_profile(psx_lsm *h, int32_t n, double *ms);
+
+/* Batched moduli: nvol sub-volumes, each with its own dims and voxel codes
+   (concatenated), all six load cases solved together in persistent launches.
+   C_out: nvol*36 (column-major 6x6 each); KG_out: nvol*2. */
+int psx_lsm_batch_moduli(const psx_lsm_params *p, int32_t nvol, const int32_t *dims,
+                         const int32_t *ind_concat, double rtol, int32_t maxit,
+                         double *C_out, double *KG_out, double *ms_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1459 @@
+// Lattice spring model (LSM) for gfx950: matrix-free stiffness operator,
+// damped-Verlet replica of the reference solver, batched preconditioned CG
+// (FP64, or FP32 inner solves with FP64 refinement), cube stress / moduli and
+// the adjoint of the moduli.
+//
+// Replaces LSM2S/Cxxxx/XX_P.f ("XX") and LSM2S/Cyzyz/YZ_P.f ("YZ"): the
+// bond/angle tables (XX:626-1262), the force loops (XX:1404-1500), the
+// Verlet loop with glued boundary pairs (XX:1366-1545) and calc_stress
+// (XX:1628-1997).
+//
+// Geometry.  A volume of nx*ny*nz voxels has one node per voxel corner on a
+// periodic nx*ny*nz node grid (node (x,y,z) is the lower corner of voxel
+// (x,y,z)).  XX pads the load axis by two layers and glues the node pairs
+// across it (XX:521, 1366-1375, 1511-1520); that is the same as a periodic
+// axis whose displacement jumps by J = 2*gstrain*e_d on crossing, which is
+// how every load case is represented here: crossing axis a forward adds
+// J[a] to the neighbour's displacement.  The six load cases of one volume
+// therefore share one operator and differ only in J.
+//
+// Operator.  The stored energy is a sum over solid voxel cubes of
+// ka*E_A + kb*E_B (ka = s*alpha_c, kb = s*beta_c for the voxel's component
+// c and stiffness multiplier s); E_A, E_B are the exact 24x24 element
+// matrices of lsm_element.h.  The force on a node gathers its 8 adjacent
+// cubes (node_force, 648 non-zero constants), so the operator needs per
+// node only the 27-node displacement neighbourhood and 8 voxel scales:
+// no atomics, no bond or angle tables in HBM.
+//
+// Layout in HBM.  Vectors are SoA [3][Ntot] per right-hand side, with all
+// volumes of a batch concatenated (volume v at node offset off[v]); voxel
+// scales ka, kb are [Ntot].  Work is dealt as tiles of 256 nodes of one
+// volume; a launch covers every tile of every volume and every active
+// system (volume, load case), so one launch serves the whole batch.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+#include "psx_common.h"
+#include "lsm_element.h"
+
+namespace {
+
+constexpr int TILE = 256;
+constexpr int MAXR = 6;  // right-hand sides per volume (load cases)
+
+struct VolDesc {
+    int32_t nx, ny, nz, pad_;
+    int64_t off;      // node / voxel offset of the volume
+    int32_t tile0;    // first tile of the volume
+    int32_t ntile;
+};
+
+struct Jump {
+    double J[MAXR][3][3];  // [rhs][crossed axis][component]
+};
+
+// per-system scalars (system s = volume*R + rhs)
+struct SysState {
+    double rz, pq, alpha, beta, rr, bb, scale, pad_;
+    int32_t active, iters, conv, pad2_;
+};
+
+__device__ __forceinline__ int wrapm(int t, int n, int &w) {
+    w = t < 0 ? -1 : (t >= n ? 1 : 0);
+    return t < 0 ? t + n : (t >= n ? t - n : t);
+}
+
+// Gathers the 27-node neighbourhood of node (x,y,z) of one right-hand side.
+template <typename T, bool JUMP>
+__device__ __forceinline__ void gather27(const VolDesc &vd, const T *__restrict__ u, int64_t Ntot, int x, int y,
+                                         int z, const double (&J)[3][3], T (&out)[27][3]) {
+#pragma unroll
+    for (int dz = -1; dz <= 1; dz++) {
+        int wz, zz = wrapm(z + dz, vd.nz, wz);
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++) {
+            int wy, yy = wrapm(y + dy, vd.ny, wy);
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++) {
+                int wx, xx = wrapm(x + dx, vd.nx, wx);
+                int64_t idx = vd.off + xx + (int64_t)vd.nx * (yy + (int64_t)vd.ny * zz);
+                int s = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+#pragma unroll
+                for (int e = 0; e < 3; e++) {
+                    T v = u[(int64_t)e * Ntot + idx];
+                    if (JUMP) v += (T)(wx * J[0][e] + wy * J[1][e] + wz * J[2][e]);
+                    out[s][e] = v;
+                }
+            }
+        }
+    }
+}
+
+// Scales of the 8 voxels around node (x,y,z): ka[p] for voxel node - CORNER[p].
+template <typename T>
+__device__ __forceinline__ bool gather_k(const VolDesc &vd, const T *__restrict__ ka, const T *__restrict__ kb,
+                                         int x, int y, int z, T (&a)[8], T (&b)[8]) {
+    bool any = false;
+#pragma unroll
+    for (int p = 0; p < 8; p++) {
+        int w, xx = wrapm(x - (p & 1), vd.nx, w), yy = wrapm(y - ((p >> 1) & 1), vd.ny, w),
+               zz = wrapm(z - (p >> 2), vd.nz, w);
+        int64_t v = vd.off + xx + (int64_t)vd.nx * (yy + (int64_t)vd.ny * zz);
+        a[p] = ka[v];
+        b[p] = kb[v];
+        any |= (a[p] != T(0)) || (b[p] != T(0));
+    }
+    return any;
+}
+
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T *sh) {
+    v = psx::wave_sum(v);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    T s = T(0);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++) s += sh[k];
+    return s;  // valid in thread 0
+}
+
+__device__ __forceinline__ void tile_node(const VolDesc &vd, int start, int &x, int &y, int &z, bool &ok) {
+    int n = start + (int)threadIdx.x;
+    int64_t N = (int64_t)vd.nx * vd.ny * vd.nz;
+    ok = n < N;
+    if (!ok) n = 0;
+    x = n % vd.nx;
+    int t = n / vd.nx;
+    y = t % vd.ny;
+    z = t / vd.ny;
+}
+
+// ------------------------------------------------------------- kernels ----
+// y = mode ? F(u) = -K(u + jump) : K u ;  optional dot partial p.y per (tile, rhs)
+template <typename T, bool JUMP, bool FORCE, bool DOT>
+__global__ __launch_bounds__(TILE) void k_apply(const VolDesc *__restrict__ vols, const int32_t *__restrict__ tvol,
+                                                const int32_t *__restrict__ tstart, int ntiles, int R,
+                                                const T *__restrict__ ka, const T *__restrict__ kb,
+                                                const T *__restrict__ u, T *__restrict__ y, int64_t Ntot,
+                                                const SysState *__restrict__ sys, const Jump *__restrict__ jp,
+                                                double *__restrict__ partial) {
+    __shared__ double sh[TILE / 64];
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int v = tvol[t];
+        const VolDesc vd = vols[v];
+        int x, yy, z;
+        bool ok;
+        tile_node(vd, tstart[t], x, yy, z, ok);
+        int64_t n = vd.off + x + (int64_t)vd.nx * (yy + (int64_t)vd.ny * z);
+        T a[8], b[8];
+        bool act = ok && gather_k(vd, ka, kb, x, yy, z, a, b);
+        for (int r = 0; r < R; r++) {
+            if (sys && !sys[v * R + r].active) {
+                if (DOT && threadIdx.x == 0) partial[(int64_t)t * R + r] = 0.0;
+                continue;
+            }
+            const T *ur = u + (int64_t)r * 3 * Ntot;
+            T *yr = y + (int64_t)r * 3 * Ntot;
+            T f[3] = {T(0), T(0), T(0)};
+            if (act) {
+                T uu[27][3];
+                double J[3][3];
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) J[i][j] = JUMP ? jp->J[r][i][j] : 0.0;
+                gather27<T, JUMP>(vd, ur, Ntot, x, yy, z, J, uu);
+                lsm_el::node_force<T>(uu, a, b, f);
+            }
+            double d = 0.0;
+            if (ok) {
+#pragma unroll
+                for (int e = 0; e < 3; e++) {
+                    T val = FORCE ? f[e] : -f[e];
+                    yr[(int64_t)e * Ntot + n] = val;
+                    if (DOT) d += (double)ur[(int64_t)e * Ntot + n] * (double)val;
+                }
+            }
+            if (DOT) {
+                double s = block_sum(d, sh);
+                if (threadIdx.x == 0) partial[(int64_t)t * R + r] = s;
+            }
+        }
+    }
+}
+
+// Block-Jacobi preconditioner: inverse of each node's 3x3 diagonal block
+// (symmetric, stored as 6 values xx,yy,zz,xy,xz,yz); zero for nodes without
+// springs.
+template <typename T>
+__global__ __launch_bounds__(TILE) void k_diag_inv(const VolDesc *__restrict__ vols, const int32_t *__restrict__ tvol,
+                                                   const int32_t *__restrict__ tstart, int ntiles,
+                                                   const double *__restrict__ ka, const double *__restrict__ kb,
+                                                   T *__restrict__ minv, int64_t Ntot) {
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const VolDesc vd = vols[tvol[t]];
+        int x, y, z;
+        bool ok;
+        tile_node(vd, tstart[t], x, y, z, ok);
+        if (!ok) continue;
+        int64_t n = vd.off + x + (int64_t)vd.nx * (y + (int64_t)vd.ny * z);
+        double a[8], b[8], D[3][3];
+        bool act = gather_k(vd, ka, kb, x, y, z, a, b);
+        double m[6] = {0, 0, 0, 0, 0, 0};
+        if (act) {
+            lsm_el::node_diag<double>(a, b, D);
+            double c00 = D[1][1] * D[2][2] - D[1][2] * D[2][1];
+            double c01 = D[1][2] * D[2][0] - D[1][0] * D[2][2];
+            double c02 = D[1][0] * D[2][1] - D[1][1] * D[2][0];
+            double det = D[0][0] * c00 + D[0][1] * c01 + D[0][2] * c02;
+            if (det > 0.0) {
+                double id = 1.0 / det;
+                m[0] = c00 * id;
+                m[1] = (D[0][0] * D[2][2] - D[0][2] * D[2][0]) * id;
+                m[2] = (D[0][0] * D[1][1] - D[0][1] * D[1][0]) * id;
+                m[3] = c01 * id;  // (1,0) = (0,1) by symmetry
+                m[4] = c02 * id;
+                m[5] = (D[0][2] * D[1][0] - D[0][0] * D[1][2]) * id;
+            } else {  // degenerate block (a node touching only edge springs): diagonal scaling
+                for (int e = 0; e < 3; e++) m[e] = D[e][e] > 0.0 ? 1.0 / D[e][e] : 0.0;
+            }
+        }
+        for (int k = 0; k < 6; k++) minv[(int64_t)k * Ntot + n] = (T)m[k];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void precond(const T *__restrict__ minv, int64_t Ntot, int64_t n, const T (&r)[3], T (&z)[3]) {
+    T m0 = minv[n], m1 = minv[Ntot + n], m2 = minv[2 * Ntot + n], m3 = minv[3 * Ntot + n],
+      m4 = minv[4 * Ntot + n], m5 = minv[5 * Ntot + n];
+    z[0] = m0 * r[0] + m3 * r[1] + m4 * r[2];
+    z[1] = m3 * r[0] + m1 * r[1] + m5 * r[2];
+    z[2] = m4 * r[0] + m5 * r[1] + m2 * r[2];
+}
+
+// Per-system reduction of (tile, rhs) partials; mode 0: alpha = rz/pq,
+// mode 1: beta = rz'/rz with convergence test, mode 2: init (rz, rr, bb).
+__global__ __launch_bounds__(256) void k_sys_reduce(const VolDesc *__restrict__ vols, int R, int mode,
+                                                    const double *__restrict__ p0, const double *__restrict__ p1,
+                                                    SysState *__restrict__ sys, double tol2, int maxit) {
+    const int s = blockIdx.x;
+    const int v = s / R, r = s % R;
+    const VolDesc vd = vols[v];
+    __shared__ double sh[4];
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < vd.ntile; k += blockDim.x) {
+        int64_t t = vd.tile0 + k;
+        a += p0[t * R + r];
+        if (p1) b += p1[t * R + r];
+    }
+    double sa = block_sum(a, sh);
+    double sb = p1 ? block_sum(b, sh) : 0.0;
+    if (threadIdx.x != 0) return;
+    SysState &st = sys[s];
+    if (mode == 2) {  // p0 = r.z, p1 = r.r  at start
+        st.rz = sa;
+        st.rr = sb;
+        st.bb = sb;
+        st.iters = 0;
+        st.conv = sb <= 0.0;
+        st.active = !st.conv;
+        return;
+    }
+    if (!st.active) return;
+    if (mode == 0) {
+        st.pq = sa;
+        st.alpha = sa != 0.0 ? st.rz / sa : 0.0;
+    } else {
+        st.beta = st.rz != 0.0 ? sa / st.rz : 0.0;
+        st.rz = sa;
+        st.rr = sb;
+        st.iters += 1;
+        if (sb <= tol2 * st.bb) { st.conv = 1; st.active = 0; }
+        else if (st.iters >= maxit || !(sb == sb)) st.active = 0;
+    }
+}
+
+// x += alpha p ; r -= alpha q ; z = M^-1 r ; partials r.z, r.r
+template <typename T>
+__global__ __launch_bounds__(TILE) void k_cg_update(const VolDesc *__restrict__ vols, const int32_t *__restrict__ tvol,
+                                                    const int32_t *__restrict__ tstart, int ntiles, int R,
+                                                    const SysState *__restrict__ sys, T *__restrict__ X,
+                                                    T *__restrict__ Rv, T *__restrict__ Z, const T *__restrict__ P,
+                                                    const T *__restrict__ Qv, const T *__restrict__ minv, int64_t Ntot,
+                                                    double *__restrict__ prz, double *__restrict__ prr, int init) {
+    __shared__ double sh[TILE / 64];
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int v = tvol[t];
+        const VolDesc vd = vols[v];
+        int x, y, z;
+        bool ok;
+        tile_node(vd, tstart[t], x, y, z, ok);
+        int64_t n = vd.off + x + (int64_t)vd.nx * (y + (int64_t)vd.ny * z);
+        for (int r = 0; r < R; r++) {
+            const SysState st = sys[v * R + r];
+            if (!init && !st.active) {
+                if (threadIdx.x == 0) { prz[(int64_t)t * R + r] = 0.0; prr[(int64_t)t * R + r] = 0.0; }
+                continue;
+            }
+            int64_t base = (int64_t)r * 3 * Ntot;
+            double drz = 0.0, drr = 0.0;
+            if (ok) {
+                T rv[3], zv[3];
+                T al = (T)st.alpha;
+#pragma unroll
+                for (int e = 0; e < 3; e++) {
+                    int64_t k = base + (int64_t)e * Ntot + n;
+                    if (!init) {
+                        X[k] += al * P[k];
+                        Rv[k] -= al * Qv[k];
+                    }
+                    rv[e] = Rv[k];
+                }
+                precond(minv, Ntot, n, rv, zv);
+#pragma unroll
+                for (int e = 0; e < 3; e++) {
+                    int64_t k = base + (int64_t)e * Ntot + n;
+                    Z[k] = zv[e];
+                    drz += (double)rv[e] * (double)zv[e];
+                    drr += (double)rv[e] * (double)rv[e];
+                }
+            }
+            double s1 = block_sum(drz, sh);
+            double s2 = block_sum(drr, sh);
+            if (threadIdx.x == 0) { prz[(int64_t)t * R + r] = s1; prr[(int64_t)t * R + r] = s2; }
+        }
+    }
+}
+
+// p = z + beta p   (init: p = z)
+template <typename T>
+__global__ __launch_bounds__(TILE) void k_cg_pupdate(const VolDesc *__restrict__ vols, const int32_t *__restrict__ tvol,
+                                                     const int32_t *__restrict__ tstart, int ntiles, int R,
+                                                     const SysState *__restrict__ sys, T *__restrict__ P,
+                                                     const T *__restrict__ Z, int64_t Ntot, int init) {
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int v = tvol[t];
+        const VolDesc vd = vols[v];
+        int x, y, z;
+        bool ok;
+        tile_node(vd, tstart[t], x, y, z, ok);
+        if (!ok) continue;
+        int64_t n = vd.off + x + (int64_t)vd.nx * (y + (int64_t)vd.ny * z);
+        for (int r = 0; r < R; r++) {
+            const SysState st = sys[v * R + r];
+            if (!init && !st.active) continue;
+            T be = init ? T(0) : (T)st.beta;
+#pragma unroll
+            for (int e = 0; e < 3; e++) {
+                int64_t k = (int64_t)r * 3 * Ntot + (int64_t)e * Ntot + n;
+                P[k] = Z[k] + be * P[k];
+            }
+        }
+    }
+}
+
+// r64 = b - K x64 for the outer refinement: Q holds K x64 (from k_apply);
+// writes r64, the FP32 scaled copy r32 = r64/scale, zeroes d32 and partial r.r.
+__global__ __launch_bounds__(TILE) void k_refine_residual(
+    const VolDesc *__restrict__ vols, const int32_t *__restrict__ tvol, const int32_t *__restrict__ tstart,
+    int ntiles, int R, const double *__restrict__ B, const double *__restrict__ KX, double *__restrict__ R64,
+    int64_t Ntot, double *__restrict__ prr) {
+    __shared__ double sh[TILE / 64];
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const VolDesc vd = vols[tvol[t]];
+        int x, y, z;
+        bool ok;
+        tile_node(vd, tstart[t], x, y, z, ok);
+        int64_t n = vd.off + x + (int64_t)vd.nx * (y + (int64_t)vd.ny * z);
+        for (int r = 0; r < R; r++) {
+            double d = 0.0;
+            if (ok)
+                for (int e = 0; e < 3; e++) {
+                    int64_t k = (int64_t)r * 3 * Ntot + (int64_t)e * Ntot + n;
+                    double v = B[k] - KX[k];
+                    R64[k] = v;
+                    d += v * v;
+                }
+            double s = block_sum(d, sh);
+            if (threadIdx.x == 0) prr[(int64_t)t * R + r] = s;
+        }
+    }
+}
+
+template <typename T>
+__global__ void k_zero(T *p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = T(0);
+}
+
+template <typename T>
+__global__ void k_copy(T *__restrict__ d, const T *__restrict__ s, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = s[i];
+}
+
+// Scales the FP32 residual of each system by 1/||r|| (tracked in SysState.scale)
+__global__ __launch_bounds__(TILE) void k_scale_sys(const VolDesc *__restrict__ vols, const int32_t *__restrict__ tvol,
+                                                    const int32_t *__restrict__ tstart, int ntiles, int R,
+                                                    const SysState *__restrict__ outer, float *__restrict__ R32,
+                                                    float *__restrict__ X32, const double *__restrict__ R64, int64_t Ntot) {
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int v = tvol[t];
+        const VolDesc vd = vols[v];
+        int x, y, z;
+        bool ok;
+        tile_node(vd, tstart[t], x, y, z, ok);
+        if (!ok) continue;
+        int64_t n = vd.off + x + (int64_t)vd.nx * (y + (int64_t)vd.ny * z);
+        for (int r = 0; r < R; r++) {
+            double sc = outer[v * R + r].scale;
+            double is = sc > 0.0 ? 1.0 / sc : 0.0;
+            for (int e = 0; e < 3; e++) {
+                int64_t k = (int64_t)r * 3 * Ntot + (int64_t)e * Ntot + n;
+                R32[k] = (float)(R64[k] * is);
+                X32[k] = 0.0f;
+            }
+        }
+    }
+}
+
+// x64 += scale * x32 per system
+__global__ __launch_bounds__(TILE) void k_unscale_add(const VolDesc *__restrict__ vols, const int32_t *__restrict__ tvol,
+                                                      const int32_t *__restrict__ tstart, int ntiles, int R,
+                                                      const SysState *__restrict__ outer, double *__restrict__ X64,
+                                                      const float *__restrict__ X32, int64_t Ntot) {
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int v = tvol[t];
+        const VolDesc vd = vols[v];
+        int x, y, z;
+        bool ok;
+        tile_node(vd, tstart[t], x, y, z, ok);
+        if (!ok) continue;
+        int64_t n = vd.off + x + (int64_t)vd.nx * (y + (int64_t)vd.ny * z);
+        for (int r = 0; r < R; r++) {
+            const SysState st = outer[v * R + r];
+            if (!st.active) continue;
+            for (int e = 0; e < 3; e++) {
+                int64_t k = (int64_t)r * 3 * Ntot + (int64_t)e * Ntot + n;
+                X64[k] += st.scale * (double)X32[k];
+            }
+        }
+    }
+}
+
+// Outer refinement bookkeeping: rr = ||b - K x||^2 -> converged / scale
+__global__ __launch_bounds__(256) void k_outer_reduce(const VolDesc *__restrict__ vols, int R,
+                                                      const double *__restrict__ prr, SysState *__restrict__ outer,
+                                                      double tol2, int first) {
+    const int s = blockIdx.x;
+    const int v = s / R, r = s % R;
+    const VolDesc vd = vols[v];
+    __shared__ double sh[4];
+    double a = 0.0;
+    for (int k = threadIdx.x; k < vd.ntile; k += blockDim.x) a += prr[(int64_t)(vd.tile0 + k) * R + r];
+    double sa = block_sum(a, sh);
+    if (threadIdx.x != 0) return;
+    SysState &st = outer[s];
+    if (first) { st.bb = sa; st.iters = 0; st.conv = 0; }
+    st.rr = sa;
+    st.scale = sqrt(sa);
+    if (sa <= tol2 * st.bb) { st.conv = 1; st.active = 0; }
+    else st.active = 1;
+}
+
+// Inner FP32 CG systems follow the outer activity.
+__global__ void k_inner_from_outer(int S, const SysState *__restrict__ outer, SysState *__restrict__ inner) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S) inner[s].active = outer[s].active;
+}
+
+// ---------------------------------------------------------------- stress --
+// Sum over solid cubes of the cube stress (ka SA + kb SB) u_cube, with the
+// jump applied to wrapped corners; partial[tile][rhs][9].
+__global__ __launch_bounds__(TILE) void k_stress(const VolDesc *__restrict__ vols, const int32_t *__restrict__ tvol,
+                                                 const int32_t *__restrict__ tstart, int ntiles, int R,
+                                                 const double *__restrict__ ka, const double *__restrict__ kb,
+                                                 const double *__restrict__ u, int64_t Ntot, const Jump *__restrict__ jp,
+                                                 double *__restrict__ partial) {
+    __shared__ double sh[TILE / 64];
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const VolDesc vd = vols[tvol[t]];
+        int x, y, z;
+        bool ok;
+        tile_node(vd, tstart[t], x, y, z, ok);  // voxel (x,y,z)
+        int64_t vx = vd.off + x + (int64_t)vd.nx * (y + (int64_t)vd.ny * z);
+        double a = ok ? ka[vx] : 0.0, b = ok ? kb[vx] : 0.0;
+        for (int r = 0; r < R; r++) {
+            double sg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            if (a != 0.0 || b != 0.0) {
+                double uc[24];
+#pragma unroll
+                for (int p = 0; p < 8; p++) {
+                    int wx, wy, wz;
+                    int xx = wrapm(x + (p & 1), vd.nx, wx), yy = wrapm(y + ((p >> 1) & 1), vd.ny, wy),
+                        zz = wrapm(z + (p >> 2), vd.nz, wz);
+                    int64_t n = vd.off + xx + (int64_t)vd.nx * (yy + (int64_t)vd.ny * zz);
+#pragma unroll
+                    for (int e = 0; e < 3; e++)
+                        uc[3 * p + e] = u[(int64_t)r * 3 * Ntot + (int64_t)e * Ntot + n] +
+                                        wx * jp->J[r][0][e] + wy * jp->J[r][1][e] + wz * jp->J[r][2][e];
+                }
+#pragma unroll
+                for (int k = 0; k < 9; k++) {
+                    double sa = 0.0, sb = 0.0;
+#pragma unroll
+                    for (int c = 0; c < 24; c++) {
+                        if (lsm_el::SA[k][c] != 0.0) sa += lsm_el::SA[k][c] * uc[c];
+                        if (lsm_el::SB[k][c] != 0.0) sb += lsm_el::SB[k][c] * uc[c];
+                    }
+                    sg[k] = a * sa + b * sb;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                double s = block_sum(sg[k], sh);
+                if (threadIdx.x == 0) partial[((int64_t)t * R + r) * 9 + k] = s;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_stress_reduce(const VolDesc *__restrict__ vols, int R,
+                                                       const double *__restrict__ partial, double *__restrict__ out) {
+    const int s = blockIdx.x / 9, k = blockIdx.x % 9;
+    const int v = s / R, r = s % R;
+    const VolDesc vd = vols[v];
+    __shared__ double sh[4];
+    double a = 0.0;
+    for (int i = threadIdx.x; i < vd.ntile; i += blockDim.x) a += partial[((int64_t)(vd.tile0 + i) * R + r) * 9 + k];
+    double sa = block_sum(a, sh);
+    if (threadIdx.x == 0) out[s * 9 + k] = sa;
+}
+
+// ---------------------------------------------------------------- Verlet --
+__global__ __launch_bounds__(TILE) void k_verlet_a(int64_t n3, double *__restrict__ U, double *__restrict__ V,
+                                                   const double *__restrict__ A, double dt) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    double v = V[i] + A[i] * dt * 0.5;  // XX:1388-1392
+    V[i] = v;
+    U[i] = U[i] + v * dt;               // XX:1398-1400
+}
+
+__global__ __launch_bounds__(TILE) void k_verlet_b(int64_t n3, const double *__restrict__ F, double *__restrict__ V,
+                                                   double *__restrict__ A, double rsi, double dt) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    double a = F[i] * rsi;       // XX:1522
+    a = a - 0.5 * V[i];          // XX:1528-1530
+    A[i] = a;
+    V[i] = V[i] + a * dt * 0.5;  // XX:1537-1539
+}
+
+// --------------------------------------------------------------- adjoint --
+// Adjoint right-hand side h_J = sum_I W[I][J] g_I with g_I the gradient of
+// the summed cube stress component I with respect to u (gathered per node).
+__global__ __launch_bounds__(TILE) void k_adj_rhs(const VolDesc *__restrict__ vols, const int32_t *__restrict__ tvol,
+                                                  const int32_t *__restrict__ tstart, int ntiles,
+                                                  const double *__restrict__ ka, const double *__restrict__ kb,
+                                                  const double *__restrict__ W /* [6 loads][9 comps] */,
+                                                  double *__restrict__ H, int64_t Ntot) {
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const VolDesc vd = vols[tvol[t]];
+        int x, y, z;
+        bool ok;
+        tile_node(vd, tstart[t], x, y, z, ok);
+        if (!ok) continue;
+        int64_t n = vd.off + x + (int64_t)vd.nx * (y + (int64_t)vd.ny * z);
+        double a[8], b[8];
+        gather_k(vd, ka, kb, x, y, z, a, b);
+        for (int r = 0; r < MAXR; r++) {
+            double h[3] = {0, 0, 0};
+#pragma unroll
+            for (int p = 0; p < 8; p++) {
+                if (a[p] == 0.0 && b[p] == 0.0) continue;
+#pragma unroll
+                for (int k = 0; k < 9; k++) {
+                    double w = W[r * 9 + k];
+                    if (w == 0.0) continue;
+#pragma unroll
+                    for (int e = 0; e < 3; e++)
+                        h[e] += w * (a[p] * lsm_el::SA[k][3 * p + e] + b[p] * lsm_el::SB[k][3 * p + e]);
+                }
+            }
+            for (int e = 0; e < 3; e++) H[(int64_t)r * 3 * Ntot + (int64_t)e * Ntot + n] = h[e];
+        }
+    }
+}
+
+// Per-voxel adjoint terms: tA = sum_J [ W_J . SA u~_J - lam_J^T KA u~_J ], tB
+// likewise with SB, KB (u~ = u + jump on wrapped corners, lam periodic).
+__global__ __launch_bounds__(TILE) void k_adj_voxel(const VolDesc *__restrict__ vols, const int32_t *__restrict__ tvol,
+                                                    const int32_t *__restrict__ tstart, int ntiles,
+                                                    const double *__restrict__ ka, const double *__restrict__ kb,
+                                                    const double *__restrict__ U, const double *__restrict__ LAM,
+                                                    const double *__restrict__ W, int64_t Ntot, const Jump *__restrict__ jp,
+                                                    double *__restrict__ tA, double *__restrict__ tB) {
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const VolDesc vd = vols[tvol[t]];
+        int x, y, z;
+        bool ok;
+        tile_node(vd, tstart[t], x, y, z, ok);
+        if (!ok) continue;
+        int64_t vx = vd.off + x + (int64_t)vd.nx * (y + (int64_t)vd.ny * z);
+        double sa_tot = 0.0, sb_tot = 0.0;
+        if (ka[vx] != 0.0 || kb[vx] != 0.0) {
+            for (int r = 0; r < MAXR; r++) {
+                double uc[24], lc[24];
+                for (int p = 0; p < 8; p++) {
+                    int wx, wy, wz;
+                    int xx = wrapm(x + (p & 1), vd.nx, wx), yy = wrapm(y + ((p >> 1) & 1), vd.ny, wy),
+                        zz = wrapm(z + (p >> 2), vd.nz, wz);
+                    int64_t n = vd.off + xx + (int64_t)vd.nx * (yy + (int64_t)vd.ny * zz);
+                    for (int e = 0; e < 3; e++) {
+                        int64_t k = (int64_t)r * 3 * Ntot + (int64_t)e * Ntot + n;
+                        uc[3 * p + e] = U[k] + wx * jp->J[r][0][e] + wy * jp->J[r][1][e] + wz * jp->J[r][2][e];
+                        lc[3 * p + e] = LAM[k];
+                    }
+                }
+                double s1 = 0.0, s2 = 0.0;
+                for (int k = 0; k < 9; k++) {
+                    double w = W[r * 9 + k];
+                    if (w == 0.0) continue;
+                    for (int c = 0; c < 24; c++) {
+                        s1 += w * lsm_el::SA[k][c] * uc[c];
+                        s2 += w * lsm_el::SB[k][c] * uc[c];
+                    }
+                }
+                for (int i = 0; i < 24; i++) {
+                    if (lc[i] == 0.0) continue;
+                    double ka_u = 0.0, kb_u = 0.0;
+                    for (int c = 0; c < 24; c++) {
+                        ka_u += lsm_el::KA[i][c] * uc[c];
+                        kb_u += lsm_el::KB[i][c] * uc[c];
+                    }
+                    s1 -= lc[i] * ka_u;
+                    s2 -= lc[i] * kb_u;
+                }
+                sa_tot += s1;
+                sb_tot += s2;
+            }
+        }
+        tA[vx] = sa_tot;
+        tB[vx] = sb_tot;
+    }
+}
+
+}  // namespace
+
+// =================================================================== host ==
+namespace {
+
+struct Batch {
+    std::vector<VolDesc> vols;
+    std::vector<int32_t> tvol, tstart;
+    int64_t Ntot = 0;
+    int ntiles = 0;
+    VolDesc *d_vols = nullptr;
+    int32_t *d_tvol = nullptr, *d_tstart = nullptr;
+    double *d_ka = nullptr, *d_kb = nullptr;
+    float *d_ka32 = nullptr, *d_kb32 = nullptr;
+    double *d_minv = nullptr;
+    float *d_minv32 = nullptr;
+    std::vector<double> ka_h, kb_h;
+    std::vector<uint8_t> comp_h;
+    std::vector<double> stiff_h;
+};
+
+struct Solver {
+    int R = 0;              // rhs per volume
+    int S = 0;              // systems
+    size_t vec = 0;         // elements of one [R][3][Ntot] vector
+    double *X = nullptr, *Rr = nullptr, *Z = nullptr, *P = nullptr, *Qv = nullptr, *B = nullptr;
+    float *X32 = nullptr, *R32 = nullptr, *Z32 = nullptr, *P32 = nullptr, *Q32 = nullptr;
+    SysState *sys = nullptr, *outer = nullptr;
+    double *prz = nullptr, *prr = nullptr, *ppq = nullptr;
+    Jump *d_jump = nullptr;
+    Jump jump_h{};
+};
+
+}  // namespace
+
+struct psx_lsm {
+    psx_lsm_params p{};
+    int nx = 0, ny = 0, nz = 0;
+    hipStream_t stream = nullptr;
+    Batch B;
+    Solver sv;
+    int64_t nsolid_nodes = 0, ncubes = 0;
+    // displacement results of the six load cases, [6][3][N] (FP64)
+    double *d_U = nullptr;
+    bool have_case[MAXR] = {false, false, false, false, false, false};
+    // Verlet state for one load case
+    double *d_vu = nullptr, *d_vv = nullptr, *d_va = nullptr, *d_vf = nullptr;
+    int verlet_case = -1;
+    double C[36] = {0};
+    bool have_moduli = false;
+    int grid = 0;
+};
+
+namespace {
+
+const int CASE_PD[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 2}, {2, 0}, {0, 1}};
+
+int grid_for(int ntiles) {
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
+    }
+    return std::max(1, std::min(ntiles, ncu * 8));
+}
+
+template <typename T>
+int dalloc(T **p, size_t n) {
+    PSX_HIP(hipMalloc((void **)p, sizeof(T) * std::max<size_t>(n, 1)));
+    return PSX_OK;
+}
+
+#define RC(x)                 \
+    do {                      \
+        int rc_ = (x);        \
+        if (rc_) return rc_;  \
+    } while (0)
+
+// Builds the batch description and uploads voxel scales.
+int batch_build(Batch &B, hipStream_t st, const psx_lsm_params &p, int nvol, const int32_t *dims,
+                const int32_t *ind, const double *stiff) {
+    int64_t off = 0;
+    int tile = 0;
+    for (int v = 0; v < nvol; v++) {
+        VolDesc d{};
+        d.nx = dims[3 * v]; d.ny = dims[3 * v + 1]; d.nz = dims[3 * v + 2];
+        if (d.nx < 2 || d.ny < 2 || d.nz < 2) {
+            psx::set_error("LSM volume %d: every dimension must be >= 2", v);
+            return PSX_EINVAL;
+        }
+        int64_t N = (int64_t)d.nx * d.ny * d.nz;
+        if (N >= INT32_MAX) { psx::set_error("LSM volume %d too large", v); return PSX_EINVAL; }
+        d.off = off;
+        d.tile0 = tile;
+        d.ntile = (int)psx::ceil_div(N, TILE);
+        for (int k = 0; k < d.ntile; k++) { B.tvol.push_back(v); B.tstart.push_back(k * TILE); }
+        tile += d.ntile;
+        off += N;
+        B.vols.push_back(d);
+    }
+    B.Ntot = off;
+    B.ntiles = tile;
+    B.ka_h.assign(off, 0.0);
+    B.kb_h.assign(off, 0.0);
+    B.comp_h.assign(off, 0);
+    B.stiff_h.assign(off, 1.0);
+    for (int64_t i = 0; i < off; i++) {
+        int32_t c = ind[i];
+        double s = stiff ? stiff[i] : 1.0;
+        B.stiff_h[i] = s;
+        if (c == 0) { B.comp_h[i] = 1; B.ka_h[i] = s * p.alf1; B.kb_h[i] = s * p.bet1; }
+        else if (c == 3) { B.comp_h[i] = 2; B.ka_h[i] = s * p.alf2; B.kb_h[i] = s * p.bet2; }
+    }
+    std::vector<float> ka32(off), kb32(off);
+    for (int64_t i = 0; i < off; i++) { ka32[i] = (float)B.ka_h[i]; kb32[i] = (float)B.kb_h[i]; }
+    RC(dalloc(&B.d_vols, B.vols.size()));
+    RC(dalloc(&B.d_tvol, B.tvol.size()));
+    RC(dalloc(&B.d_tstart, B.tstart.size()));
+    RC(dalloc(&B.d_ka, off)); RC(dalloc(&B.d_kb, off));
+    RC(dalloc(&B.d_ka32, off)); RC(dalloc(&B.d_kb32, off));
+    RC(dalloc(&B.d_minv, 6 * off)); RC(dalloc(&B.d_minv32, 6 * off));
+    PSX_HIP(hipMemcpyAsync(B.d_vols, B.vols.data(), sizeof(VolDesc) * B.vols.size(), hipMemcpyHostToDevice, st));
+    PSX_HIP(hipMemcpyAsync(B.d_tvol, B.tvol.data(), sizeof(int32_t) * B.tvol.size(), hipMemcpyHostToDevice, st));
+    PSX_HIP(hipMemcpyAsync(B.d_tstart, B.tstart.data(), sizeof(int32_t) * B.tstart.size(), hipMemcpyHostToDevice, st));
+    PSX_HIP(hipMemcpyAsync(B.d_ka, B.ka_h.data(), sizeof(double) * off, hipMemcpyHostToDevice, st));
+    PSX_HIP(hipMemcpyAsync(B.d_kb, B.kb_h.data(), sizeof(double) * off, hipMemcpyHostToDevice, st));
+    PSX_HIP(hipMemcpyAsync(B.d_ka32, ka32.data(), sizeof(float) * off, hipMemcpyHostToDevice, st));
+    PSX_HIP(hipMemcpyAsync(B.d_kb32, kb32.data(), sizeof(float) * off, hipMemcpyHostToDevice, st));
+    int g = grid_for(B.ntiles);
+    hipLaunchKernelGGL(k_diag_inv<double>, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles,
+                       B.d_ka, B.d_kb, B.d_minv, B.Ntot);
+    hipLaunchKernelGGL(k_diag_inv<float>, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles,
+                       B.d_ka, B.d_kb, B.d_minv32, B.Ntot);
+    PSX_HIP(hipGetLastError());
+    PSX_HIP(hipStreamSynchronize(st));
+    return PSX_OK;
+}
+
+void batch_free(Batch &B) {
+    void *bufs[] = {B.d_vols, B.d_tvol, B.d_tstart, B.d_ka, B.d_kb, B.d_ka32, B.d_kb32, B.d_minv, B.d_minv32};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    B = Batch();
+}
+
+int solver_alloc(Solver &S, const Batch &B, int R, bool f32) {
+    S.R = R;
+    S.S = (int)B.vols.size() * R;
+    S.vec = (size_t)R * 3 * B.Ntot;
+    RC(dalloc(&S.X, S.vec)); RC(dalloc(&S.Rr, S.vec)); RC(dalloc(&S.Z, S.vec));
+    RC(dalloc(&S.P, S.vec)); RC(dalloc(&S.Qv, S.vec)); RC(dalloc(&S.B, S.vec));
+    if (f32) {
+        RC(dalloc(&S.X32, S.vec)); RC(dalloc(&S.R32, S.vec)); RC(dalloc(&S.Z32, S.vec));
+        RC(dalloc(&S.P32, S.vec)); RC(dalloc(&S.Q32, S.vec));
+    }
+    RC(dalloc(&S.sys, S.S)); RC(dalloc(&S.outer, S.S));
+    size_t np = (size_t)B.ntiles * R;
+    RC(dalloc(&S.prz, np)); RC(dalloc(&S.prr, np)); RC(dalloc(&S.ppq, np * 9));
+    RC(dalloc(&S.d_jump, 1));
+    return PSX_OK;
+}
+
+void solver_free(Solver &S) {
+    void *bufs[] = {S.X, S.Rr, S.Z, S.P, S.Qv, S.B, S.X32, S.R32, S.Z32, S.P32, S.Q32,
+                    S.sys, S.outer, S.prz, S.prr, S.ppq, S.d_jump};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    S = Solver();
+}
+
+void set_jumps(Solver &S, const int *cases, int R, double gstrain) {
+    std::memset(&S.jump_h, 0, sizeof S.jump_h);
+    for (int r = 0; r < R; r++) {
+        int pa = CASE_PD[cases[r]][0], d = CASE_PD[cases[r]][1];
+        S.jump_h.J[r][pa][d] = 2.0 * gstrain;
+    }
+}
+
+template <typename T, bool JUMP, bool FORCE, bool DOT>
+void launch_apply(hipStream_t st, const Batch &B, int R, const T *ka, const T *kb, const T *u, T *y,
+                  const SysState *sys, const Jump *jp, double *partial) {
+    int g = grid_for(B.ntiles);
+    hipLaunchKernelGGL((k_apply<T, JUMP, FORCE, DOT>), dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol,
+                       B.d_tstart, B.ntiles, R, ka, kb, u, y, B.Ntot, sys, jp, partial);
+}
+
+// rhs b = F(u = 0) = -K(jump) for every system
+int build_rhs(hipStream_t st, const Batch &B, Solver &S) {
+    int64_t n = (int64_t)S.vec;
+    int gb = (int)psx::ceil_div(n, 256);
+    hipLaunchKernelGGL(k_zero<double>, dim3(gb), dim3(256), 0, st, S.Z, n);
+    launch_apply<double, true, true, false>(st, B, S.R, B.d_ka, B.d_kb, S.Z, S.B, nullptr, S.d_jump, nullptr);
+    PSX_HIP(hipGetLastError());
+    return PSX_OK;
+}
+
+// One PCG iteration on all active systems (5 launches, no host sync).
+template <typename T>
+void cg_iteration(hipStream_t st, const Batch &B, Solver &S, SysState *sys, T *X, T *Rv, T *Z, T *P, T *Qv,
+                  const T *ka, const T *kb, const T *minv, double tol2, int maxit) {
+    int g = grid_for(B.ntiles);
+    launch_apply<T, false, false, true>(st, B, S.R, ka, kb, P, Qv, sys, S.d_jump, S.ppq);
+    hipLaunchKernelGGL(k_sys_reduce, dim3(S.S), dim3(256), 0, st, B.d_vols, S.R, 0, S.ppq, (const double *)nullptr,
+                       sys, tol2, maxit);
+    hipLaunchKernelGGL(k_cg_update<T>, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles, S.R,
+                       sys, X, Rv, Z, P, Qv, minv, B.Ntot, S.prz, S.prr, 0);
+    hipLaunchKernelGGL(k_sys_reduce, dim3(S.S), dim3(256), 0, st, B.d_vols, S.R, 1, S.prz, S.prr, sys, tol2, maxit);
+    hipLaunchKernelGGL(k_cg_pupdate<T>, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles, S.R,
+                       sys, P, Z, B.Ntot, 0);
+}
+
+// CG start: x = 0, r = rhs, z = M^-1 r, p = z, scalars.
+template <typename T>
+void cg_start(hipStream_t st, const Batch &B, Solver &S, SysState *sys, T *X, T *Rv, T *Z, T *P, const T *rhs,
+              const T *minv) {
+    int64_t n = (int64_t)S.vec;
+    int gb = (int)psx::ceil_div(n, 256);
+    int g = grid_for(B.ntiles);
+    hipLaunchKernelGGL(k_zero<T>, dim3(gb), dim3(256), 0, st, X, n);
+    if (Rv != rhs) hipLaunchKernelGGL(k_copy<T>, dim3(gb), dim3(256), 0, st, Rv, rhs, n);
+    hipLaunchKernelGGL(k_cg_update<T>, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles, S.R,
+                       sys, X, Rv, Z, (const T *)nullptr, (const T *)nullptr, minv, B.Ntot, S.prz, S.prr, 1);
+    hipLaunchKernelGGL(k_sys_reduce, dim3(S.S), dim3(256), 0, st, B.d_vols, S.R, 2, S.prz, S.prr, sys, 0.0, 0);
+    hipLaunchKernelGGL(k_cg_pupdate<T>, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles, S.R,
+                       sys, P, Z, B.Ntot, 1);
+}
+
+// Runs CG until every system has stopped; host checks every `chunk`
+// iterations.  The chunk of iterations is captured once into a hipGraph
+// and replayed.
+template <typename T>
+int cg_run(hipStream_t st, const Batch &B, Solver &S, SysState *sys, T *X, T *Rv, T *Z, T *P, T *Qv, const T *ka,
+           const T *kb, const T *minv, double tol, int maxit, int chunk, int use_graph) {
+    double tol2 = tol * tol;
+    std::vector<SysState> hs(S.S);
+    hipGraphExec_t exec = nullptr;
+    if (use_graph) {
+        hipGraph_t graph;
+        PSX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < chunk; k++) cg_iteration<T>(st, B, S, sys, X, Rv, Z, P, Qv, ka, kb, minv, tol2, maxit);
+        PSX_HIP(hipStreamEndCapture(st, &graph));
+        PSX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(graph);
+    }
+    int done_iters = 0;
+    while (true) {
+        if (exec) {
+            PSX_HIP(hipGraphLaunch(exec, st));
+        } else {
+            for (int k = 0; k < chunk; k++) cg_iteration<T>(st, B, S, sys, X, Rv, Z, P, Qv, ka, kb, minv, tol2, maxit);
+            PSX_HIP(hipGetLastError());
+        }
+        done_iters += chunk;
+        PSX_HIP(hipMemcpyAsync(hs.data(), sys, sizeof(SysState) * S.S, hipMemcpyDeviceToHost, st));
+        PSX_HIP(hipStreamSynchronize(st));
+        bool any = false;
+        for (auto &s : hs) any |= s.active != 0;
+        if (!any || done_iters >= maxit) break;
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    return PSX_OK;
+}
+
+// Solves K x = b for every system of the batch.  FP64: plain PCG.  FP32:
+// outer FP64 residual r = b - K x, inner FP32 PCG on r/||r|| to `inner_tol`,
+// x += ||r|| d, until ||r|| <= tol ||b||.
+int solve_batch(hipStream_t st, const Batch &B, Solver &S, bool f32, double tol, int maxit, int use_graph,
+                std::vector<SysState> *out_state, int *out_iters) {
+    const int chunk = 16;
+    int total_iters = 0;
+    if (!f32) {
+        cg_start<double>(st, B, S, S.sys, S.X, S.Rr, S.Z, S.P, S.B, B.d_minv);
+        PSX_HIP(hipGetLastError());
+        RC(cg_run<double>(st, B, S, S.sys, S.X, S.Rr, S.Z, S.P, S.Qv, B.d_ka, B.d_kb, B.d_minv, tol, maxit, chunk,
+                          use_graph));
+        if (out_state) {
+            out_state->resize(S.S);
+            PSX_HIP(hipMemcpy(out_state->data(), S.sys, sizeof(SysState) * S.S, hipMemcpyDeviceToHost));
+        }
+        if (out_iters) {
+            std::vector<SysState> hs(S.S);
+            PSX_HIP(hipMemcpy(hs.data(), S.sys, sizeof(SysState) * S.S, hipMemcpyDeviceToHost));
+            int mx = 0;
+            for (auto &s : hs) mx = std::max(mx, s.iters);
+            *out_iters = mx;
+        }
+        return PSX_OK;
+    }
+    int64_t n = (int64_t)S.vec;
+    int gb = (int)psx::ceil_div(n, 256);
+    int g = grid_for(B.ntiles);
+    hipLaunchKernelGGL(k_zero<double>, dim3(gb), dim3(256), 0, st, S.X, n);
+    const double inner_tol = 1e-5;
+    std::vector<SysState> ho(S.S);
+    for (int outer = 0; outer < 12; outer++) {
+        // r = b - K x
+        launch_apply<double, false, false, false>(st, B, S.R, B.d_ka, B.d_kb, S.X, S.Qv, nullptr, S.d_jump, nullptr);
+        hipLaunchKernelGGL(k_refine_residual, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles,
+                           S.R, S.B, S.Qv, S.Rr, B.Ntot, S.prr);
+        hipLaunchKernelGGL(k_outer_reduce, dim3(S.S), dim3(256), 0, st, B.d_vols, S.R, S.prr, S.outer, tol * tol,
+                           outer == 0);
+        PSX_HIP(hipGetLastError());
+        PSX_HIP(hipMemcpyAsync(ho.data(), S.outer, sizeof(SysState) * S.S, hipMemcpyDeviceToHost, st));
+        PSX_HIP(hipStreamSynchronize(st));
+        bool any = false;
+        for (auto &s : ho) any |= s.active != 0;
+        if (!any || total_iters >= maxit) break;
+        // inner FP32 solve on the scaled residual
+        hipLaunchKernelGGL(k_scale_sys, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles, S.R,
+                           S.outer, S.R32, S.X32, S.Rr, B.Ntot);
+        cg_start<float>(st, B, S, S.sys, S.X32, S.R32, S.Z32, S.P32, S.R32, B.d_minv32);
+        hipLaunchKernelGGL(k_inner_from_outer, dim3((unsigned)psx::ceil_div(S.S, 64)), dim3(64), 0, st, S.S, S.outer,
+                           S.sys);
+        PSX_HIP(hipGetLastError());
+        RC(cg_run<float>(st, B, S, S.sys, S.X32, S.R32, S.Z32, S.P32, S.Q32, B.d_ka32, B.d_kb32, B.d_minv32,
+                         inner_tol, maxit - total_iters, chunk, use_graph));
+        std::vector<SysState> hs(S.S);
+        PSX_HIP(hipMemcpy(hs.data(), S.sys, sizeof(SysState) * S.S, hipMemcpyDeviceToHost));
+        int mx = 0;
+        for (auto &s : hs) mx = std::max(mx, s.iters);
+        total_iters += mx;
+        hipLaunchKernelGGL(k_unscale_add, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles, S.R,
+                           S.outer, S.X, S.X32, B.Ntot);
+        PSX_HIP(hipGetLastError());
+    }
+    PSX_HIP(hipMemcpy(ho.data(), S.outer, sizeof(SysState) * S.S, hipMemcpyDeviceToHost));
+    for (auto &s : ho) s.iters = total_iters;
+    if (out_state) *out_state = ho;
+    if (out_iters) *out_iters = total_iters;
+    return PSX_OK;
+}
+
+// Summed cube stress for every system into host out[S][9].
+int stress_batch(hipStream_t st, const Batch &B, Solver &S, const double *U, std::vector<double> &out) {
+    int g = grid_for(B.ntiles);
+    hipLaunchKernelGGL(k_stress, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles, S.R, B.d_ka,
+                       B.d_kb, U, B.Ntot, S.d_jump, S.ppq);
+    double *d_out = S.prz;  // reuse: needs S*9 doubles <= ntiles*R
+    bool tmp = (size_t)S.S * 9 > (size_t)B.ntiles * S.R;
+    if (tmp) RC(dalloc(&d_out, (size_t)S.S * 9));
+    hipLaunchKernelGGL(k_stress_reduce, dim3(S.S * 9), dim3(256), 0, st, B.d_vols, S.R, S.ppq, d_out);
+    PSX_HIP(hipGetLastError());
+    out.resize((size_t)S.S * 9);
+    PSX_HIP(hipMemcpyAsync(out.data(), d_out, sizeof(double) * S.S * 9, hipMemcpyDeviceToHost, st));
+    PSX_HIP(hipStreamSynchronize(st));
+    if (tmp) (void)hipFree(d_out);
+    return PSX_OK;
+}
+
+// stress component of asig (d,c) used for Voigt row I (xx,yy,zz,yz,zx,xy);
+// shear rows use the reference's YZ convention asigmap(2,3) (YZ:1936).
+const int VOIGT_DC[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 2}, {2, 0}, {0, 1}};
+
+void moduli_from_stress(const double *asig /*[6][9]*/, const VolDesc &vd, double g, double *C, double *K,
+                        double *G) {
+    double V = (double)vd.nx * vd.ny * vd.nz;
+    int dims[3] = {vd.nx, vd.ny, vd.nz};
+    for (int J = 0; J < 6; J++) {
+        double eps = 2.0 * g / (double)dims[CASE_PD[J][0]];
+        for (int I = 0; I < 6; I++) {
+            int d = VOIGT_DC[I][0], c = VOIGT_DC[I][1];
+            C[I + 6 * J] = asig[J * 9 + d + 3 * c] / (V * eps);
+        }
+    }
+    double a = C[0] + C[7] + C[14];
+    double b = C[6] + C[12] + C[13] + C[1] + C[2] + C[8];  // off-diagonal normal terms (both halves)
+    double c44 = C[21] + C[28] + C[35];
+    *K = (a + b) / 9.0;
+    *G = (a - 0.5 * b + 3.0 * c44) / 15.0;
+}
+
+}  // namespace
+
+// ============================================================== C-ABI ======
+extern "C" int psx_lsm_create(const psx_lsm_params *p, int32_t nx, int32_t ny, int32_t nz, const int32_t *ind,
+                              const double *stiff, const int32_t *cube_bon, const int32_t *cube_ang,
+                              const double *cube_sbond, const double *cube_abond, psx_lsm **out) {
+    PSX_CHECK_ARG(p && ind && out, "psx_lsm_create: NULL argument");
+    PSX_CHECK_ARG(nx >= 2 && ny >= 2 && nz >= 2, "psx_lsm_create: every dimension must be >= 2");
+    PSX_CHECK_ARG(p->precision == PSX_PREC_F64 || p->precision == PSX_PREC_F32, "psx_lsm_create: unknown precision");
+    PSX_CHECK_ARG(p->rs > 0.0 && p->dt > 0.0, "psx_lsm_create: rs and dt must be > 0");
+    *out = nullptr;
+    // The element matrices are derived from the standard cube lattice; a
+    // caller-supplied table must be that lattice (same bonds, angles, weights).
+    if (cube_bon || cube_ang || cube_sbond || cube_abond) {
+        PSX_CHECK_ARG(cube_bon && cube_ang && cube_sbond && cube_abond,
+                      "psx_lsm_create: pass all four cube tables or none");
+        static const int CORNER[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0},
+                                         {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+        static const int IB[9][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1},  {1, 1, 0}, {1, -1, 0},
+                                     {1, 0, 1}, {1, 0, -1}, {0, 1, 1}, {0, 1, -1}};
+        for (int l = 0; l < 24; l++) {
+            int p1 = cube_bon[3 * l] - 1, p2 = cube_bon[3 * l + 1] - 1, b = cube_bon[3 * l + 2] - 1;
+            bool okb = p1 >= 0 && p1 < 8 && p2 >= 0 && p2 < 8 && b >= 0 && b < 9;
+            if (okb)
+                for (int d = 0; d < 3; d++) okb &= CORNER[p2][d] - CORNER[p1][d] == IB[b][d];
+            double w = b < 3 ? 0.25 : 0.5;
+            PSX_CHECK_ARG(okb && cube_sbond[l] == w, "psx_lsm_create: cube_bon/cube_sbond is not the D3Q19 cube lattice");
+        }
+        for (int l = 0; l < 72; l++) {
+            int qj = cube_ang[6 * l + 3], qk = cube_ang[6 * l + 4];
+            double w = (qj > 7 && qk > 7) ? 1.0 : 0.5;
+            PSX_CHECK_ARG(cube_abond[l] == w, "psx_lsm_create: cube_abond is not the D3Q19 cube lattice");
+        }
+    }
+    int rc = psx::require_device();
+    if (rc) return rc;
+    psx_lsm *h = nullptr;
+    PSX_GUARD_BEGIN
+    h = new psx_lsm();
+    h->p = *p;
+    h->nx = nx; h->ny = ny; h->nz = nz;
+    PSX_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    int32_t dims[3] = {nx, ny, nz};
+    rc = batch_build(h->B, h->stream, *p, 1, dims, ind, stiff);
+    if (rc) { psx_lsm_destroy(h); return rc; }
+    const int64_t N = h->B.Ntot;
+    for (int64_t i = 0; i < N; i++) h->ncubes += h->B.comp_h[i] != 0;
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) {
+                bool s = false;
+                for (int c = 0; c < 8 && !s; c++) {
+                    int xx = (x - (c & 1) + nx) % nx, yy = (y - ((c >> 1) & 1) + ny) % ny, zz = (z - (c >> 2) + nz) % nz;
+                    s = h->B.comp_h[xx + (int64_t)nx * (yy + (int64_t)ny * zz)] != 0;
+                }
+                h->nsolid_nodes += s;
+            }
+    rc = dalloc(&h->d_U, (size_t)MAXR * 3 * N);
+    if (rc) { psx_lsm_destroy(h); return rc; }
+    *out = h;
+    return PSX_OK;
+    PSX_GUARD_END
+}
+
+extern "C" void psx_lsm_destroy(psx_lsm *h) {
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    batch_free(h->B);
+    solver_free(h->sv);
+    void *bufs[] = {h->d_U, h->d_vu, h->d_vv, h->d_va, h->d_vf};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+extern "C" int psx_lsm_counts(psx_lsm *h, int64_t *c) {
+    PSX_CHECK_ARG(h && c, "psx_lsm_counts: NULL argument");
+    c[0] = h->nsolid_nodes; c[1] = h->ncubes; c[2] = h->B.Ntot; c[3] = 0;
+    return PSX_OK;
+}
+
+namespace {
+int ensure_solver(psx_lsm *h, int R) {
+    if (h->sv.R == R) return PSX_OK;
+    solver_free(h->sv);
+    return solver_alloc(h->sv, h->B, R, true);
+}
+int check_case(int lc) {
+    if (lc < 0 || lc > 5) { psx::set_error("load case must be 0..5"); return PSX_EINVAL; }
+    return PSX_OK;
+}
+}  // namespace
+
+extern "C" int psx_lsm_set_u(psx_lsm *h, int32_t lc, const double *u) {
+    PSX_CHECK_ARG(h && u, "psx_lsm_set_u: NULL argument");
+    RC(check_case(lc));
+    const int64_t N = h->B.Ntot;
+    PSX_HIP(hipMemcpyAsync(h->d_U + (int64_t)lc * 3 * N, u, sizeof(double) * 3 * N, hipMemcpyHostToDevice, h->stream));
+    PSX_HIP(hipStreamSynchronize(h->stream));
+    h->have_case[lc] = true;
+    return PSX_OK;
+}
+
+extern "C" int psx_lsm_get_u(psx_lsm *h, int32_t lc, double *u) {
+    PSX_CHECK_ARG(h && u, "psx_lsm_get_u: NULL argument");
+    RC(check_case(lc));
+    const int64_t N = h->B.Ntot;
+    const double *src = (h->verlet_case == lc && h->d_vu) ? h->d_vu : h->d_U + (int64_t)lc * 3 * N;
+    PSX_HIP(hipMemcpyAsync(u, src, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, h->stream));
+    PSX_HIP(hipStreamSynchronize(h->stream));
+    return PSX_OK;
+}
+
+extern "C" int psx_lsm_forces(psx_lsm *h, int32_t lc, const double *u, double *f) {
+    PSX_CHECK_ARG(h && u && f, "psx_lsm_forces: NULL argument");
+    RC(check_case(lc));
+    PSX_GUARD_BEGIN
+    RC(ensure_solver(h, 1));
+    int cs[1] = {lc};
+    set_jumps(h->sv, cs, 1, h->p.gstrain);
+    const int64_t N = h->B.Ntot;
+    PSX_HIP(hipMemcpyAsync(h->sv.d_jump, &h->sv.jump_h, sizeof(Jump), hipMemcpyHostToDevice, h->stream));
+    PSX_HIP(hipMemcpyAsync(h->sv.Z, u, sizeof(double) * 3 * N, hipMemcpyHostToDevice, h->stream));
+    launch_apply<double, true, true, false>(h->stream, h->B, 1, h->B.d_ka, h->B.d_kb, h->sv.Z, h->sv.Qv, nullptr,
+                                            h->sv.d_jump, nullptr);
+    PSX_HIP(hipGetLastError());
+    PSX_HIP(hipMemcpyAsync(f, h->sv.Qv, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, h->stream));
+    PSX_HIP(hipStreamSynchronize(h->stream));
+    return PSX_OK;
+    PSX_GUARD_END
+}
+
+extern "C" int psx_lsm_verlet(psx_lsm *h, int32_t lc, int64_t nsteps, int32_t restart) {
+    PSX_CHECK_ARG(h && nsteps >= 0, "psx_lsm_verlet: bad argument");
+    RC(check_case(lc));
+    PSX_GUARD_BEGIN
+    RC(ensure_solver(h, 1));
+    const int64_t N = h->B.Ntot, n3 = 3 * N;
+    if (!h->d_vu) {
+        RC(dalloc(&h->d_vu, n3)); RC(dalloc(&h->d_vv, n3)); RC(dalloc(&h->d_va, n3)); RC(dalloc(&h->d_vf, n3));
+    }
+    int cs[1] = {lc};
+    set_jumps(h->sv, cs, 1, h->p.gstrain);
+    PSX_HIP(hipMemcpyAsync(h->sv.d_jump, &h->sv.jump_h, sizeof(Jump), hipMemcpyHostToDevice, h->stream));
+    if (restart || h->verlet_case != lc) {
+        // XX:1286-1287 zero state, then the glued pairs start at -+gstrain
+        // (XX:1366-1375).  On the periodic grid the near node of a pair is
+        // node 0 of the load axis; its far twin is the same node plus J.
+        // Nodes solid on one side only are not glued by the reference; the
+        // initial offset reproduces the reference's zero start there.
+        std::vector<double> u0(n3, 0.0);
+        int pa = CASE_PD[lc][0], d = CASE_PD[lc][1];
+        int dims[3] = {h->nx, h->ny, h->nz};
+        int o1 = (pa + 1) % 3, o2 = (pa + 2) % 3;
+        const double g = h->p.gstrain;
+        auto solid_vox = [&](int c0, int c1, int c2) {
+            int cc[3];
+            cc[pa] = c0; cc[o1] = (c1 + dims[o1]) % dims[o1]; cc[o2] = (c2 + dims[o2]) % dims[o2];
+            return h->B.comp_h[cc[0] + (int64_t)h->nx * (cc[1] + (int64_t)h->ny * cc[2])] != 0;
+        };
+        for (int s = 0; s < dims[o1]; s++)
+            for (int t = 0; t < dims[o2]; t++) {
+                bool nearS = false, farS = false;
+                for (int a = 0; a < 2; a++)
+                    for (int b = 0; b < 2; b++) {
+                        nearS |= solid_vox(0, s - a, t - b);
+                        farS |= solid_vox(dims[pa] - 1, s - a, t - b);
+                    }
+                int cc[3];
+                cc[pa] = 0; cc[o1] = s; cc[o2] = t;
+                int64_t n = cc[0] + (int64_t)h->nx * (cc[1] + (int64_t)h->ny * cc[2]);
+                double val = (nearS && farS) ? -g : ((!nearS && farS) ? -2.0 * g : 0.0);
+                u0[(int64_t)d * N + n] = val;
+            }
+        PSX_HIP(hipMemcpyAsync(h->d_vu, u0.data(), sizeof(double) * n3, hipMemcpyHostToDevice, h->stream));
+        int gb = (int)psx::ceil_div(n3, 256);
+        hipLaunchKernelGGL(k_zero<double>, dim3(gb), dim3(256), 0, h->stream, h->d_vv, n3);
+        hipLaunchKernelGGL(k_zero<double>, dim3(gb), dim3(256), 0, h->stream, h->d_va, n3);
+        PSX_HIP(hipGetLastError());
+        h->verlet_case = lc;
+    }
+    const double dt = h->p.dt, rsi = 1.0 / h->p.rs;
+    int gb = (int)psx::ceil_div(n3, 256);
+    for (int64_t s = 0; s < nsteps; s++) {
+        hipLaunchKernelGGL(k_verlet_a, dim3(gb), dim3(256), 0, h->stream, n3, h->d_vu, h->d_vv, h->d_va, dt);
+        launch_apply<double, true, true, false>(h->stream, h->B, 1, h->B.d_ka, h->B.d_kb, h->d_vu, h->d_vf, nullptr,
+                                                h->sv.d_jump, nullptr);
+        hipLaunchKernelGGL(k_verlet_b, dim3(gb), dim3(256), 0, h->stream, n3, h->d_vf, h->d_vv, h->d_va, rsi, dt);
+    }
+    PSX_HIP(hipGetLastError());
+    PSX_HIP(hipMemcpyAsync(h->d_U + (int64_t)lc * 3 * N, h->d_vu, sizeof(double) * n3, hipMemcpyDeviceToDevice, h->stream));
+    PSX_HIP(hipStreamSynchronize(h->stream));
+    h->have_case[lc] = true;
+    return PSX_OK;
+    PSX_GUARD_END
+}
+
+extern "C" int psx_lsm_solve(psx_lsm *h, int32_t lc, double rtol, int32_t maxit, int32_t *iters, double *relres) {
+    PSX_CHECK_ARG(h && rtol > 0.0 && maxit > 0, "psx_lsm_solve: bad argument");
+    RC(check_case(lc));
+    PSX_GUARD_BEGIN
+    RC(ensure_solver(h, 1));
+    int cs[1] = {lc};
+    set_jumps(h->sv, cs, 1, h->p.gstrain);
+    PSX_HIP(hipMemcpyAsync(h->sv.d_jump, &h->sv.jump_h, sizeof(Jump), hipMemcpyHostToDevice, h->stream));
+    RC(build_rhs(h->stream, h->B, h->sv));
+    std::vector<SysState> st;
+    int it = 0;
+    RC(solve_batch(h->stream, h->B, h->sv, h->p.precision == PSX_PREC_F32, rtol, maxit, 1, &st, &it));
+    const int64_t N = h->B.Ntot;
+    PSX_HIP(hipMemcpyAsync(h->d_U + (int64_t)lc * 3 * N, h->sv.X, sizeof(double) * 3 * N, hipMemcpyDeviceToDevice, h->stream));
+    PSX_HIP(hipStreamSynchronize(h->stream));
+    h->have_case[lc] = true;
+    if (h->verlet_case == lc) h->verlet_case = -1;
+    double rel = st[0].bb > 0 ? std::sqrt(st[0].rr / st[0].bb) : 0.0;
+    if (iters) *iters = it;
+    if (relres) *relres = rel;
+    if (!st[0].conv) {
+        psx::set_error("psx_lsm_solve: relative residual %.3e after %d iterations (tolerance %.1e)", rel, it, rtol);
+        return PSX_ECONV;
+    }
+    return PSX_OK;
+    PSX_GUARD_END
+}
+
+extern "C" int psx_lsm_stress(psx_lsm *h, int32_t lc, double *asig, double *moduli) {
+    PSX_CHECK_ARG(h && asig, "psx_lsm_stress: NULL argument");
+    RC(check_case(lc));
+    PSX_GUARD_BEGIN
+    RC(ensure_solver(h, 1));
+    int cs[1] = {lc};
+    set_jumps(h->sv, cs, 1, h->p.gstrain);
+    PSX_HIP(hipMemcpyAsync(h->sv.d_jump, &h->sv.jump_h, sizeof(Jump), hipMemcpyHostToDevice, h->stream));
+    const int64_t N = h->B.Ntot;
+    const double *U = (h->verlet_case == lc && h->d_vu) ? h->d_vu : h->d_U + (int64_t)lc * 3 * N;
+    std::vector<double> out;
+    RC(stress_batch(h->stream, h->B, h->sv, U, out));
+    std::memcpy(asig, out.data(), 9 * sizeof(double));
+    if (moduli) {
+        int dims[3] = {h->nx, h->ny, h->nz};
+        double V = (double)h->nx * h->ny * h->nz;
+        double den = V * (h->p.gstrain * 2.0) / (double)dims[CASE_PD[lc][0]];
+        for (int k = 0; k < 9; k++) moduli[k] = asig[k] / den;
+    }
+    return PSX_OK;
+    PSX_GUARD_END
+}
+
+extern "C" int psx_lsm_moduli(psx_lsm *h, double rtol, int32_t maxit, double *C, double *K, double *G) {
+    PSX_CHECK_ARG(h && C && K && G && rtol > 0.0 && maxit > 0, "psx_lsm_moduli: bad argument");
+    PSX_GUARD_BEGIN
+    RC(ensure_solver(h, MAXR));
+    int cs[MAXR] = {0, 1, 2, 3, 4, 5};
+    set_jumps(h->sv, cs, MAXR, h->p.gstrain);
+    PSX_HIP(hipMemcpyAsync(h->sv.d_jump, &h->sv.jump_h, sizeof(Jump), hipMemcpyHostToDevice, h->stream));
+    RC(build_rhs(h->stream, h->B, h->sv));
+    std::vector<SysState> st;
+    int it = 0;
+    RC(solve_batch(h->stream, h->B, h->sv, h->p.precision == PSX_PREC_F32, rtol, maxit, 1, &st, &it));
+    const int64_t N = h->B.Ntot;
+    PSX_HIP(hipMemcpyAsync(h->d_U, h->sv.X, sizeof(double) * MAXR * 3 * N, hipMemcpyDeviceToDevice, h->stream));
+    for (int r = 0; r < MAXR; r++) h->have_case[r] = true;
+    h->verlet_case = -1;
+    std::vector<double> asig;
+    RC(stress_batch(h->stream, h->B, h->sv, h->d_U, asig));
+    moduli_from_stress(asig.data(), h->B.vols[0], h->p.gstrain, C, K, G);
+    std::memcpy(h->C, C, sizeof h->C);
+    h->have_moduli = true;
+    for (auto &s : st)
+        if (!s.conv) {
+            psx::set_error("psx_lsm_moduli: a load case did not reach rtol %.1e in %d iterations", rtol, it);
+            return PSX_ECONV;
+        }
+    return PSX_OK;
+    PSX_GUARD_END
+}
+
+extern "C" int psx_lsm_adjoint(psx_lsm *h, double wK, double wG, double rtol, int32_t maxit, double *dab,
+                               double *dstiff) {
+    PSX_CHECK_ARG(h && dab && rtol > 0.0 && maxit > 0, "psx_lsm_adjoint: bad argument");
+    if (!h->have_moduli) {
+        psx::set_error("psx_lsm_adjoint: call psx_lsm_moduli first");
+        return PSX_ESTATE;
+    }
+    PSX_GUARD_BEGIN
+    RC(ensure_solver(h, MAXR));
+    const VolDesc vd = h->B.vols[0];
+    const int64_t N = h->B.Ntot;
+    int dims[3] = {vd.nx, vd.ny, vd.nz};
+    double V = (double)vd.nx * vd.ny * vd.nz;
+    // dM/dC(I,J): K = (sum_{a,b<3} C_ab)/9, G = (tr - (off)/2 + 3 sum C44..66)/15
+    double dMdC[36] = {0};
+    for (int I = 0; I < 6; I++)
+        for (int J = 0; J < 6; J++) {
+            double dk = (I < 3 && J < 3) ? 1.0 / 9.0 : 0.0;
+            double dg = 0.0;
+            if (I < 3 && J < 3) dg = (I == J) ? 1.0 / 15.0 : -0.5 / 15.0;
+            if (I >= 3 && I == J) dg = 3.0 / 15.0;
+            dMdC[I + 6 * J] = wK * dk + wG * dg;
+        }
+    // W[J][d+3c]: weight of asig(d,c) of load J
+    std::vector<double> W(MAXR * 9, 0.0);
+    for (int J = 0; J < 6; J++) {
+        double eps = 2.0 * h->p.gstrain / (double)dims[CASE_PD[J][0]];
+        for (int I = 0; I < 6; I++) {
+            int d = VOIGT_DC[I][0], c = VOIGT_DC[I][1];
+            W[J * 9 + d + 3 * c] += dMdC[I + 6 * J] / (V * eps);
+        }
+    }
+    double *d_W = nullptr, *d_tA = nullptr, *d_tB = nullptr;
+    RC(dalloc(&d_W, W.size())); RC(dalloc(&d_tA, N)); RC(dalloc(&d_tB, N));
+    PSX_HIP(hipMemcpyAsync(d_W, W.data(), sizeof(double) * W.size(), hipMemcpyHostToDevice, h->stream));
+    int cs[MAXR] = {0, 1, 2, 3, 4, 5};
+    set_jumps(h->sv, cs, MAXR, h->p.gstrain);
+    PSX_HIP(hipMemcpyAsync(h->sv.d_jump, &h->sv.jump_h, sizeof(Jump), hipMemcpyHostToDevice, h->stream));
+    int g = grid_for(h->B.ntiles);
+    // adjoint rhs into sv.B, solve K lam = h
+    hipLaunchKernelGGL(k_adj_rhs, dim3(g), dim3(TILE), 0, h->stream, h->B.d_vols, h->B.d_tvol, h->B.d_tstart,
+                       h->B.ntiles, h->B.d_ka, h->B.d_kb, d_W, h->sv.B, N);
+    PSX_HIP(hipGetLastError());
+    std::vector<SysState> st;
+    int it = 0;
+    int rc = solve_batch(h->stream, h->B, h->sv, h->p.precision == PSX_PREC_F32, rtol, maxit, 1, &st, &it);
+    if (rc) { (void)hipFree(d_W); (void)hipFree(d_tA); (void)hipFree(d_tB); return rc; }
+    hipLaunchKernelGGL(k_adj_voxel, dim3(g), dim3(TILE), 0, h->stream, h->B.d_vols, h->B.d_tvol, h->B.d_tstart,
+                       h->B.ntiles, h->B.d_ka, h->B.d_kb, h->d_U, h->sv.X, d_W, N, h->sv.d_jump, d_tA, d_tB);
+    PSX_HIP(hipGetLastError());
+    std::vector<double> tA(N), tB(N);
+    PSX_HIP(hipMemcpyAsync(tA.data(), d_tA, sizeof(double) * N, hipMemcpyDeviceToHost, h->stream));
+    PSX_HIP(hipMemcpyAsync(tB.data(), d_tB, sizeof(double) * N, hipMemcpyDeviceToHost, h->stream));
+    PSX_HIP(hipStreamSynchronize(h->stream));
+    (void)hipFree(d_W); (void)hipFree(d_tA); (void)hipFree(d_tB);
+    double g4[4] = {0, 0, 0, 0};
+    for (int64_t i = 0; i < N; i++) {
+        int c = h->B.comp_h[i];
+        double s = h->B.stiff_h[i];
+        if (c == 1) {
+            g4[0] += s * tA[i]; g4[1] += s * tB[i];
+            if (dstiff) dstiff[i] = h->p.alf1 * tA[i] + h->p.bet1 * tB[i];
+        } else if (c == 2) {
+            g4[2] += s * tA[i]; g4[3] += s * tB[i];
+            if (dstiff) dstiff[i] = h->p.alf2 * tA[i] + h->p.bet2 * tB[i];
+        } else if (dstiff) {
+            dstiff[i] = 0.0;
+        }
+    }
+    std::memcpy(dab, g4, sizeof g4);
+    for (auto &s : st)
+        if (!s.conv) {
+            psx::set_error("psx_lsm_adjoint: adjoint solve did not reach rtol %.1e", rtol);
+            return PSX_ECONV;
+        }
+    return PSX_OK;
+    PSX_GUARD_END
+}
+
+extern "C" int psx_lsm_profile(psx_lsm *h, int32_t n, double *ms) {
+    PSX_CHECK_ARG(h && ms && n > 0, "psx_lsm_profile: bad argument");
+    PSX_GUARD_BEGIN
+    RC(ensure_solver(h, 1));
+    hipEvent_t e0, e1, e2;
+    PSX_HIP(hipEventCreate(&e0)); PSX_HIP(hipEventCreate(&e1)); PSX_HIP(hipEventCreate(&e2));
+    PSX_HIP(hipEventRecord(e0, h->stream));
+    for (int i = 0; i < n; i++)
+        launch_apply<double, false, false, false>(h->stream, h->B, 1, h->B.d_ka, h->B.d_kb, h->sv.P, h->sv.Qv, nullptr,
+                                                  h->sv.d_jump, nullptr);
+    PSX_HIP(hipEventRecord(e1, h->stream));
+    for (int i = 0; i < n; i++)
+        launch_apply<float, false, false, false>(h->stream, h->B, 1, h->B.d_ka32, h->B.d_kb32, h->sv.P32, h->sv.Q32,
+                                                 nullptr, h->sv.d_jump, nullptr);
+    PSX_HIP(hipEventRecord(e2, h->stream));
+    PSX_HIP(hipEventSynchronize(e2));
+    float a = 0, b = 0;
+    PSX_HIP(hipEventElapsedTime(&a, e0, e1));
+    PSX_HIP(hipEventElapsedTime(&b, e1, e2));
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2);
+    ms[0] = a / n;
+    ms[1] = b / n;
+    return PSX_OK;
+    PSX_GUARD_END
+}
+
+extern "C" int psx_lsm_batch_moduli(const psx_lsm_params *p, int32_t nvol, const int32_t *dims,
+                                    const int32_t *ind_concat, double rtol, int32_t maxit, double *C_out,
+                                    double *KG_out, double *ms_out) {
+    PSX_CHECK_ARG(p && dims && ind_concat && C_out && KG_out && nvol > 0 && rtol > 0.0 && maxit > 0,
+                  "psx_lsm_batch_moduli: bad argument");
+    int rc = psx::require_device();
+    if (rc) return rc;
+    PSX_GUARD_BEGIN
+    hipStream_t st;
+    PSX_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    Batch B;
+    Solver S;
+    auto cleanup = [&]() {
+        batch_free(B);
+        solver_free(S);
+        (void)hipStreamDestroy(st);
+    };
+    rc = batch_build(B, st, *p, nvol, dims, ind_concat, nullptr);
+    if (rc) { cleanup(); return rc; }
+    rc = solver_alloc(S, B, MAXR, true);
+    if (rc) { cleanup(); return rc; }
+    int cs[MAXR] = {0, 1, 2, 3, 4, 5};
+    set_jumps(S, cs, MAXR, p->gstrain);
+    PSX_HIP(hipMemcpyAsync(S.d_jump, &S.jump_h, sizeof(Jump), hipMemcpyHostToDevice, st));
+    PSX_HIP(hipStreamSynchronize(st));
+    hipEvent_t e0, e1;
+    PSX_HIP(hipEventCreate(&e0)); PSX_HIP(hipEventCreate(&e1));
+    PSX_HIP(hipEventRecord(e0, st));
+    rc = build_rhs(st, B, S);
+    std::vector<SysState> sst;
+    int it = 0;
+    if (!rc) rc = solve_batch(st, B, S, p->precision == PSX_PREC_F32, rtol, maxit, 1, &sst, &it);
+    std::vector<double> asig;
+    if (!rc) rc = stress_batch(st, B, S, S.X, asig);
+    PSX_HIP(hipEventRecord(e1, st));
+    PSX_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    PSX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+    if (rc) { cleanup(); return rc; }
+    for (int v = 0; v < nvol; v++)
+        moduli_from_stress(&asig[(size_t)v * MAXR * 9], B.vols[v], p->gstrain, &C_out[36 * v], &KG_out[2 * v],
+                           &KG_out[2 * v + 1]);
+    if (ms_out) { ms_out[0] = ms; ms_out[1] = it; }
+    bool allc = true;
+    for (auto &s : sst) allc &= s.conv != 0;
+    cleanup();
+    if (!allc) {
+        psx::set_error("psx_lsm_batch_moduli: some systems did not reach rtol %.1e", rtol);
+        return PSX_ECONV;
+    }
+    return PSX_OK;
+    PSX_GUARD_END
+}

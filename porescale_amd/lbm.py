@@ -1,0 +1,161 @@
+"""Two-phase D3Q19 colour-gradient LBM on the GPU (host-side interface).
+
+Mirrors the reference program ``twophaseflow``
+(``LBM-LSM/lsm_lbm_static_alpha_beta_multipore.f``): the same inputs
+(INPUT_PARAMETERS values, ``ind`` codes), the same state (``fr``/``fb`` of
+``iteration2`` in the reference's node order) and the same diagnostics
+(``check_state``, one ``output.m`` row).  The arithmetic runs in
+libporescale's HIP kernels; this module only moves arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from .io import LBMInput
+
+__all__ = ["LBM", "trt_rates", "mrt_rates"]
+
+
+def trt_rates(nu: float, lam: float, dt: float = 1.0) -> np.ndarray:
+    """TRT relaxation vector in the reference's moment order (collmatrTRT,
+    ``lsm_lbm_static_alpha_beta_multipore.f:2858-2893``) with magic parameter
+    ``lam`` instead of the hard-wired 3/16."""
+    sv = 1.0 / (3.0 * nu * dt + 0.5)
+    sq = 1.0 / (lam / (1.0 / sv - 0.5) + 0.5)
+    return np.array([0, sv, sv, 0, sq, 0, sq, 0, sq, sv, sv, sv, sv, sv, sv, sv, sq, sq, sq], float)
+
+
+def mrt_rates(nu: float, dt: float = 1.0, se=1.19, seps=1.4, sq=1.2, spi=1.4, sm=1.98) -> np.ndarray:
+    """MRT relaxation vector (d'Humieres basis order of collision,
+    ``lsm_lbm_static_alpha_beta_multipore.f:2519-2559``): viscous rates from
+    ``nu``, the others free."""
+    sv = 1.0 / (3.0 * nu * dt + 0.5)
+    return np.array([0, se, seps, 0, sq, 0, sq, 0, sq, sv, spi, sv, spi, sv, sv, sv, sm, sm, sm], float)
+
+
+class LBM:
+    """One LBM domain on the GPU.
+
+    Parameters follow the reference's INPUT_PARAMETERS (``inp``).  ``ind`` is
+    the voxel code array indexed ``[k, j, i]`` (what ``io.read_ind`` returns).
+    ``collision``: "ref" (collmatrTRT), ("trt", lam) or ("mrt", Sr, Sb).
+    ``precision``: "f64" (bit-identical to the reference) or "f32".
+    """
+
+    def __init__(self, inp: LBMInput, ind: np.ndarray, collision="ref", precision: str = "f64"):
+        L = _lib.lib()
+        self.inp = inp
+        nz, ny, nx = ind.shape
+        self.shape = (nz, ny, nx)
+        p = _lib.LBMParams()
+        for n in ("cr2", "rr0", "rb0", "B", "sig", "vir", "vib", "wet", "ee", "lsize", "dt"):
+            setattr(p, n, float(getattr(inp, n)))
+        p.idir, p.iipore = int(inp.idir), int(inp.iipore)
+        p.bfr[:] = [float(x) for x in inp.bfr]
+        p.bfb[:] = [float(x) for x in inp.bfb]
+        if collision == "ref":
+            p.collision = _lib.COLL_REF_TRT
+        elif collision[0] == "trt":
+            p.collision, p.lambda_ = _lib.COLL_TRT, float(collision[1])
+        elif collision[0] == "mrt":
+            p.collision = _lib.COLL_MRT
+            p.Sr[:] = [float(x) for x in collision[1]]
+            p.Sb[:] = [float(x) for x in collision[2]]
+        else:
+            raise ValueError(f"unknown collision {collision!r}")
+        p.precision = {"f64": _lib.PREC_F64, "f32": _lib.PREC_F32}[precision]
+        self.precision = precision
+        self._ind = _lib.i32(ind)
+        h = C.c_void_p()
+        _lib.check(L.psx_lbm_create(C.byref(p), nx, ny, nz, _lib.ptr(self._ind), C.byref(h)))
+        self._h = h
+        self._L = L
+        c = np.zeros(5, np.int64)
+        _lib.check(L.psx_lbm_counts(h, _lib.ptr(c)))
+        self.nl, self.nps, self.npb2, self.npb4, self.ns = (int(x) for x in c)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.psx_lbm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ------------------------------------------------------------------
+    def tables(self):
+        """Reference-format tables (1-based): icor (nl,3), ips (nps,3),
+        ipb2 (npb2,4), ipb4 (npb4,6); rows are Fortran columns."""
+        icor = np.zeros((self.nl, 3), np.int32)
+        ips = np.zeros((self.nps, 3), np.int32)
+        ipb2 = np.zeros((self.npb2, 4), np.int32)
+        ipb4 = np.zeros((self.npb4, 6), np.int32)
+        _lib.check(self._L.psx_lbm_tables(self._h, _lib.ptr(icor), _lib.ptr(ips), _lib.ptr(ipb2),
+                                          _lib.ptr(ipb4)))
+        return icor, ips, ipb2, ipb4
+
+    def rates(self):
+        Sr, Sb = np.zeros(19), np.zeros(19)
+        _lib.check(self._L.psx_lbm_rates(self._h, _lib.ptr(Sr), _lib.ptr(Sb)))
+        return Sr, Sb
+
+    def initdistr(self, sat: float):
+        """Random all-red / all-blue start of initdistr (icon = 0)."""
+        _lib.check(self._L.psx_lbm_init_random(self._h, float(sat)))
+
+    def set_state(self, fr: np.ndarray, fb: np.ndarray):
+        """fr, fb shaped (nl, 19): the reference's (19, nl) populations."""
+        fr, fb = _lib.f64(fr), _lib.f64(fb)
+        if fr.shape != (self.nl, 19) or fb.shape != (self.nl, 19):
+            raise ValueError(f"state must be ({self.nl}, 19)")
+        _lib.check(self._L.psx_lbm_set_state(self._h, _lib.ptr(fr), _lib.ptr(fb)))
+
+    def get_state(self):
+        fr, fb = np.zeros((self.nl, 19)), np.zeros((self.nl, 19))
+        _lib.check(self._L.psx_lbm_get_state(self._h, _lib.ptr(fr), _lib.ptr(fb)))
+        return fr, fb
+
+    def fields(self):
+        rr, rb, cf = np.zeros(self.nl), np.zeros(self.nl), np.zeros(self.nl)
+        _lib.check(self._L.psx_lbm_get_fields(self._h, _lib.ptr(rr), _lib.ptr(rb), _lib.ptr(cf)))
+        return rr, rb, cf
+
+    def step(self, nsteps: int = 1):
+        """``nsteps`` iterations of the majorityrule time loop body."""
+        _lib.check(self._L.psx_lbm_step(self._h, int(nsteps)))
+
+    def sync(self):
+        _lib.check(self._L.psx_lbm_sync(self._h))
+
+    def check_state(self, it1: int = 1, itw: int = 1) -> _lib.LBMDiag:
+        d = _lib.LBMDiag()
+        _lib.check(self._L.psx_lbm_check_state(self._h, int(it1), int(itw), C.byref(d)))
+        return d
+
+    def profile(self, nsteps: int):
+        """(total ms, mean collide-kernel ms, mean density-kernel ms)."""
+        ms = np.zeros(3)
+        _lib.check(self._L.psx_lbm_profile(self._h, int(nsteps), _lib.ptr(ms)))
+        return tuple(ms)
+
+    def run(self, itm: int, itw: int, prestep: bool = True, icon: int = 0):
+        """The majorityrule schedule: 2 pre-steps, then itm*itw steps with a
+        check_state after every itw; returns the output.m rows."""
+        if prestep:
+            self.step(2)
+        rows = []
+        it0 = 1
+        for it1 in range(it0, itm + 1):
+            self.step(itw)
+            rows.append(list(self.check_state(it1, itw).outm))
+        return np.array(rows)

@@ -1,0 +1,82 @@
+"""Writes the golden fixtures of tests/golden/ by running the flang-compiled
+reference programs (oracle/_ref, built by `make -C oracle ref` from
+/root/reference) on small seeded inputs.
+
+    python3 tests/golden/make_golden.py
+
+Fixtures (compressed .npz, inputs and reference outputs only):
+  lbm_icon0.npz   L1, random all-red/all-blue start (initdistr icon=0),
+                  12^3 sphere pack, 2 pre-steps + 5 steps.
+  lbm_blobs.npz   L1, blob start injected through iteration2 (icon=1),
+                  18x16x14 pack, wetting 0.7071, anisotropic body forces,
+                  unequal viscosities, 2 + 3*20 steps, output.m rows.
+  lsm_xx.npz      XX_P.f, two-component 10x12x14 pack, 100 Verlet steps,
+                  nthread=3: r, v, a and stress_tensor moduli.
+  lsm_yz.npz      YZ_P.f, same geometry, nthread=1.
+  ref_tables.npz  the reference's velocities19/velocities9 and cube_* files
+                  as read from /root/reference (data tables of the API).
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+import oracle_bind as ob  # noqa: E402
+from porescale_amd import geometry as geo  # noqa: E402
+from porescale_amd import io as pio  # noqa: E402
+from porescale_amd.lsm import spring_constants  # noqa: E402
+
+
+def lbm_cases():
+    n = 12
+    ind0 = geo.sphere_pack((n, n, n), 0.5, 3.0, seed=1)
+    inp0 = pio.LBMInput(B=0.7, sig=0.01, vir=1 / 6, vib=1 / 6, wet=0.0, idir=3, itm=1, itw=5, sat=0.5,
+                        ncx=n, ncy=n, ncz=n, dt=1.0, bfr=[0, 0, 1e-5], bfb=[0, 0, 1e-5])
+    ind1 = geo.sphere_pack((14, 16, 18), 0.45, 3.0, seed=2)
+    nz, ny, nx = ind1.shape
+    inp1 = pio.LBMInput(B=0.7, sig=0.02, vir=0.1, vib=0.3, wet=0.7071, idir=3, itm=3, itw=20, sat=0.4,
+                        ncx=nx, ncy=ny, ncz=nz, dt=1.0, bfr=[0, 1e-5, 2e-5], bfb=[1e-6, 0, 2e-5])
+    blue = geo.phase_blobs(ind1, 0.4, seed=3, radius=2.5)
+    return (ind0, inp0), (ind1, inp1, blue)
+
+
+def lsm_geometry():
+    ind = geo.sphere_pack((10, 12, 14), 0.4, 3.0, seed=5)
+    return geo.two_component(ind, 0.3, seed=6, radius=2.0)
+
+
+LSM_INP = dict(Em1=18.639, nu1=0.35065, Em2=94.529, nu2=0.0742, rs=1.0, dt=0.01, itm=2, its=50,
+               gstrain=0.01)
+
+
+def main():
+    (ind0, inp0), (ind1, inp1, blue) = lbm_cases()
+    fr, fb, outm, it1 = ob.run_ref_lbm(inp0, ind0)
+    np.savez_compressed(os.path.join(HERE, "lbm_icon0.npz"), fr=fr, fb=fb, outm=outm, steps=7)
+    fr0, fb0 = ob.equilibrium_state(inp1, ob.fluid_order_mask(ind1, blue))
+    fr, fb, outm, it1 = ob.run_ref_lbm(inp1, ind1, restart=(fr0, fb0))
+    np.savez_compressed(os.path.join(HERE, "lbm_blobs.npz"), fr=fr, fb=fb, outm=outm, steps=62)
+    ind = lsm_geometry()
+    nz, ny, nx = ind.shape
+    for case, nth in (("xx", 3), ("yz", 1)):
+        inp = pio.LSMInput(ncx=nx, ncy=ny, ncz=nz, nthread=nth, **LSM_INP)
+        r, v, a, st = ob.run_ref_lsm(case, ind, inp)
+        mods = ob.parse_stress_tensor(st)
+        np.savez_compressed(os.path.join(HERE, f"lsm_{case}.npz"), r=r, v=v, a=a, steps=100, nthread=nth,
+                            **{k: np.float64(v_) for k, v_ in mods.items()})
+    ref = "/root/reference/LSM2S/Cxxxx"
+    b, an, sb, ab = pio.read_cube_tables(ref)
+    np.savez_compressed(os.path.join(HERE, "ref_tables.npz"), velocities19=pio.read_velocities(
+        os.path.join(ref, "velocities19"), 19), velocities9=pio.read_velocities(os.path.join(ref, "velocities9"), 9),
+        cube_bon=b, cube_ang=an, cube_sbond=sb, cube_abond=ab)
+    print("golden fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

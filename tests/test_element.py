@@ -1,0 +1,138 @@
+"""The LSM element-matrix formulation, checked on the CPU against the oracle's
+bond/angle force loops (the reference algorithm) through the reference's node
+numbering.  This pins the exact matrices generated into lsm_element.h, the
+periodic-with-jump representation of the glued boundary, and
+reference_node_map, before any GPU run."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from porescale_amd import geometry as geo
+from porescale_amd.lsm import CASES, reference_node_map
+
+sys.path.insert(0, os.path.join(ob.ROOT, "porescale_amd", "csrc"))
+from gen_lsm_element import build  # noqa: E402
+
+CORNER = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0], [0, 0, 1], [1, 0, 1], [0, 1, 1], [1, 1, 1]])
+
+
+@pytest.fixture(scope="module")
+def mats():
+    KA, KB, SA, SB = build()
+    return tuple(np.array(M, dtype=float) for M in (KA, KB, SA, SB))
+
+
+def cube_fields(ind, alf, bet):
+    comp = np.where(ind == 0, 1, np.where(ind == 3, 2, 0))
+    ka = np.where(comp == 1, alf[0], np.where(comp == 2, alf[1], 0.0))
+    kb = np.where(comp == 1, bet[0], np.where(comp == 2, bet[1], 0.0))
+    return ka, kb
+
+
+def gather_cubes(u, J, vox):
+    """u (3,nz,ny,nx) -> (nvox, 24) corner displacements with the jump."""
+    _, nz, ny, nx = u.shape
+    kz, ky, kx = vox
+    out = np.zeros((len(kx), 24))
+    for p, (a, b, c) in enumerate(CORNER):
+        x, y, z = kx + a, ky + b, kz + c
+        wx, wy, wz = x // nx, y // ny, z // nz
+        for e in range(3):
+            out[:, 3 * p + e] = u[e, z % nz, y % ny, x % nx] + wx * J[0][e] + wy * J[1][e] + wz * J[2][e]
+    return out, (kx, ky, kz)
+
+
+def element_forces(u, ind, J, KA, KB, alf, bet):
+    ka, kb = cube_fields(ind, alf, bet)
+    vox = np.nonzero((ka != 0) | (kb != 0))
+    uc, (kx, ky, kz) = gather_cubes(u, J, vox)
+    fc = -(ka[vox][:, None] * (uc @ KA.T) + kb[vox][:, None] * (uc @ KB.T))
+    f = np.zeros_like(u)
+    nz, ny, nx = ind.shape
+    for p, (a, b, c) in enumerate(CORNER):
+        for e in range(3):
+            np.add.at(f[e], ((kz + c) % nz, (ky + b) % ny, (kx + a) % nx), fc[:, 3 * p + e])
+    return f
+
+
+def jump(case, g):
+    p, d = CASES[case]
+    J = np.zeros((3, 3))
+    J[p][d] = 2 * g
+    return J
+
+
+@pytest.mark.parametrize("case", ["xx", "yz", "zz", "xy"])
+def test_element_operator_equals_reference_force_loops(case, mats):
+    KA, KB, _, _ = mats
+    ind = geo.two_component(geo.sphere_pack((7, 8, 9), 0.45, 2.0, seed=21), 0.4, seed=22, radius=1.5)
+    alf, bet = (1.0, 0.37), (0.1, 0.021)
+    g = 0.01
+    o = ob.OracleLSM(ind, case, alf=alf[0], bet=bet[0], alf2=alf[1], bet2=bet[1], gstrain=g)
+    grid, twin, G = reference_node_map(ind, case)
+    icors, _ = o.nodes()
+    assert len(grid) == o.ns
+    nz, ny, nx = ind.shape
+    # the map reproduces the reference's node coordinates
+    pa = CASES[case][0]
+    gx = grid % nx
+    gy = (grid // nx) % ny
+    gz = grid // (nx * ny)
+    coords = np.stack([gx, gy, gz], 1)
+    coords[:, pa] += np.where(twin, ind.shape[2 - pa], 0)
+    assert np.array_equal(coords + 1, icors)
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal((3, nz, ny, nx)) * 1e-3
+    J = jump(case, g)
+    uref = u.reshape(3, -1)[:, grid].T + np.where(twin[:, None], J[pa][None, :], 0.0)
+    fref = o.forces(uref)
+    # glue: both twins act on the one periodic node
+    fgrid = np.zeros((3, nx * ny * nz))
+    for e in range(3):
+        np.add.at(fgrid[e], grid, fref[:, e])
+    fel = element_forces(u, ind, J, KA, KB, alf, bet).reshape(3, -1)
+    scale = np.abs(fgrid).max()
+    np.testing.assert_allclose(fel, fgrid, atol=1e-12 * scale)
+
+
+@pytest.mark.parametrize("case", ["xx", "yz"])
+def test_element_stress_equals_calc_stress(case, mats):
+    KA, KB, SA, SB = mats
+    ind = geo.two_component(geo.sphere_pack((6, 7, 8), 0.4, 2.0, seed=31), 0.3, seed=32, radius=1.5)
+    alf, bet = (1.0, 0.5), (0.1, 0.05)
+    g = 0.01
+    o = ob.OracleLSM(ind, case, alf=alf[0], bet=bet[0], alf2=alf[1], bet2=bet[1], gstrain=g)
+    grid, twin, _ = reference_node_map(ind, case)
+    nz, ny, nx = ind.shape
+    rng = np.random.default_rng(4)
+    u = rng.standard_normal((3, nz, ny, nx)) * 1e-3
+    J = jump(case, g)
+    pa = CASES[case][0]
+    uref = u.reshape(3, -1)[:, grid].T + np.where(twin[:, None], J[pa][None, :], 0.0)
+    o.set_state(r=uref)
+    asig, _, _, _ = o.stress()
+    ka, kb = cube_fields(ind, alf, bet)
+    vox = np.nonzero((ka != 0) | (kb != 0))
+    uc, _ = gather_cubes(u, J, vox)
+    sig = (ka[vox][:, None] * (uc @ SA.T) + kb[vox][:, None] * (uc @ SB.T)).sum(0)
+    np.testing.assert_allclose(sig.reshape(3, 3, order="F"), asig, atol=1e-12 * np.abs(asig).max())
+
+
+def test_element_matrices_structure(mats):
+    KA, KB, SA, SB = mats
+    assert np.array_equal(KA, KA.T) and np.array_equal(KB, KB.T)
+    assert np.linalg.eigvalsh(KA).min() > -1e-12 and np.linalg.eigvalsh(KB).min() > -1e-12
+    for d in range(3):  # translations carry no energy and no stress
+        t = np.zeros(24)
+        t[d::3] = 1
+        assert np.abs(KA @ t).max() == 0 and np.abs(KB @ t).max() == 0
+        assert np.abs(SA @ t).max() == 0 and np.abs(SB @ t).max() == 0
+    # homogeneous uniaxial strain: energy density 3a+8b (C11), stress the same
+    u = np.zeros(24)
+    for p, x in enumerate(CORNER):
+        u[3 * p] = x[0]
+    assert u @ KA @ u == pytest.approx(3.0) and u @ KB @ u == pytest.approx(8.0)
+    assert (SA @ u)[0] == pytest.approx(3.0) and (SB @ u)[0] == pytest.approx(8.0)

@@ -1,0 +1,168 @@
+"""LSM HIP kernels against the oracle and closed forms (GPU)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from golden.make_golden import LSM_INP, lsm_geometry
+from porescale_amd import geometry as geo
+from porescale_amd.lsm import CASES, LSM, batch_moduli, moduli_from_C, reference_node_map, spring_constants
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _to_ref(u, ind, case, g):
+    grid, twin, _ = reference_node_map(ind, case)
+    p, d = CASES[case]
+    J = np.zeros(3)
+    J[d] = 2 * g
+    return u.reshape(3, -1)[:, grid].T + np.where(twin[:, None], J[None, :], 0.0)
+
+
+@pytest.mark.parametrize("case", ["xx", "yz", "zx"])
+def test_forces_match_reference_force_loops(case):
+    ind = geo.two_component(geo.sphere_pack((9, 10, 11), 0.45, 2.5, seed=1), 0.4, seed=2, radius=1.5)
+    alf, bet = (1.0, 0.4), (0.1, 0.03)
+    g = 0.01
+    _say(f"case forces {case}")
+    o = ob.OracleLSM(ind, case, alf=alf[0], bet=bet[0], alf2=alf[1], bet2=bet[1], gstrain=g)
+    rng = np.random.default_rng(1)
+    u = rng.standard_normal((3,) + ind.shape) * 1e-3
+    with LSM(ind, alf[0], bet[0], alf[1], bet[1], gstrain=g) as s:
+        f = s.forces(case, u).reshape(3, -1)
+    fref = o.forces(_to_ref(u, ind, case, g))
+    grid, _, _ = reference_node_map(ind, case)
+    fg = np.zeros_like(f)
+    for e in range(3):
+        np.add.at(fg[e], grid, fref[:, e])
+    np.testing.assert_allclose(f, fg, atol=1e-12 * np.abs(fg).max())
+
+
+@pytest.mark.parametrize("case", ["xx", "yz"])
+def test_verlet_trajectory_matches_golden_reference(case):
+    ind = lsm_geometry()
+    g = np.load(os.path.join(GOLD, f"lsm_{case}.npz"))
+    a1, b1 = spring_constants(LSM_INP["Em1"], LSM_INP["nu1"])
+    a2, b2 = spring_constants(LSM_INP["Em2"], LSM_INP["nu2"])
+    _say(f"case verlet {case}")
+    with LSM(ind, a1, b1, a2, b2, rs=1.0, dt=0.01, gstrain=0.01) as s:
+        s.verlet(case, int(g["steps"]))
+        u = s.get_u(case)
+        _, m = s.calc_stress(case)
+    r = _to_ref(u, ind, case, 0.01)
+    np.testing.assert_allclose(r, g["r"], rtol=0, atol=1e-13 * np.abs(g["r"]).max())
+    if case == "xx":
+        assert m[0, 0] == pytest.approx(float(g["Exxxx"]), rel=1e-11)
+        assert m[1, 1] == pytest.approx(float(g["Eyyxx"]), rel=1e-10)
+    else:
+        assert m[1, 2] == pytest.approx(float(g["Eyzyz"]), rel=1e-11)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_all_solid_closed_form(prec):
+    a, b = 1.0, 0.1
+    ind = np.zeros((8, 8, 8), np.int32)
+    _say(f"case all-solid {prec}")
+    with LSM(ind, a, b, precision=prec) as s:
+        C6, K, G = s.moduli(rtol=1e-12, maxit=5000)
+    c11, c12, c44 = 3 * a + 8 * b, a - 4 * b, a + 6 * b
+    expect = np.zeros((6, 6))
+    expect[:3, :3] = c12
+    np.fill_diagonal(expect[:3, :3], c11)
+    expect[3, 3] = expect[4, 4] = expect[5, 5] = c44
+    np.testing.assert_allclose(C6, expect, atol=1e-9)
+    assert K == pytest.approx(5 * a / 3, rel=1e-10) and G == pytest.approx(c44, rel=1e-10)
+
+
+def test_solve_equals_converged_verlet_and_fp32_refinement():
+    ind = geo.sphere_pack((10, 10, 10), 0.3, 3.0, seed=3)
+    _say("case solve vs verlet")
+    with LSM(ind, 1.0, 0.1, dt=0.05) as s:
+        it, rel = s.solve("xx", 1e-11, 20000)
+        assert rel <= 1e-11
+        _, m_cg = s.calc_stress("xx")
+        s.verlet("xx", 40000)
+        _, m_v = s.calc_stress("xx")
+    with LSM(ind, 1.0, 0.1, precision="f32") as s32:
+        it32, rel32 = s32.solve("xx", 1e-11, 20000)
+        _, m32 = s32.calc_stress("xx")
+    assert rel32 <= 1e-11
+    assert m_v[0, 0] == pytest.approx(m_cg[0, 0], rel=1e-6)
+    assert m32[0, 0] == pytest.approx(m_cg[0, 0], rel=1e-9)
+
+
+def test_moduli_symmetry_and_batch_equals_single():
+    vols = [geo.two_component(geo.sphere_pack(sh, 0.35, 2.5, seed=s), 0.3, seed=s + 1, radius=1.5)
+            for s, sh in ((1, (10, 12, 9)), (2, (8, 8, 8)), (3, (12, 10, 11)))]
+    pars = (1.0, 0.1, 0.3, 0.02)
+    singles = []
+    for v in vols:
+        with LSM(v, *pars) as s:
+            C6, K, G = s.moduli(rtol=1e-10)
+            singles.append((C6, K, G))
+            assert (K, G) == pytest.approx(moduli_from_C(C6), rel=1e-12)
+            np.testing.assert_allclose(C6, C6.T, atol=1e-7 * np.abs(C6).max())
+    _say("case batch of 3")
+    for prec in ("f64", "f32"):
+        Cb, KG, ms, its = batch_moduli(vols, *pars, precision=prec, rtol=1e-10)
+        for (C6, K, G), cb, kg in zip(singles, Cb, KG):
+            np.testing.assert_allclose(cb, C6, atol=1e-8 * np.abs(C6).max())
+            assert kg[0] == pytest.approx(K, rel=1e-8) and kg[1] == pytest.approx(G, rel=1e-8)
+
+
+def test_adjoint_closed_form_homogeneous():
+    a, b = 1.0, 0.1
+    ind = np.zeros((6, 6, 6), np.int32)
+    with LSM(ind, a, b) as s:
+        s.moduli(rtol=1e-12)
+        dK, dsK = s.adjoint(1.0, 0.0, rtol=1e-12)
+        dG, dsG = s.adjoint(0.0, 1.0, rtol=1e-12)
+    # K = 5a/3, G = a + 6b for one component (code 0)
+    assert dK[0] == pytest.approx(5.0 / 3.0, rel=1e-8) and abs(dK[1]) < 1e-9
+    assert dG[0] == pytest.approx(1.0, rel=1e-8) and dG[1] == pytest.approx(6.0, rel=1e-8)
+    # uniform stiffness field: dK/ds_v = K/N
+    np.testing.assert_allclose(dsK, (5 * a / 3) / ind.size, rtol=1e-7)
+
+
+def test_adjoint_matches_finite_differences():
+    ind = geo.two_component(geo.sphere_pack((8, 9, 10), 0.35, 2.5, seed=8), 0.4, seed=9, radius=1.5)
+    ab = np.array([1.0, 0.1, 0.3, 0.02])
+    rng = np.random.default_rng(5)
+    stiff = np.exp(0.3 * rng.standard_normal(ind.shape))
+    _say("case adjoint fd")
+
+    def KG(ab_, st_):
+        with LSM(ind, *ab_, stiff=st_) as s:
+            _, K, G = s.moduli(rtol=1e-12, maxit=20000)
+        return np.array([K, G])
+
+    with LSM(ind, *ab, stiff=stiff) as s:
+        s.moduli(rtol=1e-12, maxit=20000)
+        gK, gsK = s.adjoint(1.0, 0.0, rtol=1e-12)
+        gG, _ = s.adjoint(0.0, 1.0, rtol=1e-12)
+    h = 1e-6
+    for k in range(4):
+        e = np.zeros(4)
+        e[k] = h * ab[k]
+        fd = (KG(ab + e, stiff) - KG(ab - e, stiff)) / (2 * e[k])
+        assert gK[k] == pytest.approx(fd[0], rel=1e-5, abs=1e-8)
+        assert gG[k] == pytest.approx(fd[1], rel=1e-5, abs=1e-8)
+    solid = np.argwhere(ind != 1)
+    for idx in solid[[0, len(solid) // 2, -1]]:
+        e = np.zeros(ind.shape)
+        e[tuple(idx)] = h
+        fd = (KG(ab, stiff + e)[0] - KG(ab, stiff - e)[0]) / (2 * h)
+        assert gsK[tuple(idx)] == pytest.approx(fd, rel=1e-4, abs=1e-9)
+
+
+def test_matvec_profile():
+    ind = geo.sphere_pack((32, 32, 32), 0.35, 6.0, seed=0)
+    with LSM(ind, 1.0, 0.1) as s:
+        t64, t32 = s.profile(5)
+    assert t64 > 0 and t32 > 0

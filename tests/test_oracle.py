@@ -1,0 +1,147 @@
+"""The CPU oracle pinned against the reference (CPU only).
+
+* golden fixtures written by the flang-compiled reference (tests/golden):
+  bit-identical states, trajectories and moduli;
+* the reference binaries themselves (oracle/_ref) on fresh seeded inputs,
+  when they were built here;
+* closed forms of the lattice spring model on an all-solid block:
+  C11 = 3a+8b, C12 = a-4b, C44 = a+6b.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from golden.make_golden import LSM_INP, lbm_cases, lsm_geometry
+from porescale_amd import geometry as geo
+from porescale_amd import io as pio
+from porescale_amd.lsm import spring_constants
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _gold(name):
+    return np.load(os.path.join(GOLD, name))
+
+
+def test_lbm_icon0_bitexact_vs_reference():
+    (ind, inp), _ = lbm_cases()
+    g = _gold("lbm_icon0.npz")
+    o = ob.OracleLBM(inp, ind)
+    o.init_random(inp.sat)
+    o.step(int(g["steps"]))
+    fr, fb = o.state()
+    assert np.array_equal(fr, g["fr"]) and np.array_equal(fb, g["fb"])
+    d = o.check_state(1, inp.itw)
+    # output.m is printed with E15.8
+    np.testing.assert_allclose(np.array(d.outm), g["outm"][-1], rtol=1e-7, equal_nan=True)
+
+
+def test_lbm_blobs_restart_bitexact_vs_reference():
+    _, (ind, inp, blue) = lbm_cases()
+    g = _gold("lbm_blobs.npz")
+    o = ob.OracleLBM(inp, ind)
+    o.set_state(*ob.equilibrium_state(inp, ob.fluid_order_mask(ind, blue)))
+    o.step(2)
+    for it1 in range(1, inp.itm + 1):
+        o.step(inp.itw)
+        d = o.check_state(it1, inp.itw)
+        np.testing.assert_allclose(np.array(d.outm), g["outm"][it1 - 1], rtol=2e-7, atol=1e-14, equal_nan=True)
+    fr, fb = o.state()
+    assert np.array_equal(fr, g["fr"]) and np.array_equal(fb, g["fb"])
+
+
+@pytest.mark.parametrize("case", ["xx", "yz"])
+def test_lsm_verlet_bitexact_vs_reference(case):
+    ind = lsm_geometry()
+    g = _gold(f"lsm_{case}.npz")
+    a1, b1 = spring_constants(LSM_INP["Em1"], LSM_INP["nu1"])
+    a2, b2 = spring_constants(LSM_INP["Em2"], LSM_INP["nu2"])
+    o = ob.OracleLSM(ind, case, alf=a1, bet=b1, alf2=a2, bet2=b2, rs=1.0, dt=0.01, gstrain=0.01,
+                     nthread=int(g["nthread"]))
+    o.verlet(int(g["steps"]))
+    r, v, a = o.state()
+    assert np.array_equal(r, g["r"]) and np.array_equal(v, g["v"]) and np.array_equal(a, g["a"])
+    _, _, m, _ = o.stress()
+    if case == "xx":
+        for key, (i, j) in {"Exxxx": (0, 0), "Eyyxx": (1, 1), "Eyyzz": (2, 2)}.items():
+            assert m[i, j] == pytest.approx(float(g[key]), rel=1e-14)
+    else:
+        assert m[1, 2] == pytest.approx(float(g["Eyzyz"]), rel=1e-14)
+
+
+def test_lsm_tables_counts_consistent():
+    ind = lsm_geometry()
+    o = ob.OracleLSM(ind, "xx", alf=1.0, bet=0.1)
+    c = o.counts()
+    bon, s1, s2, ang, a1, a2 = o.tables()
+    assert bon.shape == (c["nbo"], 3) and ang.shape == (c["nna"], 6)
+    # weights are quarter/half counts of adjacent cubes of each component
+    assert set(np.unique(np.round((s1 + s2) * 4))) <= {1, 2, 3, 4}
+    assert np.all((ang[:, 5] >= 1) & (ang[:, 5] <= 48))
+    icors, mijks = o.nodes()
+    assert icors.shape[0] == c["ns"] == mijks.max()
+
+
+@pytest.mark.parametrize("case,key,expect", [("xx", (0, 0), "c11"), ("xx", (1, 1), "c12"), ("yz", (1, 2), "c44")])
+def test_lsm_closed_form_all_solid(case, key, expect):
+    a, b = 1.0, 0.1
+    ind = np.zeros((6, 6, 6), np.int32)
+    o = ob.OracleLSM(ind, case, alf=a, bet=b, dt=0.01, gstrain=0.01)
+    o.verlet(20000)
+    _, _, m, ncub = o.stress()
+    assert ncub == 216
+    val = {"c11": 3 * a + 8 * b, "c12": a - 4 * b, "c44": a + 6 * b}[expect]
+    assert m[key] == pytest.approx(val, rel=1e-9)
+
+
+def test_lsm_anf_geometry():
+    """anf rows: unit plane normal, fj = n x xj, fk = -(n x xk), orthogonal to n."""
+    anf = np.zeros(9 * 48)
+    nta = np.zeros(2 * 48, np.int32)
+    ob.lib().or_lsm_anf(anf, nta)
+    anf = anf.reshape(48, 9)
+    nta = nta.reshape(48, 2)
+    assert np.array_equal(nta, geo.angle_types())
+    for t in range(48):
+        n, fj, fk = anf[t, :3], anf[t, 3:6], anf[t, 6:9]
+        cj = pio.D3Q19[:, nta[t, 0] - 1].astype(float)
+        ck = pio.D3Q19[:, nta[t, 1] - 1].astype(float)
+        assert np.linalg.norm(n) == pytest.approx(1.0)
+        assert abs(n @ cj) < 1e-15 and abs(n @ ck) < 1e-15
+        np.testing.assert_allclose(fj, np.cross(n, cj / (cj @ cj)), atol=1e-15)
+        np.testing.assert_allclose(fk, -np.cross(n, ck / (ck @ ck)), atol=1e-15)
+
+
+@pytest.mark.ref
+@pytest.mark.skipif(not ob.ref_available("l1_ref"), reason="oracle/_ref not built (needs /root/reference)")
+def test_lbm_fresh_case_vs_reference_binary():
+    ind = geo.sphere_pack((10, 11, 13), 0.55, 2.5, seed=11)
+    nz, ny, nx = ind.shape
+    inp = pio.LBMInput(B=0.5, sig=0.05, vir=0.05, vib=0.2, wet=-0.3, idir=1, itm=2, itw=7, sat=0.3,
+                       ncx=nx, ncy=ny, ncz=nz, dt=1.0, bfr=[2e-5, 0, 0], bfb=[3e-5, 1e-6, 0])
+    fr, fb, outm, _ = ob.run_ref_lbm(inp, ind)
+    o = ob.OracleLBM(inp, ind)
+    o.init_random(inp.sat)
+    o.step(2 + 2 * 7)
+    ofr, ofb = o.state()
+    assert np.array_equal(ofr, fr) and np.array_equal(ofb, fb)
+    icor, ips, ipb2, ipb4 = o.tables()
+    assert len(ips) + len(ipb2) + len(ipb4) == 18 * o.nl
+
+
+@pytest.mark.ref
+@pytest.mark.skipif(not ob.ref_available("xx_ref"), reason="oracle/_ref not built (needs /root/reference)")
+def test_lsm_fresh_case_vs_reference_binary():
+    ind = geo.two_component(geo.sphere_pack((9, 8, 7), 0.5, 2.0, seed=12), 0.5, seed=13, radius=1.5)
+    nz, ny, nx = ind.shape
+    inp = pio.LSMInput(ncx=nx, ncy=ny, ncz=nz, Em1=3.0, nu1=0.2, Em2=7.0, nu2=0.1, rs=1.3, dt=0.02,
+                       itm=1, its=60, gstrain=0.005, nthread=2)
+    r, v, a, st = ob.run_ref_lsm("xx", ind, inp)
+    a1, b1 = spring_constants(inp.Em1, inp.nu1)
+    a2, b2 = spring_constants(inp.Em2, inp.nu2)
+    o = ob.OracleLSM(ind, "xx", alf=a1, bet=b1, alf2=a2, bet2=b2, rs=1.3, dt=0.02, gstrain=0.005, nthread=2)
+    o.verlet(60)
+    orr, ov, oa = o.state()
+    assert np.array_equal(orr, r) and np.array_equal(ov, v) and np.array_equal(oa, a)
