@@ -20,9 +20,9 @@
 //                streaming link, or -1 for a bounce-back link (the population
 //                returns to the same node in the opposite direction).
 //   vox[nl]      voxel index of each node in the phi grid.
-//   phi grid     dense nx*ny*nz, z fastest (index z + nz*(y + ny*x)): colour
-//                field cf at fluid voxels, `wet` at solid voxels, so the
-//                27-point stencil of calcFs is plain loads.
+//   phi grid     dense nx*ny*nz FP64, z fastest (index z + nz*(y + ny*x)):
+//                colour field cf at fluid voxels, `wet` at solid voxels, so
+//                the 27-point stencil of calcFs is plain loads.
 //
 // One time step = two kernels: collide+push (reads own populations, writes
 // the streamed post-collision populations into the other buffer) and density
@@ -75,7 +75,7 @@ struct StoreF64 {
 };
 
 struct StoreF32 {
-    using phi_t = float;
+    using phi_t = double;  // the colour field stays FP64: its differences drive the surface force
     float *__restrict__ p;  // T[19][nl] then R[19][nl]
     int64_t nl;
     __device__ __forceinline__ void load_c(const LbmConst &c, int q, int64_t i, double &fr, double &fb) const {
@@ -637,7 +637,7 @@ int launch_density(psx_lbm *h, int buf) {
     if (h->f32) {
         StoreF32 s{(float *)h->d_pop[buf], h->nl};
         hipLaunchKernelGGL(k_density<StoreF32>, dim3((unsigned)nb), dim3(256), 0, h->stream, h->c, s,
-                           h->d_vox, (float *)h->d_phi, h->xcd);
+                           h->d_vox, (double *)h->d_phi, h->xcd);
     } else {
         StoreF64 s{(double *)h->d_pop[buf], h->nl};
         hipLaunchKernelGGL(k_density<StoreF64>, dim3((unsigned)nb), dim3(256), 0, h->stream, h->c, s,
@@ -654,7 +654,7 @@ int launch_collide(psx_lbm *h) {
     if (h->f32) {
         StoreF32 si{(float *)h->d_pop[a], h->nl}, so{(float *)h->d_pop[b], h->nl};
         hipLaunchKernelGGL(k_collide<StoreF32>, dim3((unsigned)nb), dim3(256), 0, h->stream, h->c, si, so,
-                           h->d_dst, h->d_vox, (const float *)h->d_phi, h->xcd);
+                           h->d_dst, h->d_vox, (const double *)h->d_phi, h->xcd);
     } else {
         StoreF64 si{(double *)h->d_pop[a], h->nl}, so{(double *)h->d_pop[b], h->nl};
         hipLaunchKernelGGL(k_collide<StoreF64>, dim3((unsigned)nb), dim3(256), 0, h->stream, h->c, si, so,
@@ -734,7 +734,7 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
     PSX_HIP(hipMemsetAsync(h->d_pop[0], 0, popb, h->stream));
     PSX_HIP(hipMalloc(&h->d_dst, sizeof(int32_t) * 18 * (size_t)(nl > 0 ? nl : 1)));
     PSX_HIP(hipMalloc(&h->d_vox, sizeof(int32_t) * (size_t)(nl > 0 ? nl : 1)));
-    size_t phib = (size_t)nv * (h->f32 ? sizeof(float) : sizeof(double));
+    size_t phib = (size_t)nv * sizeof(double);
     PSX_HIP(hipMalloc(&h->d_phi, phib));
     int64_t nb = std::max<int64_t>(1, psx::ceil_div(nl, 256));
     PSX_HIP(hipMalloc(&h->d_partial, sizeof(double) * NDIAG * nb));
@@ -744,8 +744,7 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
         PSX_HIP(hipMemcpyAsync(h->d_vox, vox.data(), sizeof(int32_t) * nl, hipMemcpyHostToDevice, h->stream));
     }
     int64_t gb = psx::ceil_div(nv, 256);
-    if (h->f32) hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)gb), dim3(256), 0, h->stream, (float *)h->d_phi, nv, (float)p->wet);
-    else hipLaunchKernelGGL(k_fill<double>, dim3((unsigned)gb), dim3(256), 0, h->stream, (double *)h->d_phi, nv, p->wet);
+    hipLaunchKernelGGL(k_fill<double>, dim3((unsigned)gb), dim3(256), 0, h->stream, (double *)h->d_phi, nv, p->wet);
     PSX_HIP(hipGetLastError());
     PSX_HIP(hipStreamSynchronize(h->stream));
     *out = h;
@@ -996,7 +995,7 @@ extern "C" int psx_lbm_check_state(psx_lbm *h, int64_t it1, int64_t itw, psx_lbm
         if (h->f32) {
             StoreF32 st{(float *)h->d_pop[h->cur], h->nl};
             hipLaunchKernelGGL(k_diag<StoreF32>, dim3((unsigned)nb), dim3(256), 0, h->stream, h->c, st, h->d_vox,
-                               (const float *)h->d_phi, h->d_partial);
+                               (const double *)h->d_phi, h->d_partial);
         } else {
             StoreF64 st{(double *)h->d_pop[h->cur], h->nl};
             hipLaunchKernelGGL(k_diag<StoreF64>, dim3((unsigned)nb), dim3(256), 0, h->stream, h->c, st, h->d_vox,

@@ -57,9 +57,12 @@ struct Jump {
 
 // per-system scalars (system s = volume*R + rhs)
 struct SysState {
-    double rz, pq, alpha, beta, rr, bb, scale, pad_;
+    double rz, pq, alpha, beta, rr, bb, scale;
+    double bref;  // squared norm of the rhs summed without cancellation (0 = unused):
+                  // a residual below 1e-14 of it is rounding noise and counts as converged
     int32_t active, iters, conv, pad2_;
 };
+constexpr double NOISE2 = 1e-28;
 
 __device__ __forceinline__ int wrapm(int t, int n, int &w) {
     w = t < 0 ? -1 : (t >= n ? 1 : 0);
@@ -260,7 +263,7 @@ __global__ __launch_bounds__(256) void k_sys_reduce(const VolDesc *__restrict__ 
         st.rr = sb;
         st.bb = sb;
         st.iters = 0;
-        st.conv = sb <= 0.0;
+        st.conv = sb <= 0.0 || sb <= NOISE2 * st.bref;
         st.active = !st.conv;
         return;
     }
@@ -273,7 +276,7 @@ __global__ __launch_bounds__(256) void k_sys_reduce(const VolDesc *__restrict__ 
         st.rz = sa;
         st.rr = sb;
         st.iters += 1;
-        if (sb <= tol2 * st.bb) { st.conv = 1; st.active = 0; }
+        if (sb <= tol2 * st.bb || sb <= NOISE2 * st.bref) { st.conv = 1; st.active = 0; }
         else if (st.iters >= maxit || !(sb == sb)) st.active = 0;
     }
 }
@@ -347,11 +350,11 @@ __global__ __launch_bounds__(TILE) void k_cg_pupdate(const VolDesc *__restrict__
         for (int r = 0; r < R; r++) {
             const SysState st = sys[v * R + r];
             if (!init && !st.active) continue;
-            T be = init ? T(0) : (T)st.beta;
+            T be = (T)st.beta;
 #pragma unroll
             for (int e = 0; e < 3; e++) {
                 int64_t k = (int64_t)r * 3 * Ntot + (int64_t)e * Ntot + n;
-                P[k] = Z[k] + be * P[k];
+                P[k] = init ? Z[k] : Z[k] + be * P[k];  // P is not read at start
             }
         }
     }
@@ -462,14 +465,32 @@ __global__ __launch_bounds__(256) void k_outer_reduce(const VolDesc *__restrict_
     if (first) { st.bb = sa; st.iters = 0; st.conv = 0; }
     st.rr = sa;
     st.scale = sqrt(sa);
-    if (sa <= tol2 * st.bb) { st.conv = 1; st.active = 0; }
+    if (sa <= tol2 * st.bb || sa <= NOISE2 * st.bref) { st.conv = 1; st.active = 0; }
     else st.active = 1;
+}
+
+// bref of every system from (tile, rhs) partials; mode 0 clears it.
+__global__ __launch_bounds__(256) void k_set_bref(const VolDesc *__restrict__ vols, int R,
+                                                  const double *__restrict__ part, SysState *__restrict__ a,
+                                                  SysState *__restrict__ b) {
+    const int s = blockIdx.x;
+    const int v = s / R, r = s % R;
+    const VolDesc vd = vols[v];
+    __shared__ double sh[4];
+    double x = 0.0;
+    if (part)
+        for (int k = threadIdx.x; k < vd.ntile; k += blockDim.x) x += part[(int64_t)(vd.tile0 + k) * R + r];
+    double sx = block_sum(x, sh);
+    if (threadIdx.x == 0) { a[s].bref = sx; b[s].bref = sx; }
 }
 
 // Inner FP32 CG systems follow the outer activity.
 __global__ void k_inner_from_outer(int S, const SysState *__restrict__ outer, SysState *__restrict__ inner) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < S) inner[s].active = outer[s].active;
+    if (s < S) {
+        inner[s].active = outer[s].active;
+        inner[s].bref = 0.0;  // the inner rhs is the scaled residual
+    }
 }
 
 // ---------------------------------------------------------------- stress --
@@ -558,22 +579,24 @@ __global__ __launch_bounds__(TILE) void k_verlet_b(int64_t n3, const double *__r
 // --------------------------------------------------------------- adjoint --
 // Adjoint right-hand side h_J = sum_I W[I][J] g_I with g_I the gradient of
 // the summed cube stress component I with respect to u (gathered per node).
+// Also writes partial sums of the squared cancellation-free magnitude
+// (sum of |terms|) per (tile, rhs) into `mag` for the noise floor.
 __global__ __launch_bounds__(TILE) void k_adj_rhs(const VolDesc *__restrict__ vols, const int32_t *__restrict__ tvol,
                                                   const int32_t *__restrict__ tstart, int ntiles,
                                                   const double *__restrict__ ka, const double *__restrict__ kb,
                                                   const double *__restrict__ W /* [6 loads][9 comps] */,
-                                                  double *__restrict__ H, int64_t Ntot) {
+                                                  double *__restrict__ H, int64_t Ntot, double *__restrict__ mag) {
+    __shared__ double sh[TILE / 64];
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const VolDesc vd = vols[tvol[t]];
         int x, y, z;
         bool ok;
         tile_node(vd, tstart[t], x, y, z, ok);
-        if (!ok) continue;
         int64_t n = vd.off + x + (int64_t)vd.nx * (y + (int64_t)vd.ny * z);
         double a[8], b[8];
         gather_k(vd, ka, kb, x, y, z, a, b);
         for (int r = 0; r < MAXR; r++) {
-            double h[3] = {0, 0, 0};
+            double h[3] = {0, 0, 0}, m[3] = {0, 0, 0};
 #pragma unroll
             for (int p = 0; p < 8; p++) {
                 if (a[p] == 0.0 && b[p] == 0.0) continue;
@@ -582,11 +605,21 @@ __global__ __launch_bounds__(TILE) void k_adj_rhs(const VolDesc *__restrict__ vo
                     double w = W[r * 9 + k];
                     if (w == 0.0) continue;
 #pragma unroll
-                    for (int e = 0; e < 3; e++)
-                        h[e] += w * (a[p] * lsm_el::SA[k][3 * p + e] + b[p] * lsm_el::SB[k][3 * p + e]);
+                    for (int e = 0; e < 3; e++) {
+                        double ta = w * a[p] * lsm_el::SA[k][3 * p + e], tb = w * b[p] * lsm_el::SB[k][3 * p + e];
+                        h[e] += ta + tb;
+                        m[e] += fabs(ta) + fabs(tb);
+                    }
                 }
             }
-            for (int e = 0; e < 3; e++) H[(int64_t)r * 3 * Ntot + (int64_t)e * Ntot + n] = h[e];
+            double d = 0.0;
+            if (ok)
+                for (int e = 0; e < 3; e++) {
+                    H[(int64_t)r * 3 * Ntot + (int64_t)e * Ntot + n] = h[e];
+                    d += m[e] * m[e];
+                }
+            double sm = block_sum(d, sh);
+            if (threadIdx.x == 0) mag[(int64_t)t * MAXR + r] = sm;
         }
     }
 }
@@ -839,6 +872,8 @@ void launch_apply(hipStream_t st, const Batch &B, int R, const T *ka, const T *k
 int build_rhs(hipStream_t st, const Batch &B, Solver &S) {
     int64_t n = (int64_t)S.vec;
     int gb = (int)psx::ceil_div(n, 256);
+    hipLaunchKernelGGL(k_set_bref, dim3(S.S), dim3(256), 0, st, B.d_vols, S.R, (const double *)nullptr, S.sys,
+                       S.outer);
     hipLaunchKernelGGL(k_zero<double>, dim3(gb), dim3(256), 0, st, S.Z, n);
     launch_apply<double, true, true, false>(st, B, S.R, B.d_ka, B.d_kb, S.Z, S.B, nullptr, S.d_jump, nullptr);
     PSX_HIP(hipGetLastError());
@@ -943,7 +978,8 @@ int solve_batch(hipStream_t st, const Batch &B, Solver &S, bool f32, double tol,
     hipLaunchKernelGGL(k_zero<double>, dim3(gb), dim3(256), 0, st, S.X, n);
     const double inner_tol = 1e-5;
     std::vector<SysState> ho(S.S);
-    for (int outer = 0; outer < 12; outer++) {
+    const int max_outer = 16;
+    for (int outer = 0; outer <= max_outer; outer++) {
         // r = b - K x
         launch_apply<double, false, false, false>(st, B, S.R, B.d_ka, B.d_kb, S.X, S.Qv, nullptr, S.d_jump, nullptr);
         hipLaunchKernelGGL(k_refine_residual, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles,
@@ -955,7 +991,7 @@ int solve_batch(hipStream_t st, const Batch &B, Solver &S, bool f32, double tol,
         PSX_HIP(hipStreamSynchronize(st));
         bool any = false;
         for (auto &s : ho) any |= s.active != 0;
-        if (!any || total_iters >= maxit) break;
+        if (!any || total_iters >= maxit || outer == max_outer) break;
         // inner FP32 solve on the scaled residual
         hipLaunchKernelGGL(k_scale_sys, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles, S.R,
                            S.outer, S.R32, S.X32, S.Rr, B.Ntot);
@@ -1339,7 +1375,9 @@ extern "C" int psx_lsm_adjoint(psx_lsm *h, double wK, double wG, double rtol, in
     int g = grid_for(h->B.ntiles);
     // adjoint rhs into sv.B, solve K lam = h
     hipLaunchKernelGGL(k_adj_rhs, dim3(g), dim3(TILE), 0, h->stream, h->B.d_vols, h->B.d_tvol, h->B.d_tstart,
-                       h->B.ntiles, h->B.d_ka, h->B.d_kb, d_W, h->sv.B, N);
+                       h->B.ntiles, h->B.d_ka, h->B.d_kb, d_W, h->sv.B, N, h->sv.prr);
+    hipLaunchKernelGGL(k_set_bref, dim3(h->sv.S), dim3(256), 0, h->stream, h->B.d_vols, h->sv.R,
+                       (const double *)h->sv.prr, h->sv.sys, h->sv.outer);
     PSX_HIP(hipGetLastError());
     std::vector<SysState> st;
     int it = 0;

@@ -129,7 +129,10 @@ def test_fp32_storage_tracks_fp64(blobs):
     # exact zeros of the minority phase survive FP32 storage
     assert d32.tmb == 0.0 if not blobs else d32.tmb > 0
     assert abs(d32.tm - d64.tm) / d64.tm < 1e-6
-    assert d32.permr == pytest.approx(d64.permr, rel=2e-4 if blobs else 1e-5)
+    # a two-phase transient amplifies storage rounding through the interface
+    # dynamics; single-phase flow is held to the storage precision
+    assert d32.permr == pytest.approx(d64.permr, rel=2e-3 if blobs else 1e-5)
+    assert abs(d32.nr - d64.nr) <= 0.002 * (d64.nr + d64.nb)
     if not blobs:
         np.testing.assert_allclose(out["f32"][1][0], out["f64"][1][0], rtol=0, atol=2e-8)
 
