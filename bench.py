@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark of the two porescale hot paths on one MI355X.
+
+Headline (the JSON line's metric/value): the D3Q19 two-phase colour-gradient
+LBM step with MRT collision at 256^3 (sphere pack, radii 4..10, porosity
+0.30, viscosity ratio 1:10, wetting 0.7071, blob saturation 0.5), FP32
+population storage with FP64 arithmetic: MLUPS over fluid nodes.  The step
+is one collide+push launch and one density launch; the roofline entry is the
+collide kernel, timed with HIP events on the library's own stream.
+
+Also reported (objects in the same line):
+  lsm          64^3 single-solid sphere pack (porosity 0.35), all six load
+               cases to rtol 1e-10: plain FP64 PCG one case at a time versus
+               the batched FP32-matvec/FP64-refinement solver; moduli error;
+               batched sub-volumes and adjoint gradients per second.
+  cpu_baseline the oracle (C restatement of the reference, 1 thread) timed on
+               this host over a bounded sample of the same LBM workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--lsm/--no-lsm]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def lbm_workload(n=256):
+    from porescale_amd import geometry as geo
+    from porescale_amd import io as pio
+    ind = geo.sphere_pack((n, n, n), 0.30, (4, 10), seed=0)
+    inp = pio.LBMInput(B=0.7, sig=0.01, vir=1.0 / 6.0, vib=1.0 / 60.0, wet=0.70710678118654752, idir=3,
+                       ncx=n, ncy=n, ncz=n, dt=1.0, bfr=[0.0, 0.0, 1e-5], bfb=[0.0, 0.0, 1e-5])
+    blue = geo.phase_blobs(ind, 0.5, seed=0, radius=4.0)
+    return ind, inp, blue
+
+
+def initial_state(inp, ind, blue):
+    import numpy as np
+    # rest equilibrium, all-red or all-blue per node (initdistr's state,
+    # lsm_lbm_static_alpha_beta_multipore.f:2834-2840) in the reference node
+    # order (x outer, z fastest)
+    dt = inp.dt
+    cb2 = inp.rr0 * inp.cr2 / inp.rb0
+    tr = np.array([1 - 2 * inp.cr2 * dt * dt] + [inp.cr2 * dt * dt / 6] * 6 + [inp.cr2 * dt * dt / 12] * 12)
+    tb = np.array([1 - 2 * cb2 * dt * dt] + [cb2 * dt * dt / 6] * 6 + [cb2 * dt * dt / 12] * 12)
+    t_ind = np.transpose(ind, (2, 1, 0))
+    bl = np.transpose(blue, (2, 1, 0))[t_ind == inp.iipore]
+    c = np.where(bl, 0.0, 1.0)
+    return tr[None, :] * c[:, None] * inp.rr0, tb[None, :] * (1 - c[:, None]) * inp.rb0
+
+
+def run_lbm(steps, warmup):
+    from porescale_amd.lbm import LBM, mrt_rates
+    ind, inp, blue = lbm_workload()
+    fr0, fb0 = initial_state(inp, ind, blue)
+    coll = ("mrt", mrt_rates(inp.vir), mrt_rates(inp.vib))
+    out = {}
+    with LBM(inp, ind, collision=coll, precision="f32") as lb:
+        lb.set_state(fr0, fb0)
+        lb.step(warmup)
+        lb.sync()
+        t0 = time.perf_counter()
+        tot, tc, td = lb.profile(steps)   # events around every kernel, same stream
+        wall = time.perf_counter() - t0
+        d = lb.check_state(1, steps)
+        out.update(nl=lb.nl, tot_ms=tot, coll_ms=tc, dens_ms=td, wall_s=wall, permr=d.permr, permb=d.permb,
+                   sat_b=d.outm[11])
+    return out, ind, inp, blue
+
+
+def cpu_baseline(ind, inp, blue, budget_s=12.0):
+    """Oracle (1 thread) on the same 256^3 workload: steps until the budget."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+    from porescale_amd.lbm import mrt_rates
+    S = (mrt_rates(inp.vir), mrt_rates(inp.vib))
+    o = ob.OracleLBM(inp, ind, S=S)
+    o.set_state(*initial_state(inp, ind, blue))
+    n, t = 0, 0.0
+    while t < budget_s and n < 50:
+        t0 = time.perf_counter()
+        o.step(1)
+        t += time.perf_counter() - t0
+        n += 1
+    return {"value": o.nl * n / t / 1e6, "unit": "MLUPS", "cores": 1, "kind": "port",
+            "sample": f"{n} steps of the same 256^3 two-phase MRT workload, oracle in C, 1 thread"}
+
+
+def run_lsm(quick=False):
+    import numpy as np
+    from porescale_amd import geometry as geo
+    from porescale_amd.lsm import LSM, batch_moduli
+    res = {}
+    ind = geo.sphere_pack((64, 64, 64), 0.35, 6.0, seed=0)
+    # plain FP64: block-Jacobi PCG, one load case per solve
+    with LSM(ind, 1.0, 0.1, precision="f64") as s:
+        s.solve(0, 1e-10, 50000)
+        t0 = time.perf_counter()
+        its = []
+        for case in range(6):
+            it, rel = s.solve(case, 1e-10, 50000)
+            its.append(it)
+        t_plain = time.perf_counter() - t0
+        _, K64, G64 = s.moduli(1e-10, 50000)
+    # batched: six cases per launch, FP32 matvec + FP64 refinement
+    with LSM(ind, 1.0, 0.1, precision="f32") as s:
+        s.moduli(1e-10, 50000)
+        t0 = time.perf_counter()
+        C6, K, G = s.moduli(1e-10, 50000)
+        t_batch = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        s.adjoint(1.0, 0.0, 1e-10, 50000)
+        t_adj = time.perf_counter() - t0
+        t64, t32 = s.profile(20)
+        nsolid = s.nsolid_nodes
+    res["cfg"] = "64^3 sphere pack (r=6, porosity 0.35), alpha=1, beta=0.1, 6 load cases, rtol 1e-10"
+    res["plain_fp64_solves_per_s"] = 6 / t_plain
+    res["plain_fp64_iters"] = its
+    res["batched_solves_per_s"] = 6 / t_batch
+    res["speedup_vs_plain_fp64"] = t_plain / t_batch
+    res["grads_per_s"] = 1.0 / t_adj
+    res["K_rel_err_vs_fp64"] = abs(K - K64) / abs(K64)
+    res["G_rel_err_vs_fp64"] = abs(G - G64) / abs(G64)
+    res["K"], res["G"] = K, G
+    res["matvec_ms"] = {"fp64": t64, "fp32": t32, "solid_nodes": nsolid}
+    if not quick:
+        rng = np.random.default_rng(0)
+        sizes = rng.integers(32, 97, size=64)
+        vols = [geo.two_component(geo.sphere_pack((int(L), int(L), int(L)), 0.35, 5.0, seed=100 + i), 0.7,
+                                  seed=200 + i, radius=4.0) for i, L in enumerate(sizes)]
+        t0 = time.perf_counter()
+        Cb, KG, ms, it = batch_moduli(vols, 1.0, 0.1, 0.3, 0.02, precision="f32", rtol=1e-10)
+        t_b = time.perf_counter() - t0
+        res["batch"] = {"volumes": len(vols), "sizes": [int(x) for x in sizes], "systems": 6 * len(vols),
+                        "seconds": t_b, "solves_per_s": 6 * len(vols) / t_b, "iterations": it}
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--lsm", dest="lsm", action="store_true", default=True)
+    ap.add_argument("--no-lsm", dest="lsm", action="store_false")
+    ap.add_argument("--no-cpu", dest="cpu", action="store_false", default=True)
+    args = ap.parse_args()
+    ws = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or ws != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}, WORLD_SIZE {ws}): the LBM and LSM "
+              "paths here are single-GPU builds; multi-GPU sharding is a design note, not measured.",
+              file=sys.stderr)
+        sys.exit(2)
+    from porescale_amd import _lib
+    dev, ncu, gb = _lib.device_info()
+    log(f"device {dev}, {ncu} CUs, {gb:.0f} GB")
+    lbm, ind, inp, blue = run_lbm(args.steps, args.warmup)
+    ms_step = lbm["tot_ms"] / args.steps
+    mlups = lbm["nl"] * args.steps / (lbm["tot_ms"] * 1e-3) / 1e6
+    alg_b = 312.0  # per fluid node and step: 2 colours x 19 pops x (read+write) x 4 B + phi r/w 2 x 4 B
+    coll_gbs = alg_b * lbm["nl"] / (lbm["coll_ms"] * 1e-3) / 1e9
+    step_gbs = alg_b * lbm["nl"] / (ms_step * 1e-3) / 1e9
+    log(f"LBM nl={lbm['nl']} {ms_step:.3f} ms/step ({lbm['coll_ms']:.3f} collide + {lbm['dens_ms']:.3f} density)"
+        f" {mlups:.0f} MLUPS, step {step_gbs:.0f} GB/s algorithmic")
+    out = {
+        "metric": "LBM MLUPS (D3Q19 two-phase colour-gradient MRT, fluid nodes)",
+        "value": mlups,
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "storage": "f32 (total deviation + red fraction)",
+        "data": "synthetic (seeded sphere pack and blob phase distribution)",
+        "config": {"workload": "lbm-d3q19-twophase-mrt-256^3", "grid": [256, 256, 256],
+                   "fluid_nodes": lbm["nl"], "porosity": 0.30, "sphere_radii": [4, 10],
+                   "viscosity_ratio": "1:10", "wet": 0.7071, "saturation_blue": 0.5},
+        "hbm_frac_step": step_gbs / HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": "k_collide_f32", "achieved": coll_gbs, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": coll_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_node": alg_b, "nodes_per_launch": lbm["nl"],
+                     "mean_launch_ms": lbm["coll_ms"]},
+        "lbm_diag": {"permr": lbm["permr"], "permb": lbm["permb"], "sat_b": lbm["sat_b"]},
+    }
+    if args.lsm:
+        out["lsm"] = run_lsm()
+        log(f"LSM {json.dumps(out['lsm'])[:400]}")
+    if args.cpu:
+        out["cpu_baseline"] = cpu_baseline(ind, inp, blue)
+        log(f"CPU baseline {out['cpu_baseline']}")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,271 @@
+// FP32-storage variant of the two-phase LBM step (see lbm.hip for the
+// layout).  Populations are stored per (direction, node) as
+//   T = (fr + fb) - tr_q*rr0   and   R = fr / (fr + fb)
+// in FP32; arithmetic runs in FP64 registers.  Compared with the exact path
+// this translation unit allows FMA contraction and replaces the per-node
+// divisions of the recolouring step (L1:3798-3816) and of calcFs by one
+// reciprocal each, which keeps the kernel memory-bound.
+#include "lbm_fast.h"
+
+using namespace psx_lbm_dev;
+
+namespace {
+
+__device__ __forceinline__ void put(const LbmConst &c, float *__restrict__ out, int64_t nl, int q, int64_t j,
+                                    double fr, double fb) {
+    double f = fr + fb;
+    float r;
+    if (fb == 0.0) r = 1.0f;
+    else if (fr == 0.0) r = 0.0f;
+    else {
+        float ff = (float)f;
+        r = ff != 0.0f ? (float)fr * __builtin_amdgcn_rcpf(ff) : 0.5f;
+    }
+    out[(int64_t)q * nl + j] = (float)(f - c.w0[q]);
+    out[(int64_t)(Q + q) * nl + j] = r;
+}
+
+// The collision of L1:2407-2620 with the relaxation vector selected per lane
+// (red: Sr, else Sb) and the force projection Fp_q = rk_q (c_q.bf) dt^2
+// formed in place, updated in place: f[0] becomes the post-collision rest
+// population and f[1..18] the post-collision populations.
+__device__ __forceinline__ void collide_fast(const LbmConst &c, double t1, double t2, double t3, double (&f)[Q],
+                                             double r, const double (&bf)[3], bool red) {
+    const double dt = c.dt, dti = c.dti, dt2 = c.dt2;
+    const double c1d6 = 1.0 / 6.0, c1d12 = 1.0 / 12.0, c5d399 = 5.0 / 399.0,
+                 c19d399 = 19.0 / 399.0, c11d2394 = 11.0 / 2394.0, c1d63 = 1.0 / 63.0,
+                 c1d18 = 1.0 / 18.0, c1d36 = 1.0 / 36.0, c4d1197 = 4.0 / 1197.0,
+                 c1d252 = 1.0 / 252.0, c1d72 = 1.0 / 72.0, c1d24 = 1.0 / 24.0;
+    double j0 = (f[1] - f[2] + f[7] - f[8] + f[9] - f[10] + f[11] - f[12] + f[13] - f[14]) * dti;
+    double j1 = (f[3] - f[4] + f[7] + f[8] - f[9] - f[10] + f[15] - f[16] + f[17] - f[18]) * dti;
+    double j2 = (f[5] - f[6] + f[11] + f[12] - f[13] - f[14] + f[15] + f[16] - f[17] - f[18]) * dti;
+    double dr1 = 1.0 / r;
+    double u0 = (j0 + bf[0] * 0.5 * dt) * dr1;
+    double u1 = (j1 + bf[1] * 0.5 * dt) * dr1;
+    double u2v = (j2 + bf[2] * 0.5 * dt) * dr1;
+    double u1x2 = u0 * u0, u2x2 = u1 * u1, u3x2 = u2v * u2v;
+    double u2 = (u1x2 + u2x2 + u3x2) * 0.5;
+    double fl[Q];
+    fl[0] = f[0] - r * (t1 - u2 * dt2);
+    double j1d6 = j0 * c1d6 * dt, j2d6 = j1 * c1d6 * dt, j3d6 = j2 * c1d6 * dt;
+    double rd4 = r * 0.25 * dt2;
+    double com1 = (t2 - u2 * c1d6 * dt2) * r;
+    double fe1 = com1 + rd4 * u1x2, fe2 = com1 + rd4 * u2x2, fe3 = com1 + rd4 * u3x2;
+    fl[1] = f[1] - (fe1 + j1d6); fl[2] = f[2] - (fe1 - j1d6);
+    fl[3] = f[3] - (fe2 + j2d6); fl[4] = f[4] - (fe2 - j2d6);
+    fl[5] = f[5] - (fe3 + j3d6); fl[6] = f[6] - (fe3 - j3d6);
+    double j1p2 = (j0 + j1) * c1d12 * dt, j1m2 = (j0 - j1) * c1d12 * dt;
+    double j1p3 = (j0 + j2) * c1d12 * dt, j1m3 = (j0 - j2) * c1d12 * dt;
+    double j2p3 = (j1 + j2) * c1d12 * dt, j2m3 = (j1 - j2) * c1d12 * dt;
+    double rd8 = r * 0.125 * dt2;
+    double a12 = rd8 * (u0 + u1) * (u0 + u1), b12 = rd8 * (u0 - u1) * (u0 - u1);
+    double a13 = rd8 * (u0 + u2v) * (u0 + u2v), b13 = rd8 * (u0 - u2v) * (u0 - u2v);
+    double a23 = rd8 * (u1 + u2v) * (u1 + u2v), b23 = rd8 * (u1 - u2v) * (u1 - u2v);
+    double com2 = (t3 - u2 * c1d12 * dt2) * r;
+    fl[7] = f[7] - (com2 + j1p2 + a12);  fl[10] = f[10] - (com2 - j1p2 + a12);
+    fl[8] = f[8] - (com2 - j1m2 + b12);  fl[9] = f[9] - (com2 + j1m2 + b12);
+    fl[11] = f[11] - (com2 + j1p3 + a13); fl[14] = f[14] - (com2 - j1p3 + a13);
+    fl[13] = f[13] - (com2 + j1m3 + b13); fl[12] = f[12] - (com2 - j1m3 + b13);
+    fl[15] = f[15] - (com2 + j2p3 + a23); fl[18] = f[18] - (com2 - j2p3 + a23);
+    fl[17] = f[17] - (com2 + j2m3 + b23); fl[16] = f[16] - (com2 - j2m3 + b23);
+    double com3 = fl[1] + fl[2] + fl[3] + fl[4] + fl[5] + fl[6];
+    double com4 = fl[7] + fl[8] + fl[9] + fl[10] + fl[11] + fl[12] + fl[13] + fl[14] + fl[15] +
+                  fl[16] + fl[17] + fl[18];
+    double com5 = fl[7] + fl[8] + fl[9] + fl[10];
+    double com6 = fl[15] + fl[16] + fl[17] + fl[18];
+    double com7 = fl[11] + fl[12] + fl[13] + fl[14];
+    double com8 = fl[3] + fl[4] - fl[5] - fl[6];
+    double com9 = fl[3] + fl[4] + fl[5] + fl[6];
+#define SEL(q) (red ? c.Sr[q] : c.Sb[q])
+    double m1 = (-30.0 * fl[0] - 11.0 * com3 + 8.0 * com4) * SEL(1);
+    double m2 = (12.0 * fl[0] - 4.0 * com3 + com4) * SEL(2);
+    double m4 = (4.0 * (fl[2] - fl[1]) + fl[7] - fl[8] + fl[9] - fl[10] + fl[11] - fl[12] + fl[13] - fl[14]) * SEL(4);
+    double m6 = (4.0 * (fl[4] - fl[3]) + fl[7] + fl[8] - fl[9] - fl[10] + fl[15] - fl[16] + fl[17] - fl[18]) * SEL(6);
+    double m8 = (4.0 * (fl[6] - fl[5]) + fl[11] + fl[12] - fl[13] - fl[14] + fl[15] + fl[16] - fl[17] - fl[18]) * SEL(8);
+    double m9 = (2.0 * (fl[1] + fl[2] - com6) - com9 + com5 + com7) * SEL(9);
+    double m10 = (-4.0 * (fl[1] + fl[2]) + 2.0 * (com9 - com6) + com5 + com7) * SEL(10);
+    double m11 = (com8 + com5 - com7) * SEL(11);
+    double m12 = (-2.0 * com8 + com5 - com7) * SEL(12);
+    double m13 = (fl[7] - fl[8] - fl[9] + fl[10]) * SEL(13);
+    double m14 = (fl[15] - fl[16] - fl[17] + fl[18]) * SEL(14);
+    double m15 = (fl[11] - fl[12] - fl[13] + fl[14]) * SEL(15);
+    double m16 = (fl[7] - fl[8] + fl[9] - fl[10] - fl[11] + fl[12] - fl[13] + fl[14]) * SEL(16);
+    double m17 = (-fl[7] - fl[8] + fl[9] + fl[10] + fl[15] - fl[16] + fl[17] - fl[18]) * SEL(17);
+    double m18 = (fl[11] + fl[12] - fl[13] - fl[14] - fl[15] - fl[16] + fl[17] + fl[18]) * SEL(18);
+#undef SEL
+    // mo(1), mo(4), mo(6), mo(8) are zero (conserved moments)
+    const double F6 = c1d6 * dt2, F12 = c1d12 * dt2;
+    f[0] = f[0] - (-c5d399 * m1 + c19d399 * m2);
+    double com10 = -c11d2394 * m1 - c1d63 * m2;
+    double mb1 = com10 - 0.1 * m4 + c1d18 * (m9 - m10);
+    double mb3 = com10 - 0.1 * m6 + c1d36 * (m10 - m9) + c1d12 * (m11 - m12);
+    double mb5 = com10 - 0.1 * m8 + c1d36 * (m10 - m9) + c1d12 * (m12 - m11);
+    f[1] = f[1] - mb1 + F6 * bf[0];
+    f[2] = f[2] - (mb1 + m4 * 0.2) - F6 * bf[0];
+    f[3] = f[3] - mb3 + F6 * bf[1];
+    f[4] = f[4] - (mb3 + m6 * 0.2) - F6 * bf[1];
+    f[5] = f[5] - mb5 + F6 * bf[2];
+    f[6] = f[6] - (mb5 + m8 * 0.2) - F6 * bf[2];
+    double com11 = c4d1197 * m1 + c1d252 * m2;
+    double mb7 = com11 + (m4 + m6) * 0.025 + c1d36 * m9 + c1d72 * m10 + c1d12 * m11 + c1d24 * m12 + m13 * 0.25 +
+                 (m16 - m17) * 0.125;
+    double mb8 = mb7 - 0.05 * m4 - 0.5 * m13 - 0.25 * m16;
+    double mb9 = mb7 + 0.25 * m17 - 0.05 * m6 - 0.5 * m13;
+    double mb10 = mb9 + 0.5 * m13 - 0.05 * m4 - 0.25 * m16;
+    double mb11 = com11 + (m4 + m8) * 0.025 + c1d36 * m9 + c1d72 * m10 - c1d12 * m11 - c1d24 * m12 + m15 * 0.25 +
+                  (m18 - m16) * 0.125;
+    double mb12 = mb11 + 0.25 * m16 - 0.5 * m15 - 0.05 * m4;
+    double mb13 = mb11 - 0.05 * m8 - 0.5 * m15 - 0.25 * m18;
+    double mb14 = mb12 + 0.5 * m15 - 0.05 * m8 - 0.25 * m18;
+    double mb15 = com11 + (m6 + m8) * 0.025 - c1d18 * m9 - c1d36 * m10 + m14 * 0.25 + (m17 - m18) * 0.125;
+    double mb16 = mb15 - 0.05 * m6 - 0.5 * m14 - 0.25 * m17;
+    double mb17 = mb15 + 0.25 * m18 - 0.05 * m8 - 0.5 * m14;
+    double mb18 = mb17 + 0.5 * m14 - 0.05 * m6 - 0.25 * m17;
+    f[7] = f[7] - mb7 + F12 * (bf[0] + bf[1]);
+    f[10] = f[10] - mb10 - F12 * (bf[0] + bf[1]);
+    f[8] = f[8] - mb8 + F12 * (-bf[0] + bf[1]);
+    f[9] = f[9] - mb9 - F12 * (-bf[0] + bf[1]);
+    f[11] = f[11] - mb11 + F12 * (bf[0] + bf[2]);
+    f[14] = f[14] - mb14 - F12 * (bf[0] + bf[2]);
+    f[12] = f[12] - mb12 + F12 * (-bf[0] + bf[2]);
+    f[13] = f[13] - mb13 - F12 * (-bf[0] + bf[2]);
+    f[15] = f[15] - mb15 + F12 * (bf[1] + bf[2]);
+    f[18] = f[18] - mb18 - F12 * (bf[1] + bf[2]);
+    f[16] = f[16] - mb16 + F12 * (-bf[1] + bf[2]);
+    f[17] = f[17] - mb17 - F12 * (-bf[1] + bf[2]);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_collide_f32(const LbmConst c, const float *__restrict__ in,
+                                                       float *__restrict__ out, const int32_t *__restrict__ dst,
+                                                       const int32_t *__restrict__ vox,
+                                                       const double *__restrict__ phi, int xcd) {
+    int64_t b = xcd ? psx::xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+    int64_t i = b * blockDim.x + threadIdx.x;
+    const int64_t nl = c.nl;
+    if (i >= nl) return;
+    double f[Q];
+    double rr = 0.0, rb = 0.0;
+    bool allred = true, allblue = true;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        double t = c.w0[q] + (double)in[(int64_t)q * nl + i];
+        float r = in[(int64_t)(Q + q) * nl + i];
+        double a = (double)r * t;
+        f[q] = t;
+        rr += a;
+        rb += t - a;
+        allred &= r == 1.0f;
+        allblue &= r == 0.0f;
+    }
+    const double dt2 = c.dt2;
+    double bf[3];
+    double t1, t2, t3, r;
+    int mode;  // 0 pure red, 1 pure blue, 2 interface (L1:3701-3721)
+    bool red;
+    double Cg[3] = {0.0, 0.0, 0.0};
+    if (rb <= c.ee) {
+        mode = 0; red = true;
+        t1 = c.tr[0]; t2 = c.tr[1]; t3 = c.tr[7]; r = rr;
+        if (!allred) {  // the colour split of a node that is not single-coloured
+#pragma unroll
+            for (int q = 0; q < Q; q++) f[q] = (double)in[(int64_t)(Q + q) * nl + i] * f[q];
+        }
+        bf[0] = c.bfr[0]; bf[1] = c.bfr[1]; bf[2] = c.bfr[2];
+    } else if (rr <= c.ee) {
+        mode = 1; red = false;
+        t1 = c.tb[0]; t2 = c.tb[1]; t3 = c.tb[7]; r = rb;
+        if (!allblue) {
+#pragma unroll
+            for (int q = 0; q < Q; q++) f[q] = f[q] - (double)in[(int64_t)(Q + q) * nl + i] * f[q];
+        }
+        bf[0] = c.bfb[0]; bf[1] = c.bfb[1]; bf[2] = c.bfb[2];
+    } else {
+        mode = 2;
+        int x, y, z;
+        vox_xyz(c, vox[i], x, y, z);
+        double Fs[3];
+        calc_fs<double, false>(c, phi, x, y, z, Cg, Fs);
+        r = rr + rb;
+        double ci2 = (c.cr2 * rr + c.cb2 * rb) / r;
+        t1 = 1.0 - 2.0 * ci2 * dt2;
+        t2 = ci2 * (1.0 / 6.0) * dt2;
+        t3 = ci2 * (1.0 / 12.0) * dt2;
+        red = rr >= rb;
+#pragma unroll
+        for (int d = 0; d < 3; d++) bf[d] = (red ? c.bfr[d] : c.bfb[d]) + Fs[d];
+    }
+    collide_fast(c, t1, t2, t3, f, r, bf, red);
+    double ar = 0.0, ab = 0.0;
+    if (mode == 2) {
+        double inv = 1.0 / r;
+        ar = rr * inv;
+        ab = rb * inv;
+    }
+    double o_r, o_b;
+    if (mode == 0) { o_r = f[0]; o_b = c.tb[0] * rb; }
+    else if (mode == 1) { o_b = f[0]; o_r = c.tr[0] * rr; }
+    else { o_r = f[0] * ar; o_b = f[0] * ab; }
+    put(c, out, nl, 0, i, o_r, o_b);
+    const double cn = 0.70710678118654752440;
+#pragma unroll
+    for (int q = 1; q < Q; q++) {
+        if (mode == 0) { o_r = f[q]; o_b = c.tb[q] * rb; }
+        else if (mode == 1) { o_b = f[q]; o_r = c.tr[q] * rr; }
+        else {
+            // cos of the angle between the colour gradient and c_q (L1:3786-3803)
+            const int cx = q == 1 ? 1 : q == 2 ? -1 : q == 7 || q == 9 || q == 11 || q == 13 ? 1
+                         : q == 8 || q == 10 || q == 12 || q == 14 ? -1 : 0;
+            const int cy = q == 3 ? 1 : q == 4 ? -1 : q == 7 || q == 8 || q == 15 || q == 17 ? 1
+                         : q == 9 || q == 10 || q == 16 || q == 18 ? -1 : 0;
+            const int cz = q == 5 ? 1 : q == 6 ? -1 : q == 11 || q == 12 || q == 15 || q == 16 ? 1
+                         : q == 13 || q == 14 || q == 17 || q == 18 ? -1 : 0;
+            double cs = cx * Cg[0] + cy * Cg[1] + cz * Cg[2];
+            if (q >= 7) cs *= cn;
+            double tw = (q < 7 ? t2 : t3) * c.B * cs;
+            o_r = (f[q] + tw * rb) * ar;
+            o_b = (f[q] - tw * rr) * ab;
+        }
+        int32_t d = dst[(int64_t)(q - 1) * nl + i];
+        if (d >= 0) put(c, out, nl, q, d, o_r, o_b);
+        else put(c, out, nl, opp(q), i, o_r, o_b);
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_density_f32(const LbmConst c, const float *__restrict__ in,
+                                                       const int32_t *__restrict__ vox, double *__restrict__ phi,
+                                                       int xcd) {
+    int64_t b = xcd ? psx::xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+    int64_t i = b * blockDim.x + threadIdx.x;
+    const int64_t nl = c.nl;
+    if (i >= nl) return;
+    double rr = 0.0, rb = 0.0;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        double t = c.w0[q] + (double)in[(int64_t)q * nl + i];
+        double a = (double)in[(int64_t)(Q + q) * nl + i] * t;
+        rr += a;
+        rb += t - a;
+    }
+    phi[vox[i]] = (rr - rb) / (rr + rb);
+}
+
+}  // namespace
+
+namespace psx_lbm_fast {
+
+hipError_t collide(const LbmConst &c, const float *in, float *out, const int32_t *dst, const int32_t *vox,
+                   const double *phi, int xcd, hipStream_t st) {
+    int64_t nb = psx::ceil_div(c.nl, BLOCK);
+    if (nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_collide_f32, dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, out, dst, vox, phi, xcd);
+    return hipGetLastError();
+}
+
+hipError_t density(const LbmConst &c, const float *in, const int32_t *vox, double *phi, int xcd, hipStream_t st) {
+    int64_t nb = psx::ceil_div(c.nl, BLOCK);
+    if (nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_density_f32, dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, vox, phi, xcd);
+    return hipGetLastError();
+}
+
+}  // namespace psx_lbm_fast
