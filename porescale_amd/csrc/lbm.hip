@@ -8,17 +8,20 @@
 // (L1:2898-3296).
 //
 // Layout in HBM (nl fluid nodes, reference node order: x outer, z fastest):
-//   populations  SoA, direction-major: pop[q][node].  FP64 mode holds fr and
-//                fb (2*19 doubles per node).  FP32 mode holds, per (q,node),
-//                T = (fr+fb) - tr_q*rr0 and R = fr/(fr+fb) as floats: T keeps
-//                the small deviation of the total population (momentum is
-//                exact to FP32 ulps of the deviation) and R is exactly 1 or 0
-//                for single-colour populations, so pure-phase nodes keep
-//                exactly zero minority density as the reference's branch
-//                test rb<=ee requires.
-//   dst[18][nl]  push target per (direction, node): destination node of the
-//                streaming link, or -1 for a bounce-back link (the population
-//                returns to the same node in the opposite direction).
+//   populations  FP64 mode: SoA, direction-major pop[q][node] holding fr and
+//                fb (2*19 doubles per node), streamed by a push through
+//                dst[18][nl] (destination node of each link, -1 for a
+//                bounce-back link: the population returns to the same node in
+//                the opposite direction).
+//                FP32 mode (lbm_fast.hip): per (q,node) T = (fr+fb) - tr_q*rr0
+//                and R = fr/(fr+fb) as floats, post-collision, in AoSoA blocks
+//                of 64 nodes, streamed by a pull through src[18][nl] (source
+//                node of each incoming link, -1 for bounce-back).  T keeps the
+//                small deviation of the total population (momentum is exact to
+//                FP32 ulps of the deviation) and R is exactly 1 or 0 for
+//                single-colour populations, so pure-phase nodes keep exactly
+//                zero minority density as the reference's branch test rb<=ee
+//                requires.
 //   vox[nl]      voxel index of each node in the phi grid.
 //   phi grid     dense nx*ny*nz FP64, z fastest (index z + nz*(y + ny*x)):
 //                colour field cf at fluid voxels, `wet` at solid voxels, so
@@ -32,6 +35,7 @@
 #include <algorithm>
 #include <cmath>
 #include <type_traits>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <stdexcept>
@@ -50,7 +54,7 @@ struct psx_lbm {
     int nx = 0, ny = 0, nz = 0;
     int64_t nl = 0, nps = 0, npb2 = 0, npb4 = 0, ns = 0;
     std::vector<int32_t> ind_raw, icor, mijk;
-    std::vector<int32_t> dst_h;  // [18][nl]
+    std::vector<int32_t> dst_h;  // [18][nl]: push destinations (FP64) or pull sources (FP32)
     LbmConst c{};
     hipStream_t stream = nullptr;
     int32_t *d_dst = nullptr, *d_vox = nullptr;
@@ -175,8 +179,8 @@ int launch_density(psx_lbm *h, int buf) {
     int64_t nb = psx::ceil_div(h->nl, 256);
     if (nb == 0) return PSX_OK;
     if (h->f32) {
-        PSX_HIP(psx_lbm_fast::density(h->c, (const float *)h->d_pop[buf], h->d_vox, (double *)h->d_phi, h->xcd,
-                                      h->stream));
+        PSX_HIP(psx_lbm_fast::density(h->c, (const float *)h->d_pop[buf], h->d_dst, h->d_vox, (double *)h->d_phi,
+                                      h->xcd, h->stream));
     } else {
         StoreF64 s{(double *)h->d_pop[buf], h->nl};
         hipLaunchKernelGGL(k_density<StoreF64>, dim3((unsigned)nb), dim3(256), 0, h->stream, h->c, s,
@@ -235,6 +239,7 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
     h->p = *p;
     h->nx = nx; h->ny = ny; h->nz = nz;
     h->f32 = p->precision == PSX_PREC_F32;
+    if (const char *e = std::getenv("PSX_LBM_XCD")) h->xcd = std::atoi(e);  // block->XCD remap on/off
     const int64_t nv = (int64_t)nx * ny * nz;
     h->ind_raw.assign(ind, ind + nv);
     h->mijk.assign(nv, 0);
@@ -264,9 +269,19 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
             }
         }
     }
+    if (h->f32) {  // pull tables: source node of every incoming streaming link
+        std::vector<int32_t> src(18 * (size_t)nl, -1);
+        for (int q = 1; q < Q; q++)
+            for (int64_t n = 0; n < nl; n++) {
+                int32_t d = h->dst_h[(size_t)(q - 1) * nl + n];
+                if (d >= 0) src[(size_t)(q - 1) * nl + d] = (int32_t)n;
+            }
+        h->dst_h.swap(src);  // the FP32 path keeps src in place of dst
+    }
     fill_const(h);
     PSX_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-    size_t popb = (size_t)(nl > 0 ? nl : 1) * 2 * Q * (h->f32 ? sizeof(float) : sizeof(double));
+    size_t popb = h->f32 ? (size_t)aos_pad(nl > 0 ? nl : 1) * 2 * Q * sizeof(float)
+                         : (size_t)(nl > 0 ? nl : 1) * 2 * Q * sizeof(double);
     PSX_HIP(hipMalloc(&h->d_pop[0], popb));
     PSX_HIP(hipMalloc(&h->d_pop[1], popb));
     PSX_HIP(hipMemsetAsync(h->d_pop[0], 0, popb, h->stream));
@@ -329,7 +344,7 @@ extern "C" int psx_lbm_tables(psx_lbm *h, int32_t *icor, int32_t *ips, int32_t *
     for (int64_t n = 0; n < h->nl; n++) {
         const int32_t *ic = &h->icor[3 * n];
         for (int q = 1; q < Q; q++) {
-            int32_t d = h->dst_h[(size_t)(q - 1) * h->nl + n];
+            int32_t d = link_dest(h, ic, q) - 1;
             const int *v = H_IV[q];
             if (d >= 0) {
                 if (ips) { ips[3 * s] = (int32_t)(n + 1); ips[3 * s + 1] = d + 1; ips[3 * s + 2] = q + 1; }
@@ -381,11 +396,20 @@ extern "C" int psx_lbm_set_state(psx_lbm *h, const double *fr, const double *fb)
     const int64_t nl = h->nl;
     h->cur = 0;
     if (h->f32) {
-        std::vector<float> st((size_t)2 * Q * nl);
+        // stored state is post-collision and not yet streamed: place f_q(i)
+        // where the pull of node i reads it (inverse of the streaming step)
+        std::vector<float> st((size_t)2 * Q * aos_pad(nl), 0.0f);
         for (int64_t i = 0; i < nl; i++)
-            for (int q = 0; q < Q; q++)
-                f32_encode(h->c.w0[q], fr[Q * i + q], fb[Q * i + q], st[(size_t)q * nl + i],
-                           st[(size_t)(Q + q) * nl + i]);
+            for (int q = 0; q < Q; q++) {
+                int64_t j = i;
+                int k = q;
+                if (q > 0) {
+                    int32_t sidx = h->dst_h[(size_t)(q - 1) * nl + i];
+                    if (sidx >= 0) j = sidx;
+                    else k = opp(q);
+                }
+                f32_encode(h->c.w0[q], fr[Q * i + q], fb[Q * i + q], st[aos(k, j)], st[aos(Q + k, j)]);
+            }
         PSX_HIP(hipMemcpyAsync(h->d_pop[0], st.data(), sizeof(float) * st.size(), hipMemcpyHostToDevice, h->stream));
         PSX_HIP(hipStreamSynchronize(h->stream));
     } else {
@@ -411,13 +435,20 @@ extern "C" int psx_lbm_get_state(psx_lbm *h, double *fr, double *fb) {
     PSX_GUARD_BEGIN
     const int64_t nl = h->nl;
     if (h->f32) {
-        std::vector<float> st((size_t)2 * Q * nl);
+        std::vector<float> st((size_t)2 * Q * aos_pad(nl));
         PSX_HIP(hipMemcpyAsync(st.data(), h->d_pop[h->cur], sizeof(float) * st.size(), hipMemcpyDeviceToHost, h->stream));
         PSX_HIP(hipStreamSynchronize(h->stream));
         for (int64_t i = 0; i < nl; i++)
             for (int q = 0; q < Q; q++) {
-                double f = h->c.w0[q] + (double)st[(size_t)q * nl + i];
-                double r = (double)st[(size_t)(Q + q) * nl + i];
+                int64_t j = i;
+                int k = q;
+                if (q > 0) {
+                    int32_t sidx = h->dst_h[(size_t)(q - 1) * nl + i];
+                    if (sidx >= 0) j = sidx;
+                    else k = opp(q);
+                }
+                double f = h->c.w0[q] + (double)st[aos(k, j)];
+                double r = (double)st[aos(Q + k, j)];
                 fr[Q * i + q] = r * f;
                 fb[Q * i + q] = f - r * f;
             }
@@ -531,9 +562,8 @@ extern "C" int psx_lbm_check_state(psx_lbm *h, int64_t it1, int64_t itw, psx_lbm
     int64_t nb = psx::ceil_div(h->nl, 256);
     if (nb > 0) {
         if (h->f32) {
-            StoreF32 st{(float *)h->d_pop[h->cur], h->nl};
-            hipLaunchKernelGGL(k_diag<StoreF32>, dim3((unsigned)nb), dim3(256), 0, h->stream, h->c, st, h->d_vox,
-                               (const double *)h->d_phi, h->d_partial);
+            PSX_HIP(psx_lbm_fast::diag(h->c, (const float *)h->d_pop[h->cur], h->d_dst, h->d_vox,
+                                       (const double *)h->d_phi, h->d_partial, h->stream));
         } else {
             StoreF64 st{(double *)h->d_pop[h->cur], h->nl};
             hipLaunchKernelGGL(k_diag<StoreF64>, dim3((unsigned)nb), dim3(256), 0, h->stream, h->c, st, h->d_vox,

@@ -69,10 +69,40 @@ struct StoreF32 {
     }
 };
 
+// FP32 pull layout: post-collision populations in AoSoA blocks of 64 nodes,
+// block-major: element (slot k, node n) at (n/64)*38*64 + k*64 + n%64, slots
+// 0..18 hold T, 19..37 hold R.  A node pulls direction q from the node src
+// streams from, or from its own opposite slot for a bounce-back link.
+constexpr int AOS = 64;
+__host__ __device__ __forceinline__ int64_t aos(int k, int64_t n) {
+    return (n >> 6) * (2 * Q * AOS) + (int64_t)k * AOS + (n & (AOS - 1));
+}
+__host__ __device__ __forceinline__ int64_t aos_pad(int64_t nl) { return (nl + AOS - 1) / AOS * AOS; }
+
+struct PullF32 {
+    using phi_t = double;
+    const float *__restrict__ p;
+    const int32_t *__restrict__ src;  // [18][nl], -1 = bounce-back
+    int64_t nl;
+    __device__ __forceinline__ void load_c(const LbmConst &c, int q, int64_t i, double &fr, double &fb) const {
+        int64_t j = i;
+        int k = q;
+        if (q > 0) {
+            int32_t s = src[(int64_t)(q - 1) * nl + i];
+            if (s >= 0) j = s;
+            else k = opp(q);
+        }
+        double f = c.w0[q] + (double)p[aos(k, j)];  // w0 is the same for q and opp(q)
+        double r = (double)p[aos(Q + k, j)];
+        fr = r * f;
+        fb = f - fr;
+    }
+};
+
 template <class St>
 __device__ __forceinline__ void load_pop(const St &s, const LbmConst &c, int q, int64_t i, double &fr,
                                          double &fb) {
-    if constexpr (std::is_same<St, StoreF32>::value) s.load_c(c, q, i, fr, fb);
+    if constexpr (std::is_same<St, StoreF32>::value || std::is_same<St, PullF32>::value) s.load_c(c, q, i, fr, fb);
     else s.load(q, i, fr, fb);
 }
 

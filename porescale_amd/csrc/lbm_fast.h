@@ -4,11 +4,15 @@
 
 namespace psx_lbm_fast {
 
-// Collide + push over all nodes: `in` and `out` hold T[19][nl] then R[19][nl].
-hipError_t collide(const psx_lbm_dev::LbmConst &c, const float *in, float *out, const int32_t *dst,
+// Pull + collide over all nodes; `in`/`out` are AoSoA (T, R) blocks, src is
+// [18][nl] (-1 = bounce-back).
+hipError_t collide(const psx_lbm_dev::LbmConst &c, const float *in, float *out, const int32_t *src,
                    const int32_t *vox, const double *phi, int xcd, hipStream_t st);
-// Densities -> colour field phi at every node's voxel.
-hipError_t density(const psx_lbm_dev::LbmConst &c, const float *in, const int32_t *vox, double *phi, int xcd,
-                   hipStream_t st);
+// Pulled densities -> colour field phi at every node's voxel.
+hipError_t density(const psx_lbm_dev::LbmConst &c, const float *in, const int32_t *src, const int32_t *vox,
+                   double *phi, int xcd, hipStream_t st);
+// check_state partial sums (k_diag) of the pulled state.
+hipError_t diag(const psx_lbm_dev::LbmConst &c, const float *in, const int32_t *src, const int32_t *vox,
+                const double *phi, double *partial, hipStream_t st);
 
 }  // namespace psx_lbm_fast

@@ -144,3 +144,28 @@ def test_profile_reports_kernel_times():
         lb.initdistr(0.5)
         tot, tc, td = lb.profile(10)
     assert tot > 0 and tc > 0 and td > 0 and tc + td <= tot * 1.01
+
+
+def test_fp32_state_round_trip_and_first_step():
+    """FP32 storage holds post-collision populations ahead of the pull; set/get
+    must invert each other to storage precision, and one step must match the
+    FP64 path to storage precision."""
+    ind = geo.sphere_pack((14, 12, 10), 0.45, 2.5, seed=13)
+    nz, ny, nx = ind.shape
+    inp = pio.LBMInput(B=0.7, sig=0.02, vir=0.1, vib=0.2, wet=0.3, ncx=nx, ncy=ny, ncz=nz, dt=1.0,
+                       bfr=[0, 1e-5, 0], bfb=[0, 1e-5, 0])
+    state = ob.equilibrium_state(inp, ob.fluid_order_mask(ind, geo.phase_blobs(ind, 0.5, 2, radius=2.0)))
+    rng = np.random.default_rng(0)
+    state = (state[0] * (1 + 1e-3 * rng.standard_normal(state[0].shape)),
+             state[1] * (1 + 1e-3 * rng.standard_normal(state[1].shape)))
+    out = {}
+    for prec in ("f32", "f64"):
+        with LBM(inp, ind, precision=prec) as lb:
+            lb.set_state(*state)
+            back = lb.get_state()
+            lb.step(1)
+            out[prec] = (back, lb.get_state())
+    for a, b in zip(out["f32"][0], state):
+        np.testing.assert_allclose(a, b, rtol=0, atol=2e-8)
+    for a, b in zip(out["f32"][1], out["f64"][1]):
+        np.testing.assert_allclose(a, b, rtol=0, atol=5e-8)
