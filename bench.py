@@ -5,8 +5,8 @@ Headline (the JSON line's metric/value): the D3Q19 two-phase colour-gradient
 LBM step with MRT collision at 256^3 (sphere pack, radii 4..10, porosity
 0.30, viscosity ratio 1:10, wetting 0.7071, blob saturation 0.5), FP32
 population storage with FP64 arithmetic: MLUPS over fluid nodes.  The step
-is one collide+push launch and one density launch; the roofline entry is the
-collide kernel, timed with HIP events on the library's own stream.
+is one pull+collide launch and one pull+density launch; the roofline entry is
+the collide kernel, timed with HIP events on the library's own stream.
 
 Also reported (objects in the same line):
   lsm          64^3 single-solid sphere pack (porosity 0.35), all six load
@@ -16,7 +16,10 @@ Also reported (objects in the same line):
   cpu_baseline the oracle (C restatement of the reference, 1 thread) timed on
                this host over a bounded sample of the same LBM workload.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--lsm/--no-lsm]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-lsm] [--no-cpu]
+    python bench.py --calib-copy MIB   # plain float32 elementwise kernel (torch.neg)
+                                       # over MIB MiB in and out, ten launches:
+                                       # scale factor for FETCH_SIZE/WRITE_SIZE
 """
 from __future__ import annotations
 
@@ -148,6 +151,17 @@ def run_lsm(quick=False):
     return res
 
 
+def calib_copy(mib):
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(10):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    log(f"calib-copy: 10 launches of torch.neg over {mib} MiB float32 in, {mib} MiB out")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +170,7 @@ def main():
     ap.add_argument("--lsm", dest="lsm", action="store_true", default=True)
     ap.add_argument("--no-lsm", dest="lsm", action="store_false")
     ap.add_argument("--no-cpu", dest="cpu", action="store_false", default=True)
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
     ws = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or ws != 1:
@@ -163,6 +178,9 @@ def main():
               "paths here are single-GPU builds; multi-GPU sharding is a design note, not measured.",
               file=sys.stderr)
         sys.exit(2)
+    if args.calib_copy:
+        calib_copy(args.calib_copy)
+        return
     from porescale_amd import _lib
     dev, ncu, gb = _lib.device_info()
     log(f"device {dev}, {ncu} CUs, {gb:.0f} GB")

@@ -53,6 +53,42 @@ def _stamp(solid: np.ndarray, centres: np.ndarray, radii: np.ndarray) -> None:
             solid[vz[m] % nz, vy[m] % ny, vx[m] % nx] = True
 
 
+def _ball_index(shape, c, r: float) -> np.ndarray:
+    """Flat indices of the voxels one ball marks (same rule as ``_stamp``)."""
+    nz, ny, nx = shape
+    ir = int(np.ceil(r))
+    g = np.arange(-ir, ir + 1)
+    dz, dy, dx = np.meshgrid(g, g, g, indexing="ij")
+    base = np.floor(c).astype(np.int64)
+    vx = base[0] + dx.ravel()
+    vy = base[1] + dy.ravel()
+    vz = base[2] + dz.ravel()
+    d2 = (vx + 0.5 - c[0]) ** 2 + (vy + 0.5 - c[1]) ** 2 + (vz + 0.5 - c[2]) ** 2
+    m = d2 <= r * r
+    return np.unique(((vz[m] % nz) * ny + vy[m] % ny) * nx + vx[m] % nx)
+
+
+def _grow_balls(mask: np.ndarray, region: np.ndarray, target: float, seed: int, stream: int,
+                radius: float) -> None:
+    """Add seeded balls (centres from counter stream ``stream``) to ``mask``,
+    one at a time, while fewer than ``target`` voxels of ``region`` are marked."""
+    nz, ny, nx = mask.shape
+    fm, fr = mask.reshape(-1), region.reshape(-1)
+    count = int(np.count_nonzero(mask & region))
+    ctr, chunk = 0, 1024
+    while count < target:
+        u = uniform(seed, stream, 3 * chunk, start=3 * ctr)
+        for b in range(chunk):
+            if count >= target:
+                break
+            c = np.array([u[3 * b] * nx, u[3 * b + 1] * ny, u[3 * b + 2] * nz])
+            idx = _ball_index(mask.shape, c, radius)
+            new = idx[~fm[idx]]
+            fm[new] = True
+            count += int(np.count_nonzero(fr[new]))
+        ctr += chunk
+
+
 def sphere_pack(shape, porosity: float, radius=6.0, seed: int = 0, invert: bool = False,
                 batch: int = 64) -> np.ndarray:
     """Periodic pack of overlapping solid spheres with uniform centres.
@@ -102,12 +138,7 @@ def two_component(ind: np.ndarray, frac2: float, seed: int = 0, radius=4.0) -> n
     if nsolid == 0 or frac2 <= 0:
         return out
     comp2 = np.zeros(ind.shape, dtype=bool)
-    nz, ny, nx = ind.shape
-    ctr = 0
-    while (comp2 & solid).sum() < frac2 * nsolid:
-        u = uniform(seed, 7, 3, start=3 * ctr)
-        ctr += 1
-        _stamp(comp2, np.array([[u[0] * nx, u[1] * ny, u[2] * nz]]), np.array([float(radius)]))
+    _grow_balls(comp2, solid, frac2 * nsolid, seed, 7, float(radius))
     out[comp2 & solid] = 3
     return out
 
@@ -121,12 +152,7 @@ def phase_blobs(ind: np.ndarray, saturation: float, seed: int = 0, radius: float
     blue = np.zeros(ind.shape, dtype=bool)
     if npore == 0 or saturation <= 0:
         return blue
-    nz, ny, nx = ind.shape
-    ctr = 0
-    while (blue & pore).sum() < saturation * npore:
-        u = uniform(seed, 3, 3, start=3 * ctr)
-        ctr += 1
-        _stamp(blue, np.array([[u[0] * nx, u[1] * ny, u[2] * nz]]), np.array([float(radius)]))
+    _grow_balls(blue, pore, saturation * npore, seed, 3, float(radius))
     return blue & pore
 
 
