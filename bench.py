@@ -151,6 +151,20 @@ def run_lsm(quick=False):
     return res
 
 
+def pmc_traffic(kernel, profile="profiles/k_collide_f32_lbm256_mrt.md"):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by a copy of known size)."""
+    try:
+        with open(os.path.join(ROOT, profile)) as f:
+            for line in f:
+                t = line.split()
+                if len(t) == 3 and t[0] == "TRAFFIC_BYTES_PER_LAUNCH" and t[1] == kernel:
+                    return float(t[2]), profile
+    except OSError:
+        pass
+    return None, None
+
+
 def calib_copy(mib):
     import torch
     n = mib * (1 << 20) // 4
@@ -187,6 +201,7 @@ def main():
     lbm, ind, inp, blue = run_lbm(args.steps, args.warmup)
     ms_step = lbm["tot_ms"] / args.steps
     mlups = lbm["nl"] * args.steps / (lbm["tot_ms"] * 1e-3) / 1e6
+    traffic, traffic_src = pmc_traffic("k_collide_f32")
     alg_b = 312.0  # per fluid node and step: 2 colours x 19 pops x (read+write) x 4 B + phi r/w 2 x 4 B
     coll_gbs = alg_b * lbm["nl"] / (lbm["coll_ms"] * 1e-3) / 1e9
     step_gbs = alg_b * lbm["nl"] / (ms_step * 1e-3) / 1e9
@@ -211,7 +226,8 @@ def main():
                    "viscosity_ratio": "1:10", "wet": 0.7071, "saturation_blue": 0.5},
         "hbm_frac_step": step_gbs / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "kernel": "k_collide_f32", "achieved": coll_gbs, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": coll_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": coll_gbs / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "algorithmic_bytes_per_node": alg_b, "nodes_per_launch": lbm["nl"],
                      "mean_launch_ms": lbm["coll_ms"]},
         "lbm_diag": {"permr": lbm["permr"], "permb": lbm["permb"], "sat_b": lbm["sat_b"]},
