@@ -1,0 +1,203 @@
+! Two-phase LBM driver on the GPU library: the LBM part of
+! LBM-LSM/lsm_lbm_static_alpha_beta_multipore.f with its input and output
+! files unchanged.
+!
+! Reads INPUT_PARAMETERS (lines 1-30 as the reference reads them,
+! lsm_lbm_static_alpha_beta_multipore.f:435-469), velocities19 and ind; with
+! icon = 1 restarts from iteration2 (:2845-2850).  Runs the 2 pre-steps and the
+! itm x itw majority-rule steps (:3454-3947), rewrites iteration2 after every
+! itw steps (:3951-3961) and appends one output.m row per check_state
+! (:3274-3282).  settings_19 receives the node and table counts.
+!
+! Optional lines after line 30 (ignored by the reference, which stops reading
+! at line 30) select the GPU options:
+!   31  precision  0 = FP64 populations (bit-identical to the reference),
+!                  1 = FP32 storage with FP64 arithmetic
+!   32  collision  0 = the reference's TRT (collmatrTRT), 1 = TRT with the
+!                  magic parameter of line 33, 2 = MRT (viscous rates from
+!                  vir/vib, free rates 1.19, 1.4, 1.2, 1.4, 1.98)
+!   33  lambda     TRT magic parameter for collision = 1
+program psx_lbm_drv
+  use porescale
+  implicit none
+  type(psx_lbm_params) :: p
+  type(psx_lbm_diag) :: d
+  type(c_ptr) :: h
+  real(8) :: cr2, rr0, rb0, B, sig, vir, vib, wet, eps, sat, ee, lsize
+  real(8) :: bfr(3), bfb(3), Em, nu, rs, dt, dl, lam
+  integer :: idir, itm, itw, its, mtype, icon, iipore, npores, ncx, ncy, ncz
+  integer :: prec, coll, ios, u, it1, itms, j, i
+  integer(c_int32_t), allocatable :: ind(:, :, :)
+  real(8), allocatable :: fr(:, :), fb(:, :)
+  integer(c_int64_t) :: counts(5), nl
+  real(8) :: Sr(19), Sb(19)
+
+  open(newunit=u, file='INPUT_PARAMETERS', status='old', iostat=ios)
+  if (ios /= 0) then
+     write(*, *) 'ERROR! cannot open INPUT_PARAMETERS'
+     stop 1
+  end if
+  read(u, *) cr2
+  read(u, *) rr0
+  read(u, *) rb0
+  read(u, *) B
+  read(u, *) sig
+  read(u, *) vir
+  read(u, *) vib
+  read(u, *) wet
+  read(u, *) idir
+  read(u, *) itm
+  read(u, *) itw
+  read(u, *) its
+  read(u, *) eps
+  read(u, *) mtype
+  read(u, *) icon
+  read(u, *) sat
+  read(u, *) iipore
+  read(u, *) npores
+  read(u, *) ee
+  read(u, *) lsize
+  read(u, *) (bfr(i), i = 1, 3)
+  read(u, *) (bfb(i), i = 1, 3)
+  read(u, *) ncx
+  read(u, *) ncy
+  read(u, *) ncz
+  read(u, *) Em
+  read(u, *) nu
+  read(u, *) rs
+  read(u, *) dt
+  read(u, *) dl
+  prec = 0
+  coll = 0
+  lam = 3.d0 / 16.d0
+  read(u, *, iostat=ios) prec
+  if (ios /= 0) prec = 0
+  if (ios == 0) then
+     read(u, *, iostat=ios) coll
+     if (ios /= 0) coll = 0
+  end if
+  if (ios == 0 .and. coll == 1) then
+     read(u, *, iostat=ios) lam
+     if (ios /= 0) then
+        write(*, *) 'ERROR! collision = 1 needs lambda on line 33'
+        stop 1
+     end if
+  end if
+  close(u)
+  if (mtype /= 0) then
+     write(*, *) 'ERROR! only the majority rule (mtype = 0) runs on the GPU'
+     stop 1
+  end if
+
+  call psx_require_velocities19('velocities19')
+  allocate(ind(ncx, ncy, ncz))
+  call psx_read_ind('ind', ncx, ncy, ncz, ind)
+
+  call psx_lbm_defaults(p)
+  p%cr2 = cr2; p%rr0 = rr0; p%rb0 = rb0; p%B = B; p%sig = sig
+  p%vir = vir; p%vib = vib; p%wet = wet; p%idir = idir; p%iipore = iipore
+  p%ee = ee; p%lsize = lsize; p%bfr = bfr; p%bfb = bfb; p%dt = dt
+  p%precision = merge(PSX_PREC_F32, PSX_PREC_F64, prec == 1)
+  select case (coll)
+  case (0)
+     p%collision = PSX_COLL_REF_TRT
+  case (1)
+     p%collision = PSX_COLL_TRT
+     p%lambda = lam
+  case (2)
+     p%collision = PSX_COLL_MRT
+     call mrt(vir, dt, p%Sr)
+     call mrt(vib, dt, p%Sb)
+  case default
+     write(*, *) 'ERROR! collision must be 0, 1 or 2'
+     stop 1
+  end select
+
+  call psx_check(psx_lbm_create(p, int(ncx, c_int32_t), int(ncy, c_int32_t), int(ncz, c_int32_t), &
+       ind, h), 'psx_lbm_create')
+  call psx_check(psx_lbm_counts(h, counts), 'psx_lbm_counts')
+  nl = counts(1)
+  call psx_check(psx_lbm_rates(h, Sr, Sb), 'psx_lbm_rates')
+  allocate(fr(19, nl), fb(19, nl))
+
+  if (icon == 0) then
+     call psx_check(psx_lbm_init_random(h, sat), 'psx_lbm_init_random')
+     itms = 1
+     open(newunit=u, file='output.m', status='replace')
+     write(u, *) '% 1 - iteration'
+     write(u, *) '% 2 - red  permeability'
+     write(u, *) '% 3 - blue permeability'
+     write(u, *) '% 4 - red capillary number, volume average'
+     write(u, *) '% 5 - blue capillary number, volume average'
+     write(u, *) '% 6 - red capillary number, phase average'
+     write(u, *) '% 7 - blue capillary number, phase average'
+     write(u, *) '% 8 - capillary pressure P_r - P_b, phase average'
+     write(u, *) '% 9 - capillary pressure P_red, phase average'
+     write(u, *) '% 10 - capillary pressure P_blue, phase average'
+     write(u, *) '% 11 - saturation red = number of red/number'
+     write(u, *) '% 12 - saturation blue = number of blue/number'
+     write(u, *) 'out = ['
+     close(u)
+  else
+     open(newunit=u, file='iteration2', status='old', iostat=ios)
+     if (ios /= 0) then
+        write(*, *) 'ERROR! icon = 1 needs iteration2'
+        stop 1
+     end if
+     read(u, *) itms
+     read(u, *) ((fr(j, i), j = 1, 19), i = 1, nl)
+     read(u, *) ((fb(j, i), j = 1, 19), i = 1, nl)
+     close(u)
+     itms = itms + 1
+     call psx_check(psx_lbm_set_state(h, fr, fb), 'psx_lbm_set_state')
+  end if
+
+  open(newunit=u, file='settings_19', status='replace')
+  write(u, *) 'LBM settings (GPU library)'
+  write(u, *) 'iipore = ', iipore
+  write(u, *) 'npores = ', npores
+  write(u, *) 'surface tension parameter sig = ', sig
+  write(u, *) 'wettability parameter wet = ', wet
+  write(u, *) 'red fluid collision vector Sr = ', Sr
+  write(u, *) 'blue fluid collision vector Sb = ', Sb
+  write(u, *) 'number of fluid nodes nl = ', counts(1)
+  write(u, *) 'streaming links nps = ', counts(2)
+  write(u, *) 'edge bounce-back links npb2 = ', counts(3)
+  write(u, *) 'face bounce-back links npb4 = ', counts(4)
+  write(u, *) 'precision = ', merge('FP32', 'FP64', prec == 1)
+  close(u)
+
+  ! two pre-steps (lsm_lbm_static_alpha_beta_multipore.f:3454-3621)
+  call psx_check(psx_lbm_step(h, 2_c_int64_t), 'psx_lbm_step')
+  do it1 = itms, itm
+     call psx_check(psx_lbm_step(h, int(itw, c_int64_t)), 'psx_lbm_step')
+     call psx_check(psx_lbm_get_state(h, fr, fb), 'psx_lbm_get_state')
+     open(newunit=u, file='iteration2', status='replace')
+     write(u, *) it1
+     write(u, *) ((fr(j, i), j = 1, 19), i = 1, nl)
+     write(u, *) ((fb(j, i), j = 1, 19), i = 1, nl)
+     close(u)
+     call psx_check(psx_lbm_check_state(h, int(it1, c_int64_t), int(itw, c_int64_t), d), &
+          'psx_lbm_check_state')
+     if (d%isneg /= 0) write(*, *) 'ATTENTION!!!, there are negative populations'
+     open(newunit=u, file='output.m', position='append')
+     write(u, 1984) it1 * itw, d%outm(2:12)
+     close(u)
+  end do
+1984 format(1X, I9, 11(1X, E15.8), 1X, ';')
+  open(newunit=u, file='output.m', position='append')
+  write(u, *) '];'
+  close(u)
+  call psx_lbm_destroy(h)
+
+contains
+
+  subroutine mrt(visc, dt_, S)
+    real(8), intent(in) :: visc, dt_
+    real(c_double), intent(out) :: S(19)
+    real(8) :: sv
+    sv = 1.d0 / (3.d0 * visc * dt_ + 0.5d0)
+    S = [0.d0, 1.19d0, 1.4d0, 0.d0, 1.2d0, 0.d0, 1.2d0, 0.d0, 1.2d0, sv, 1.4d0, sv, 1.4d0, &
+         sv, sv, sv, 1.98d0, 1.98d0, 1.98d0]
+  end subroutine mrt
+end program psx_lbm_drv

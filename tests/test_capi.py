@@ -69,3 +69,13 @@ def test_argument_validation_precedes_device_use():
     q.rs, q.dt, q.precision = 1.0, 0.01, 9
     rc = L.psx_lsm_create(C.byref(q), 4, 4, 4, _lib.ptr(ind), None, None, None, None, None, C.byref(h))
     assert rc == _lib.PSX_EINVAL and b"precision" in L.psx_last_error()
+
+
+def test_fortran_drivers_built_and_report_missing_input(tmp_path):
+    import subprocess
+    fort = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "porescale_amd", "fortran")
+    for name in ("psx_lbm_drv", "psx_lsm_drv"):
+        exe = os.path.join(fort, name)
+        assert os.path.exists(exe), f"{name} not built"
+        r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+        assert r.returncode != 0 and "INPUT_PARAMETERS" in r.stdout
