@@ -13,7 +13,7 @@
  * Every function cites the reference lines it follows.  Arrays use the
  * reference's Fortran layout (column-major, 1-based ids stored in tables).
  *
- * Pinning (see tests/test_oracle_*.py):
+ * Pinning (tests/test_oracle.py, tests/test_element.py):
  *   - LBM tables, state after N steps and check_state values: bit-exact
  *     against the flang-compiled L1 binary (oracle/_ref) on seeded inputs.
  *   - LSM tables (counts), Verlet trajectories and stress moduli: against the

@@ -38,19 +38,50 @@ class LSMModuli(torch.autograd.Function):
         return gab, gs, None, None, None, None
 
 
-def calibrate(ind, ab0, target_KG, steps=50, lr=0.2, gstrain=0.01, rtol=1e-10, precision="f64"):
-    """Fits (alf1, bet1, alf2, bet2) to target (K, G) by gradient descent on
-    log-parameters; returns (ab history (steps+1, 4), loss history)."""
-    t = torch.tensor(target_KG, dtype=torch.float64)
-    logab = torch.tensor(np.log(np.asarray(ab0, float)), dtype=torch.float64, requires_grad=True)
-    opt = torch.optim.Adam([logab], lr=lr)
-    hist, losses = [logab.detach().exp().numpy().copy()], []
+def calibrate(ind, ab0, target_KG, steps=50, lr=0.2, gstrain=0.01, rtol=1e-10, precision="f64",
+              method="gauss-newton", stiff=None):
+    """Fits (alf1, bet1, alf2, bet2) to target (K, G) in log-parameters.
+
+    method "gauss-newton": each step solves the six load cases, then two
+    adjoint solves give the 2x4 Jacobian of (K, G); the step is the
+    minimum-norm Gauss-Newton step (length-capped at 0.5 in log space).
+    method "adam": first-order steps through ``LSMModuli`` (torch.autograd)
+    on the relative squared error.
+    Returns (ab history (steps+1, 4), loss history (steps,)), loss being the
+    relative squared error at the parameters before each step.
+    """
+    t = np.asarray(target_KG, float)
+    logab = np.log(np.asarray(ab0, float))
+    hist, losses = [np.exp(logab)], []
+    if method == "adam":
+        tt = torch.tensor(t, dtype=torch.float64)
+        x = torch.tensor(logab, dtype=torch.float64, requires_grad=True)
+        opt = torch.optim.Adam([x], lr=lr)
+        st = None if stiff is None else torch.as_tensor(stiff, dtype=torch.float64)
+        for _ in range(steps):
+            opt.zero_grad()
+            KG = LSMModuli.apply(x.exp(), st, ind, gstrain, rtol, precision)
+            loss = (((KG - tt) / tt) ** 2).sum()
+            loss.backward()
+            opt.step()
+            losses.append(float(loss))
+            hist.append(x.detach().exp().numpy().copy())
+        return np.array(hist), np.array(losses)
+    if method != "gauss-newton":
+        raise ValueError(f"unknown method {method!r}")
     for _ in range(steps):
-        opt.zero_grad()
-        KG = LSMModuli.apply(logab.exp(), None, ind, gstrain, rtol, precision)
-        loss = (((KG - t) / t) ** 2).sum()
-        loss.backward()
-        opt.step()
-        losses.append(float(loss))
-        hist.append(logab.detach().exp().numpy().copy())
+        ab = np.exp(logab)
+        with LSM(ind, *ab, gstrain=gstrain, precision=precision, stiff=stiff) as s:
+            _, K, G = s.moduli(rtol=rtol)
+            dK, _ = s.adjoint(1.0, 0.0, rtol=rtol, want_stiff=False)
+            dG, _ = s.adjoint(0.0, 1.0, rtol=rtol, want_stiff=False)
+        r = np.array([K, G]) / t - 1.0
+        losses.append(float(r @ r))
+        J = np.stack([dK / t[0], dG / t[1]]) * ab[None, :]
+        delta = -np.linalg.lstsq(J, r, rcond=None)[0]
+        n = np.linalg.norm(delta)
+        if n > 0.5:
+            delta *= 0.5 / n
+        logab = logab + delta
+        hist.append(np.exp(logab))
     return np.array(hist), np.array(losses)

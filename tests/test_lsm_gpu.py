@@ -166,3 +166,24 @@ def test_matvec_profile():
     with LSM(ind, 1.0, 0.1) as s:
         t64, t32 = s.profile(5)
     assert t64 > 0 and t32 > 0
+
+
+def test_cyclic_axis_permutation_maps_all_six_cases():
+    """The reference ships only the xx and yz programs.  yy, zz, zx and xy are
+    pinned to them by symmetry: relabelling the axes cyclically (x->y->z->x)
+    maps load case xx->yy->zz, yz->zx->xy and the stress rows the same way,
+    so C'[s(I), s(J)] = C[I, J] exactly (to the solver tolerance)."""
+    ind = geo.two_component(geo.sphere_pack((7, 9, 8), 0.35, 2.0, seed=21), 0.4, seed=22, radius=1.5)
+    rot = np.transpose(ind, (1, 2, 0))  # new (z', y', x') = old (y, x, z)
+    _say("case cyclic permutation")
+    with LSM(ind, 1.0, 0.1, 0.3, 0.02) as s:
+        C, K, G = s.moduli(rtol=1e-12, maxit=20000)
+    with LSM(rot, 1.0, 0.1, 0.3, 0.02) as s:
+        Cr, Kr, Gr = s.moduli(rtol=1e-12, maxit=20000)
+    sig = np.array([1, 2, 0, 4, 5, 3])
+    mapped = np.empty_like(C)
+    mapped[np.ix_(sig, sig)] = C
+    np.testing.assert_allclose(Cr, mapped, rtol=0, atol=1e-9 * np.abs(C).max())
+    assert Kr == pytest.approx(K, rel=1e-10) and Gr == pytest.approx(G, rel=1e-10)
+    # a non-trivial geometry: the six cases really differ
+    assert abs(C[0, 0] - C[1, 1]) > 1e-4 * C[0, 0]
