@@ -31,8 +31,9 @@ class LSMModuli(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad):
         g = grad.detach().cpu().double().numpy()
+        # the handle stays open until the graph is freed: backward may run
+        # more than once on one forward (retain_graph, gradcheck)
         dab, ds = ctx.lsm.adjoint(float(g[0]), float(g[1]), rtol=ctx.rtol, want_stiff=ctx.has_stiff)
-        ctx.lsm.close()
         gab = torch.from_numpy(dab)
         gs = torch.from_numpy(ds) if ctx.has_stiff else None
         return gab, gs, None, None, None, None

@@ -188,6 +188,8 @@ class LSM:
                 want_stiff: bool = True):
         """Gradient of wK*K + wG*G: d/d(alf1, bet1, alf2, bet2) and per-voxel
         d/d(stiffness multiplier) (shape of ind)."""
+        if not getattr(self, "_h", None):
+            raise ValueError("LSM handle is closed")
         dab = np.zeros(4)
         ds = np.zeros(self.shape) if want_stiff else None
         _lib.check(self._L.psx_lsm_adjoint(self._h, float(wK), float(wG), float(rtol), int(maxit),
