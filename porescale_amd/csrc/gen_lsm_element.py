@@ -186,6 +186,30 @@ def emit(KA, KB, SA, SB, out):
                 if vb != 0:
                     parts.append(f"T({dbl(vb)}) * kb[{p}]")
             w(f"    D[{e}][{e2}] = " + (" + ".join(parts) if parts else "T(0)") + ";\n")
+    w("}\n\n")
+    # full 27-slot stencil of a node (assembled operator rows, for the
+    # multigrid hierarchy): out[(s*9 + 3e + e2) * stride] = K(node, node+off(s))[e][e2]
+    w("// K(node, node + offset(s))[e][e2] for the 27 offsets s = (dx+1) + 3(dy+1) + 9(dz+1),\n")
+    w("// written to out[(9 s + 3 e + e2) * stride].\n")
+    w("template <typename T>\n__device__ __forceinline__ void node_stencil(const T (&ka)[8], const T (&kb)[8], T *out,\n"
+      "                                             int64_t stride) {\n")
+    for sidx in range(27):
+        off = (sidx % 3 - 1, (sidx // 3) % 3 - 1, sidx // 9 - 1)
+        for e in range(3):
+            for e2 in range(3):
+                parts = []
+                for pp in range(8):
+                    q = [qq for qq in range(8) if all(CORNER[qq][d] - CORNER[pp][d] == off[d] for d in range(3))]
+                    if not q:
+                        continue
+                    q = q[0]
+                    va, vb = KA[3 * pp + e][3 * q + e2], KB[3 * pp + e][3 * q + e2]
+                    if va != 0:
+                        parts.append(f"T({dbl(va)}) * ka[{pp}]")
+                    if vb != 0:
+                        parts.append(f"T({dbl(vb)}) * kb[{pp}]")
+                expr = " + ".join(parts) if parts else "T(0)"
+                w(f"    out[{9 * sidx + 3 * e + e2} * stride] = {expr};\n")
     w("}\n\n} // namespace lsm_el\n")
 
 

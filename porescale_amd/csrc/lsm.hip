@@ -32,109 +32,20 @@
 // system (volume, load case), so one launch serves the whole batch.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <stdexcept>
 #include <vector>
 
 #include "psx_common.h"
+#include "lsm_common.h"
 #include "lsm_element.h"
+#include "lsm_mg.h"
 
 namespace {
-
-constexpr int TILE = 256;
-constexpr int MAXR = 6;  // right-hand sides per volume (load cases)
-
-struct VolDesc {
-    int32_t nx, ny, nz, pad_;
-    int64_t off;      // node / voxel offset of the volume
-    int32_t tile0;    // first tile of the volume
-    int32_t ntile;
-};
-
-struct Jump {
-    double J[MAXR][3][3];  // [rhs][crossed axis][component]
-};
-
-// per-system scalars (system s = volume*R + rhs)
-struct SysState {
-    double rz, pq, alpha, beta, rr, bb, scale;
-    double bref;  // squared norm of the rhs summed without cancellation (0 = unused):
-                  // a residual below 1e-14 of it is rounding noise and counts as converged
-    int32_t active, iters, conv, pad2_;
-};
-constexpr double NOISE2 = 1e-28;
-
-__device__ __forceinline__ int wrapm(int t, int n, int &w) {
-    w = t < 0 ? -1 : (t >= n ? 1 : 0);
-    return t < 0 ? t + n : (t >= n ? t - n : t);
-}
-
-// Gathers the 27-node neighbourhood of node (x,y,z) of one right-hand side.
-template <typename T, bool JUMP>
-__device__ __forceinline__ void gather27(const VolDesc &vd, const T *__restrict__ u, int64_t Ntot, int x, int y,
-                                         int z, const double (&J)[3][3], T (&out)[27][3]) {
-#pragma unroll
-    for (int dz = -1; dz <= 1; dz++) {
-        int wz, zz = wrapm(z + dz, vd.nz, wz);
-#pragma unroll
-        for (int dy = -1; dy <= 1; dy++) {
-            int wy, yy = wrapm(y + dy, vd.ny, wy);
-#pragma unroll
-            for (int dx = -1; dx <= 1; dx++) {
-                int wx, xx = wrapm(x + dx, vd.nx, wx);
-                int64_t idx = vd.off + xx + (int64_t)vd.nx * (yy + (int64_t)vd.ny * zz);
-                int s = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
-#pragma unroll
-                for (int e = 0; e < 3; e++) {
-                    T v = u[(int64_t)e * Ntot + idx];
-                    if (JUMP) v += (T)(wx * J[0][e] + wy * J[1][e] + wz * J[2][e]);
-                    out[s][e] = v;
-                }
-            }
-        }
-    }
-}
-
-// Scales of the 8 voxels around node (x,y,z): ka[p] for voxel node - CORNER[p].
-template <typename T>
-__device__ __forceinline__ bool gather_k(const VolDesc &vd, const T *__restrict__ ka, const T *__restrict__ kb,
-                                         int x, int y, int z, T (&a)[8], T (&b)[8]) {
-    bool any = false;
-#pragma unroll
-    for (int p = 0; p < 8; p++) {
-        int w, xx = wrapm(x - (p & 1), vd.nx, w), yy = wrapm(y - ((p >> 1) & 1), vd.ny, w),
-               zz = wrapm(z - (p >> 2), vd.nz, w);
-        int64_t v = vd.off + xx + (int64_t)vd.nx * (yy + (int64_t)vd.ny * zz);
-        a[p] = ka[v];
-        b[p] = kb[v];
-        any |= (a[p] != T(0)) || (b[p] != T(0));
-    }
-    return any;
-}
-
-template <typename T>
-__device__ __forceinline__ T block_sum(T v, T *sh) {
-    v = psx::wave_sum(v);
-    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    __syncthreads();
-    if (lane == 0) sh[w] = v;
-    __syncthreads();
-    T s = T(0);
-    if (threadIdx.x == 0)
-        for (int k = 0; k < (int)(blockDim.x >> 6); k++) s += sh[k];
-    return s;  // valid in thread 0
-}
-
-__device__ __forceinline__ void tile_node(const VolDesc &vd, int start, int &x, int &y, int &z, bool &ok) {
-    int n = start + (int)threadIdx.x;
-    int64_t N = (int64_t)vd.nx * vd.ny * vd.nz;
-    ok = n < N;
-    if (!ok) n = 0;
-    x = n % vd.nx;
-    int t = n / vd.nx;
-    y = t % vd.ny;
-    z = t / vd.ny;
-}
+using namespace lsmk;
 
 // ------------------------------------------------------------- kernels ----
 // y = mode ? F(u) = -K(u + jump) : K u ;  optional dot partial p.y per (tile, rhs)
@@ -270,14 +181,15 @@ __global__ __launch_bounds__(256) void k_sys_reduce(const VolDesc *__restrict__ 
     if (!st.active) return;
     if (mode == 0) {
         st.pq = sa;
-        st.alpha = sa != 0.0 ? st.rz / sa : 0.0;
+        st.alpha = sa > 0.0 ? st.rz / sa : 0.0;
+        if (!(sa > 0.0)) st.active = 0;  // no positive curvature left along p: stop this system
     } else {
         st.beta = st.rz != 0.0 ? sa / st.rz : 0.0;
         st.rz = sa;
         st.rr = sb;
         st.iters += 1;
         if (sb <= tol2 * st.bb || sb <= NOISE2 * st.bref) { st.conv = 1; st.active = 0; }
-        else if (st.iters >= maxit || !(sb == sb)) st.active = 0;
+        else if (st.iters >= maxit || !(sb == sb) || sb > 1e12 * st.bb) st.active = 0;
     }
 }
 
@@ -317,14 +229,17 @@ __global__ __launch_bounds__(TILE) void k_cg_update(const VolDesc *__restrict__ 
                     }
                     rv[e] = Rv[k];
                 }
-                precond(minv, Ntot, n, rv, zv);
+                if (minv) {  // block Jacobi; without minv the caller preconditions
+                    precond(minv, Ntot, n, rv, zv);
 #pragma unroll
-                for (int e = 0; e < 3; e++) {
-                    int64_t k = base + (int64_t)e * Ntot + n;
-                    Z[k] = zv[e];
-                    drz += (double)rv[e] * (double)zv[e];
-                    drr += (double)rv[e] * (double)rv[e];
+                    for (int e = 0; e < 3; e++) {
+                        int64_t k = base + (int64_t)e * Ntot + n;
+                        Z[k] = zv[e];
+                        drz += (double)rv[e] * (double)zv[e];
+                    }
                 }
+#pragma unroll
+                for (int e = 0; e < 3; e++) drr += (double)rv[e] * (double)rv[e];
             }
             double s1 = block_sum(drz, sh);
             double s2 = block_sum(drr, sh);
@@ -360,12 +275,16 @@ __global__ __launch_bounds__(TILE) void k_cg_pupdate(const VolDesc *__restrict__
     }
 }
 
-// r64 = b - K x64 for the outer refinement: Q holds K x64 (from k_apply);
-// writes r64, the FP32 scaled copy r32 = r64/scale, zeroes d32 and partial r.r.
+// r64 = b - K x64 for the outer refinement: KX holds K x64 (from k_apply).
+// Partials per (tile, rhs): r.r into prr; into pmean[4] the sums of r over
+// the nodes that carry springs and their count (the rigid translation of
+// those nodes is in the null space of K; its share of r is rounding noise
+// that the inner solve must not see).
 __global__ __launch_bounds__(TILE) void k_refine_residual(
     const VolDesc *__restrict__ vols, const int32_t *__restrict__ tvol, const int32_t *__restrict__ tstart,
-    int ntiles, int R, const double *__restrict__ B, const double *__restrict__ KX, double *__restrict__ R64,
-    int64_t Ntot, double *__restrict__ prr) {
+    int ntiles, int R, const double *__restrict__ ka, const double *__restrict__ kb, const double *__restrict__ B,
+    const double *__restrict__ KX, double *__restrict__ R64, int64_t Ntot, double *__restrict__ prr,
+    double *__restrict__ pmean) {
     __shared__ double sh[TILE / 64];
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const VolDesc vd = vols[tvol[t]];
@@ -373,17 +292,26 @@ __global__ __launch_bounds__(TILE) void k_refine_residual(
         bool ok;
         tile_node(vd, tstart[t], x, y, z, ok);
         int64_t n = vd.off + x + (int64_t)vd.nx * (y + (int64_t)vd.ny * z);
+        double a8[8], b8[8];
+        const bool act = ok && gather_k(vd, ka, kb, x, y, z, a8, b8);
         for (int r = 0; r < R; r++) {
-            double d = 0.0;
+            double d = 0.0, m[3] = {0.0, 0.0, 0.0};
             if (ok)
                 for (int e = 0; e < 3; e++) {
                     int64_t k = (int64_t)r * 3 * Ntot + (int64_t)e * Ntot + n;
                     double v = B[k] - KX[k];
                     R64[k] = v;
                     d += v * v;
+                    if (act) m[e] = v;
                 }
             double s = block_sum(d, sh);
             if (threadIdx.x == 0) prr[(int64_t)t * R + r] = s;
+            for (int e = 0; e < 3; e++) {
+                double se = block_sum(m[e], sh);
+                if (threadIdx.x == 0) pmean[((int64_t)t * R + r) * 4 + e] = se;
+            }
+            double c = block_sum(act ? 1.0 : 0.0, sh);
+            if (threadIdx.x == 0) pmean[((int64_t)t * R + r) * 4 + 3] = c;
         }
     }
 }
@@ -400,10 +328,12 @@ __global__ void k_copy(T *__restrict__ d, const T *__restrict__ s, int64_t n) {
     if (i < n) d[i] = s[i];
 }
 
-// Scales the FP32 residual of each system by 1/||r|| (tracked in SysState.scale)
+// Inner right-hand side: the outer residual without its rigid-translation
+// share (mean over the nodes with springs), scaled by 1/||r||.
 __global__ __launch_bounds__(TILE) void k_scale_sys(const VolDesc *__restrict__ vols, const int32_t *__restrict__ tvol,
                                                     const int32_t *__restrict__ tstart, int ntiles, int R,
-                                                    const SysState *__restrict__ outer, float *__restrict__ R32,
+                                                    const SysState *__restrict__ outer, const double *__restrict__ ka,
+                                                    const double *__restrict__ kb, float *__restrict__ R32,
                                                     float *__restrict__ X32, const double *__restrict__ R64, int64_t Ntot) {
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int v = tvol[t];
@@ -413,12 +343,16 @@ __global__ __launch_bounds__(TILE) void k_scale_sys(const VolDesc *__restrict__ 
         tile_node(vd, tstart[t], x, y, z, ok);
         if (!ok) continue;
         int64_t n = vd.off + x + (int64_t)vd.nx * (y + (int64_t)vd.ny * z);
+        double a8[8], b8[8];
+        const bool act = gather_k(vd, ka, kb, x, y, z, a8, b8);
         for (int r = 0; r < R; r++) {
-            double sc = outer[v * R + r].scale;
+            const SysState &so = outer[v * R + r];
+            double sc = so.scale;
             double is = sc > 0.0 ? 1.0 / sc : 0.0;
             for (int e = 0; e < 3; e++) {
                 int64_t k = (int64_t)r * 3 * Ntot + (int64_t)e * Ntot + n;
-                R32[k] = (float)(R64[k] * is);
+                double val = act ? R64[k] - so.mean[e] : R64[k];
+                R32[k] = (float)(val * is);
                 X32[k] = 0.0f;
             }
         }
@@ -451,17 +385,24 @@ __global__ __launch_bounds__(TILE) void k_unscale_add(const VolDesc *__restrict_
 
 // Outer refinement bookkeeping: rr = ||b - K x||^2 -> converged / scale
 __global__ __launch_bounds__(256) void k_outer_reduce(const VolDesc *__restrict__ vols, int R,
-                                                      const double *__restrict__ prr, SysState *__restrict__ outer,
-                                                      double tol2, int first) {
+                                                      const double *__restrict__ prr, const double *__restrict__ pmean,
+                                                      SysState *__restrict__ outer, double tol2, int first) {
     const int s = blockIdx.x;
     const int v = s / R, r = s % R;
     const VolDesc vd = vols[v];
     __shared__ double sh[4];
-    double a = 0.0;
-    for (int k = threadIdx.x; k < vd.ntile; k += blockDim.x) a += prr[(int64_t)(vd.tile0 + k) * R + r];
+    double a = 0.0, m[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = threadIdx.x; k < vd.ntile; k += blockDim.x) {
+        const int64_t tr = (int64_t)(vd.tile0 + k) * R + r;
+        a += prr[tr];
+        for (int e = 0; e < 4; e++) m[e] += pmean[tr * 4 + e];
+    }
     double sa = block_sum(a, sh);
+    double sm[4];
+    for (int e = 0; e < 4; e++) sm[e] = block_sum(m[e], sh);
     if (threadIdx.x != 0) return;
     SysState &st = outer[s];
+    for (int e = 0; e < 3; e++) st.mean[e] = sm[3] > 0.0 ? sm[e] / sm[3] : 0.0;
     if (first) { st.bb = sa; st.iters = 0; st.conv = 0; }
     st.rr = sa;
     st.scale = sqrt(sa);
@@ -687,22 +628,6 @@ __global__ __launch_bounds__(TILE) void k_adj_voxel(const VolDesc *__restrict__ 
 // =================================================================== host ==
 namespace {
 
-struct Batch {
-    std::vector<VolDesc> vols;
-    std::vector<int32_t> tvol, tstart;
-    int64_t Ntot = 0;
-    int ntiles = 0;
-    VolDesc *d_vols = nullptr;
-    int32_t *d_tvol = nullptr, *d_tstart = nullptr;
-    double *d_ka = nullptr, *d_kb = nullptr;
-    float *d_ka32 = nullptr, *d_kb32 = nullptr;
-    double *d_minv = nullptr;
-    float *d_minv32 = nullptr;
-    std::vector<double> ka_h, kb_h;
-    std::vector<uint8_t> comp_h;
-    std::vector<double> stiff_h;
-};
-
 struct Solver {
     int R = 0;              // rhs per volume
     int S = 0;              // systems
@@ -713,6 +638,7 @@ struct Solver {
     double *prz = nullptr, *prr = nullptr, *ppq = nullptr;
     Jump *d_jump = nullptr;
     Jump jump_h{};
+    MG mg;  // multigrid preconditioner of the FP32 inner solves (built on first use)
 };
 
 }  // namespace
@@ -738,27 +664,6 @@ struct psx_lsm {
 namespace {
 
 const int CASE_PD[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 2}, {2, 0}, {0, 1}};
-
-int grid_for(int ntiles) {
-    int dev = 0, ncu = 256;
-    if (hipGetDevice(&dev) == hipSuccess) {
-        hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
-    }
-    return std::max(1, std::min(ntiles, ncu * 8));
-}
-
-template <typename T>
-int dalloc(T **p, size_t n) {
-    PSX_HIP(hipMalloc((void **)p, sizeof(T) * std::max<size_t>(n, 1)));
-    return PSX_OK;
-}
-
-#define RC(x)                 \
-    do {                      \
-        int rc_ = (x);        \
-        if (rc_) return rc_;  \
-    } while (0)
 
 // Builds the batch description and uploads voxel scales.
 int batch_build(Batch &B, hipStream_t st, const psx_lsm_params &p, int nvol, const int32_t *dims,
@@ -849,6 +754,7 @@ void solver_free(Solver &S) {
                     S.sys, S.outer, S.prz, S.prr, S.ppq, S.d_jump};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    mg_free(S.mg);
     S = Solver();
 }
 
@@ -881,15 +787,22 @@ int build_rhs(hipStream_t st, const Batch &B, Solver &S) {
 }
 
 // One PCG iteration on all active systems (5 launches, no host sync).
+// z = M^-1 r with the multigrid V-cycle (FP32 only), partials of r.z into prz.
+template <typename T>
+void mg_precond(hipStream_t st, const Batch &B, Solver &S, const SysState *sys, const T *Rv, T *Z) {
+    if constexpr (std::is_same<T, float>::value) mg_apply(st, B, S.mg, sys, Rv, Z, S.prz);
+}
+
 template <typename T>
 void cg_iteration(hipStream_t st, const Batch &B, Solver &S, SysState *sys, T *X, T *Rv, T *Z, T *P, T *Qv,
-                  const T *ka, const T *kb, const T *minv, double tol2, int maxit) {
+                  const T *ka, const T *kb, const T *minv, double tol2, int maxit, bool mg) {
     int g = grid_for(B.ntiles);
     launch_apply<T, false, false, true>(st, B, S.R, ka, kb, P, Qv, sys, S.d_jump, S.ppq);
     hipLaunchKernelGGL(k_sys_reduce, dim3(S.S), dim3(256), 0, st, B.d_vols, S.R, 0, S.ppq, (const double *)nullptr,
                        sys, tol2, maxit);
     hipLaunchKernelGGL(k_cg_update<T>, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles, S.R,
-                       sys, X, Rv, Z, P, Qv, minv, B.Ntot, S.prz, S.prr, 0);
+                       sys, X, Rv, Z, P, Qv, mg ? (const T *)nullptr : minv, B.Ntot, S.prz, S.prr, 0);
+    if (mg) mg_precond<T>(st, B, S, sys, Rv, Z);
     hipLaunchKernelGGL(k_sys_reduce, dim3(S.S), dim3(256), 0, st, B.d_vols, S.R, 1, S.prz, S.prr, sys, tol2, maxit);
     hipLaunchKernelGGL(k_cg_pupdate<T>, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles, S.R,
                        sys, P, Z, B.Ntot, 0);
@@ -898,14 +811,16 @@ void cg_iteration(hipStream_t st, const Batch &B, Solver &S, SysState *sys, T *X
 // CG start: x = 0, r = rhs, z = M^-1 r, p = z, scalars.
 template <typename T>
 void cg_start(hipStream_t st, const Batch &B, Solver &S, SysState *sys, T *X, T *Rv, T *Z, T *P, const T *rhs,
-              const T *minv) {
+              const T *minv, bool mg = false) {
     int64_t n = (int64_t)S.vec;
     int gb = (int)psx::ceil_div(n, 256);
     int g = grid_for(B.ntiles);
     hipLaunchKernelGGL(k_zero<T>, dim3(gb), dim3(256), 0, st, X, n);
     if (Rv != rhs) hipLaunchKernelGGL(k_copy<T>, dim3(gb), dim3(256), 0, st, Rv, rhs, n);
     hipLaunchKernelGGL(k_cg_update<T>, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles, S.R,
-                       sys, X, Rv, Z, (const T *)nullptr, (const T *)nullptr, minv, B.Ntot, S.prz, S.prr, 1);
+                       sys, X, Rv, Z, (const T *)nullptr, (const T *)nullptr, mg ? (const T *)nullptr : minv, B.Ntot,
+                       S.prz, S.prr, 1);
+    if (mg) mg_precond<T>(st, B, S, (const SysState *)nullptr, Rv, Z);
     hipLaunchKernelGGL(k_sys_reduce, dim3(S.S), dim3(256), 0, st, B.d_vols, S.R, 2, S.prz, S.prr, sys, 0.0, 0);
     hipLaunchKernelGGL(k_cg_pupdate<T>, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles, S.R,
                        sys, P, Z, B.Ntot, 1);
@@ -916,14 +831,14 @@ void cg_start(hipStream_t st, const Batch &B, Solver &S, SysState *sys, T *X, T 
 // and replayed.
 template <typename T>
 int cg_run(hipStream_t st, const Batch &B, Solver &S, SysState *sys, T *X, T *Rv, T *Z, T *P, T *Qv, const T *ka,
-           const T *kb, const T *minv, double tol, int maxit, int chunk, int use_graph) {
+           const T *kb, const T *minv, double tol, int maxit, int chunk, int use_graph, bool mg = false) {
     double tol2 = tol * tol;
     std::vector<SysState> hs(S.S);
     hipGraphExec_t exec = nullptr;
     if (use_graph) {
         hipGraph_t graph;
         PSX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < chunk; k++) cg_iteration<T>(st, B, S, sys, X, Rv, Z, P, Qv, ka, kb, minv, tol2, maxit);
+        for (int k = 0; k < chunk; k++) cg_iteration<T>(st, B, S, sys, X, Rv, Z, P, Qv, ka, kb, minv, tol2, maxit, mg);
         PSX_HIP(hipStreamEndCapture(st, &graph));
         PSX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         (void)hipGraphDestroy(graph);
@@ -933,7 +848,7 @@ int cg_run(hipStream_t st, const Batch &B, Solver &S, SysState *sys, T *X, T *Rv
         if (exec) {
             PSX_HIP(hipGraphLaunch(exec, st));
         } else {
-            for (int k = 0; k < chunk; k++) cg_iteration<T>(st, B, S, sys, X, Rv, Z, P, Qv, ka, kb, minv, tol2, maxit);
+            for (int k = 0; k < chunk; k++) cg_iteration<T>(st, B, S, sys, X, Rv, Z, P, Qv, ka, kb, minv, tol2, maxit, mg);
             PSX_HIP(hipGetLastError());
         }
         done_iters += chunk;
@@ -953,6 +868,7 @@ int cg_run(hipStream_t st, const Batch &B, Solver &S, SysState *sys, T *X, T *Rv
 int solve_batch(hipStream_t st, const Batch &B, Solver &S, bool f32, double tol, int maxit, int use_graph,
                 std::vector<SysState> *out_state, int *out_iters) {
     const int chunk = 16;
+    if (const char *e = std::getenv("PSX_LSM_GRAPH")) use_graph = std::atoi(e);  // 0: plain launches (profiling)
     int total_iters = 0;
     if (!f32) {
         cg_start<double>(st, B, S, S.sys, S.X, S.Rr, S.Z, S.P, S.B, B.d_minv);
@@ -977,15 +893,19 @@ int solve_batch(hipStream_t st, const Batch &B, Solver &S, bool f32, double tol,
     int g = grid_for(B.ntiles);
     hipLaunchKernelGGL(k_zero<double>, dim3(gb), dim3(256), 0, st, S.X, n);
     const double inner_tol = 1e-5;
+    // multigrid-preconditioned inner solves (PSX_MG=0 selects block Jacobi)
+    const char *mge = std::getenv("PSX_MG");
+    const bool use_mg = !mge || std::atoi(mge) != 0;
+    if (use_mg && !S.mg.built) RC(mg_build(st, B, S.R, mg_params_from_env(), S.mg));
     std::vector<SysState> ho(S.S);
     const int max_outer = 16;
     for (int outer = 0; outer <= max_outer; outer++) {
         // r = b - K x
         launch_apply<double, false, false, false>(st, B, S.R, B.d_ka, B.d_kb, S.X, S.Qv, nullptr, S.d_jump, nullptr);
         hipLaunchKernelGGL(k_refine_residual, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles,
-                           S.R, S.B, S.Qv, S.Rr, B.Ntot, S.prr);
-        hipLaunchKernelGGL(k_outer_reduce, dim3(S.S), dim3(256), 0, st, B.d_vols, S.R, S.prr, S.outer, tol * tol,
-                           outer == 0);
+                           S.R, B.d_ka, B.d_kb, S.B, S.Qv, S.Rr, B.Ntot, S.prr, S.ppq);
+        hipLaunchKernelGGL(k_outer_reduce, dim3(S.S), dim3(256), 0, st, B.d_vols, S.R, S.prr, S.ppq, S.outer,
+                           tol * tol, outer == 0);
         PSX_HIP(hipGetLastError());
         PSX_HIP(hipMemcpyAsync(ho.data(), S.outer, sizeof(SysState) * S.S, hipMemcpyDeviceToHost, st));
         PSX_HIP(hipStreamSynchronize(st));
@@ -994,18 +914,22 @@ int solve_batch(hipStream_t st, const Batch &B, Solver &S, bool f32, double tol,
         if (!any || total_iters >= maxit || outer == max_outer) break;
         // inner FP32 solve on the scaled residual
         hipLaunchKernelGGL(k_scale_sys, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles, S.R,
-                           S.outer, S.R32, S.X32, S.Rr, B.Ntot);
-        cg_start<float>(st, B, S, S.sys, S.X32, S.R32, S.Z32, S.P32, S.R32, B.d_minv32);
+                           S.outer, B.d_ka, B.d_kb, S.R32, S.X32, S.Rr, B.Ntot);
+        cg_start<float>(st, B, S, S.sys, S.X32, S.R32, S.Z32, S.P32, S.R32, B.d_minv32, use_mg);
         hipLaunchKernelGGL(k_inner_from_outer, dim3((unsigned)psx::ceil_div(S.S, 64)), dim3(64), 0, st, S.S, S.outer,
                            S.sys);
         PSX_HIP(hipGetLastError());
         RC(cg_run<float>(st, B, S, S.sys, S.X32, S.R32, S.Z32, S.P32, S.Q32, B.d_ka32, B.d_kb32, B.d_minv32,
-                         inner_tol, maxit - total_iters, chunk, use_graph));
+                         inner_tol, maxit - total_iters, chunk, use_graph, use_mg));
         std::vector<SysState> hs(S.S);
         PSX_HIP(hipMemcpy(hs.data(), S.sys, sizeof(SysState) * S.S, hipMemcpyDeviceToHost));
         int mx = 0;
         for (auto &s : hs) mx = std::max(mx, s.iters);
         total_iters += mx;
+        if (std::getenv("PSX_MG_DEBUG"))
+            for (int k = 0; k < S.S; k++)
+                std::fprintf(stderr, "outer %d sys %d: outer rr %.3e (bb %.3e bref %.3e) active %d | inner iters %d rr %.3e conv %d\n",
+                             outer, k, ho[k].rr, ho[k].bb, ho[k].bref, ho[k].active, hs[k].iters, hs[k].rr, hs[k].conv);
         hipLaunchKernelGGL(k_unscale_add, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles, S.R,
                            S.outer, S.X, S.X32, B.Ntot);
         PSX_HIP(hipGetLastError());

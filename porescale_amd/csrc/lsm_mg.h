@@ -1,0 +1,61 @@
+// Geometric multigrid preconditioner for the batched LSM solver.
+//
+// Level 0 is the batch's own node grid with the matrix-free element
+// operator.  Level l+1 halves every axis of level l (ceil(n/2) coarse nodes
+// on the even fine indices of the periodic axis; odd fine nodes interpolate
+// from their two even neighbours), P is that trilinear interpolation and the
+// coarse operators are the Galerkin products A_{l+1} = P^T A_l P, stored as
+// 27-point stencils of 3x3 blocks.  Smoothing is Chebyshev on the
+// block-Jacobi scaled operator D^-1 A over [lam/ratio, lam], lam per
+// volume and level the smaller of the Gershgorin bound and 1.1 times a
+// power-iteration estimate of the largest eigenvalue.  The V-cycle is symmetric (same
+// polynomial before and after the coarse correction), so it is a valid PCG
+// preconditioner.
+#pragma once
+#include "lsm_common.h"
+
+namespace lsmk {
+
+struct MGLevel {
+    std::vector<VolDesc> vols;
+    std::vector<int32_t> tvol, tstart;
+    VolDesc *d_vols = nullptr;
+    int32_t *d_tvol = nullptr, *d_tstart = nullptr;
+    int ntiles = 0;
+    int64_t N = 0;
+    float *A = nullptr;     // [27*9][N] stencil (levels >= 1)
+    float *minv = nullptr;  // [6][N] inverse diagonal blocks
+    float *lam = nullptr;   // [nvol] spectral bound of D^-1 A
+    // work vectors [R][3][N]; level 0 uses the caller's r (b) and z (x)
+    float *b = nullptr, *x = nullptr, *r = nullptr, *dA = nullptr, *dB = nullptr;
+    bool own_bx = false, own_minv = false;
+};
+
+struct MGParams {
+    int levels = 4;        // at most this many levels (including level 0)
+    int nu = 2;            // Chebyshev degree of each smoothing pass
+    int coarse_deg = 4;    // Chebyshev degree on the coarsest level
+    float ratio = 30.f;    // smoothing interval [lam/ratio, lam]
+    float coarse_ratio = 200.f;
+    int power_steps = 30;        // power-iteration steps per level for the spectral bound
+    float power_safety = 1.1f;   // bound = min(Gershgorin, safety * power estimate)
+};
+
+struct MG {
+    std::vector<MGLevel> lv;
+    MGParams prm;
+    int R = 0;
+    bool built = false;
+};
+
+// Builds the hierarchy for the batch (R right-hand sides per volume).
+int mg_build(hipStream_t st, const Batch &B, int R, const MGParams &prm, MG &mg);
+void mg_free(MG &mg);
+// z = M^-1 r for every active system; r, z are [R][3][Ntot] FP32 on the
+// batch grid.  prz (may be null) receives the (tile, rhs) partials of r.z.
+void mg_apply(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, const float *r, float *z, double *prz);
+// Parameters from the environment (PSX_MG_LEVELS, PSX_MG_NU, PSX_MG_CDEG,
+// PSX_MG_RATIO, PSX_MG_CRATIO, PSX_MG_POWER); defaults otherwise.
+MGParams mg_params_from_env();
+
+}  // namespace lsmk

@@ -1,0 +1,829 @@
+// Geometric multigrid preconditioner for the batched LSM solver (see
+// lsm_mg.h for the construction).  All levels of all volumes of a batch are
+// processed by the same launches: work is dealt as 256-node tiles of every
+// volume, as in the solver.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+
+#include "lsm_element.h"
+#include "lsm_mg.h"
+
+namespace lsmk {
+namespace {
+
+constexpr int CENTER = 13;  // stencil slot of offset (0,0,0)
+constexpr int NSTEN = 27 * 9;  // coefficients per node: 27 offsets x 3x3 block
+
+// ------------------------------------------------------------ coarsening --
+// Coarse node I of an axis of n fine nodes sits on fine node 2I; odd fine
+// nodes interpolate from their two (periodic) even neighbours.
+__device__ inline int axis_support(int I, int n, int (&fi)[3], double (&w)[3]) {
+    int k = 0;
+    fi[k] = 2 * I; w[k] = 1.0; k++;
+    if (2 * I + 1 < n) { fi[k] = 2 * I + 1; w[k] = 0.5; k++; }
+    if (I >= 1) { fi[k] = 2 * I - 1; w[k] = 0.5; k++; }
+    else if ((n & 1) == 0) { fi[k] = n - 1; w[k] = 0.5; k++; }
+    return k;
+}
+
+__device__ inline int axis_parents(int j, int n, int (&J)[2], double (&w)[2]) {
+    if ((j & 1) == 0) { J[0] = j >> 1; w[0] = 1.0; return 1; }
+    J[0] = (j - 1) >> 1; w[0] = 0.5;
+    int jp = j + 1 == n ? 0 : j + 1;
+    J[1] = jp >> 1; w[1] = 0.5;
+    return 2;
+}
+
+// stencil slot (-1, 0, 1) of coarse node J seen from I on a periodic axis of m >= 3 nodes
+__device__ inline int cslot(int J, int I, int m) {
+    int d = J - I;
+    if (d > 1) d -= m;
+    else if (d < -1) d += m;
+    return d;
+}
+
+struct AxisCombo {
+    int n;
+    int i[12], d[12];
+    double w[12];
+};
+
+// (fine node, fine offset, weight) triples of P^T A P coupling coarse I to
+// the coarse neighbour in slot `target` along one axis.
+__device__ inline void axis_combos(int I, int n, int m, int target, AxisCombo &c) {
+    int fi[3];
+    double fw[3];
+    int ns = axis_support(I, n, fi, fw);
+    c.n = 0;
+    for (int a = 0; a < ns; a++)
+        for (int dl = -1; dl <= 1; dl++) {
+            int j = fi[a] + dl;
+            j = j < 0 ? j + n : (j >= n ? j - n : j);
+            int J[2];
+            double wj[2];
+            int np = axis_parents(j, n, J, wj);
+            for (int b = 0; b < np; b++)
+                if (cslot(J[b], I, m) == target && c.n < 12) {
+                    c.i[c.n] = fi[a]; c.d[c.n] = dl; c.w[c.n] = fw[a] * wj[b]; c.n++;
+                }
+        }
+}
+
+__device__ inline int64_t node_index(const VolDesc &vd, int x, int y, int z) {
+    return vd.off + x + (int64_t)vd.nx * (y + (int64_t)vd.ny * z);
+}
+
+__device__ inline int wrap1(int t, int n) { return t < 0 ? t + n : (t >= n ? t - n : t); }
+
+// ------------------------------------------------------------- set-up -----
+// Assembled fine operator: A0[(9s + 3e + e2) * N + n] = K(n, n + off(s))[e][e2].
+__global__ __launch_bounds__(256) void k_assemble_fine(const VolDesc *__restrict__ vols,
+                                                       const int32_t *__restrict__ tvol,
+                                                       const int32_t *__restrict__ tstart, int ntiles,
+                                                       const double *__restrict__ ka, const double *__restrict__ kb,
+                                                       double *__restrict__ A0, int64_t N) {
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const VolDesc vd = vols[tvol[t]];
+        int x, y, z;
+        bool ok;
+        tile_node(vd, tstart[t], x, y, z, ok);
+        if (!ok) continue;
+        double a[8], b[8];
+        gather_k(vd, ka, kb, x, y, z, a, b);
+        lsm_el::node_stencil<double>(a, b, A0 + node_index(vd, x, y, z), N);
+    }
+}
+
+// Galerkin product on one level: Ac(I, slot) = sum P(i,I) Af(i, d) P(i+d, J).
+__global__ __launch_bounds__(256) void k_galerkin(const VolDesc *__restrict__ fvols, const VolDesc *__restrict__ cvols,
+                                                  const int32_t *__restrict__ ctvol,
+                                                  const int32_t *__restrict__ ctstart, int cntiles,
+                                                  const double *__restrict__ Af, int64_t Nf, double *__restrict__ Ac,
+                                                  int64_t Nc) {
+    for (int t = blockIdx.x; t < cntiles; t += gridDim.x) {
+        const int v = ctvol[t];
+        const VolDesc fd = fvols[v], cd = cvols[v];
+        int X, Y, Z;
+        bool ok;
+        tile_node(cd, ctstart[t], X, Y, Z, ok);
+        if (!ok) continue;
+        const int64_t I = node_index(cd, X, Y, Z);
+        for (int sz = -1; sz <= 1; sz++) {
+            AxisCombo cz;
+            axis_combos(Z, fd.nz, cd.nz, sz, cz);
+            for (int sy = -1; sy <= 1; sy++) {
+                AxisCombo cy;
+                axis_combos(Y, fd.ny, cd.ny, sy, cy);
+                for (int sx = -1; sx <= 1; sx++) {
+                    AxisCombo cx;
+                    axis_combos(X, fd.nx, cd.nx, sx, cx);
+                    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                    for (int kz = 0; kz < cz.n; kz++)
+                        for (int ky = 0; ky < cy.n; ky++) {
+                            double wyz = cz.w[kz] * cy.w[ky];
+                            for (int kx = 0; kx < cx.n; kx++) {
+                                double w = wyz * cx.w[kx];
+                                int64_t fi = node_index(fd, cx.i[kx], cy.i[ky], cz.i[kz]);
+                                int s = (cx.d[kx] + 1) + 3 * (cy.d[ky] + 1) + 9 * (cz.d[kz] + 1);
+#pragma unroll
+                                for (int k = 0; k < 9; k++) acc[k] += w * Af[(int64_t)(9 * s + k) * Nf + fi];
+                            }
+                        }
+                    int sc = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+#pragma unroll
+                    for (int k = 0; k < 9; k++) Ac[(int64_t)(9 * sc + k) * Nc + I] = acc[k];
+                }
+            }
+        }
+    }
+}
+
+__device__ inline void inv_block(const double (&D)[3][3], double (&m)[6]) {
+    double c00 = D[1][1] * D[2][2] - D[1][2] * D[2][1];
+    double c01 = D[1][2] * D[2][0] - D[1][0] * D[2][2];
+    double c02 = D[1][0] * D[2][1] - D[1][1] * D[2][0];
+    double det = D[0][0] * c00 + D[0][1] * c01 + D[0][2] * c02;
+    for (int k = 0; k < 6; k++) m[k] = 0.0;
+    double scale = fabs(D[0][0]) + fabs(D[1][1]) + fabs(D[2][2]);
+    if (det > 1e-12 * scale * scale * scale && scale > 0.0) {
+        double id = 1.0 / det;
+        m[0] = c00 * id;
+        m[1] = (D[0][0] * D[2][2] - D[0][2] * D[2][0]) * id;
+        m[2] = (D[0][0] * D[1][1] - D[0][1] * D[1][0]) * id;
+        m[3] = c01 * id;
+        m[4] = c02 * id;
+        m[5] = (D[0][2] * D[1][0] - D[0][0] * D[1][2]) * id;
+    } else {
+        for (int e = 0; e < 3; e++) m[e] = D[e][e] > 0.0 ? 1.0 / D[e][e] : 0.0;
+    }
+}
+
+// Per level: inverse diagonal blocks (unless given), FP32 copies and the
+// Gershgorin bound max_i sum_j |(D^-1 A)_ij| as per-tile partial maxima.
+__global__ __launch_bounds__(256) void k_level_prep(const VolDesc *__restrict__ vols, const int32_t *__restrict__ tvol,
+                                                    const int32_t *__restrict__ tstart, int ntiles,
+                                                    const double *__restrict__ A, int64_t N,
+                                                    const double *__restrict__ minv_in, float *__restrict__ A32,
+                                                    float *__restrict__ minv32, double *__restrict__ pmax) {
+    __shared__ double sh[4];
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const VolDesc vd = vols[tvol[t]];
+        int x, y, z;
+        bool ok;
+        tile_node(vd, tstart[t], x, y, z, ok);
+        double g = 0.0;
+        if (ok) {
+            const int64_t n = node_index(vd, x, y, z);
+            double m[6];
+            if (minv_in) {
+                for (int k = 0; k < 6; k++) m[k] = minv_in[(int64_t)k * N + n];
+            } else {
+                double D[3][3];
+                for (int e = 0; e < 3; e++)
+                    for (int e2 = 0; e2 < 3; e2++) D[e][e2] = A[(int64_t)(9 * CENTER + 3 * e + e2) * N + n];
+                inv_block(D, m);
+            }
+            if (minv32)
+                for (int k = 0; k < 6; k++) minv32[(int64_t)k * N + n] = (float)m[k];
+            const double M[3][3] = {{m[0], m[3], m[4]}, {m[3], m[1], m[5]}, {m[4], m[5], m[2]}};
+            double rs[3] = {0, 0, 0};
+            for (int s = 0; s < 27; s++) {
+                double B[3][3];
+                for (int e = 0; e < 3; e++)
+                    for (int e2 = 0; e2 < 3; e2++) {
+                        B[e][e2] = A[(int64_t)(9 * s + 3 * e + e2) * N + n];
+                        if (A32) A32[(int64_t)(9 * s + 3 * e + e2) * N + n] = (float)B[e][e2];
+                    }
+                for (int e = 0; e < 3; e++)
+                    for (int e2 = 0; e2 < 3; e2++)
+                        rs[e] += fabs(M[e][0] * B[0][e2] + M[e][1] * B[1][e2] + M[e][2] * B[2][e2]);
+            }
+            g = fmax(rs[0], fmax(rs[1], rs[2]));
+        }
+        // block max
+        for (int off = 32; off > 0; off >>= 1) g = fmax(g, __shfl_down(g, off, 64));
+        int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        __syncthreads();
+        if (lane == 0) sh[w] = g;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double mx = 0.0;
+            for (int k = 0; k < (int)(blockDim.x >> 6); k++) mx = fmax(mx, sh[k]);
+            pmax[t] = mx;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_vol_max(const VolDesc *__restrict__ vols, const double *__restrict__ pmax,
+                                                 float *__restrict__ lam) {
+    const VolDesc vd = vols[blockIdx.x];
+    __shared__ double sh[4];
+    double g = 0.0;
+    for (int k = threadIdx.x; k < vd.ntile; k += blockDim.x) g = fmax(g, pmax[vd.tile0 + k]);
+    for (int off = 32; off > 0; off >>= 1) g = fmax(g, __shfl_down(g, off, 64));
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = g;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mx = 0.0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++) mx = fmax(mx, sh[k]);
+        lam[blockIdx.x] = (float)mx;
+    }
+}
+
+// --------------------------------------------------------- application ----
+struct LevelArgs {
+    const VolDesc *vols;
+    const int32_t *tvol, *tstart;
+    int ntiles;
+    int64_t N;
+    const float *A;     // stencil, or null for the element operator
+    const float *ka, *kb;
+    const float *minv;
+    const float *lam;
+    float ratio;
+};
+
+// Chebyshev coefficients of step k (k = -1: restart, d = D^-1 r / theta):
+// d_{k+1} = c1 d_k + c2 D^-1 r_{k+1}.
+__device__ inline void cheb_coef(float lam, float ratio, int k, float &c1, float &c2) {
+    if (!(lam > 0.f)) { c1 = 0.f; c2 = 0.f; return; }
+    const float beta = lam, alpha = lam / ratio;
+    const float theta = 0.5f * (beta + alpha), delta = 0.5f * (beta - alpha);
+    if (k < 0) { c1 = 0.f; c2 = 1.f / theta; return; }
+    const float sigma = theta / delta;
+    float rho = 1.f / sigma, rho_next = rho;
+    for (int j = 0; j <= k; j++) {
+        rho_next = 1.f / (2.f * sigma - rho);
+        if (j < k) rho = rho_next;
+    }
+    c1 = rho_next * rho;
+    c2 = 2.f * rho_next / delta;
+}
+
+__device__ inline void apply_minv(const float *__restrict__ minv, int64_t N, int64_t n, const float (&r)[3],
+                                  float (&z)[3]) {
+    float m0 = minv[n], m1 = minv[N + n], m2 = minv[2 * N + n], m3 = minv[3 * N + n], m4 = minv[4 * N + n],
+          m5 = minv[5 * N + n];
+    z[0] = m0 * r[0] + m3 * r[1] + m4 * r[2];
+    z[1] = m3 * r[0] + m1 * r[1] + m5 * r[2];
+    z[2] = m4 * r[0] + m5 * r[1] + m2 * r[2];
+}
+
+// (A u)(node) for one right-hand side u ([3][N], offset to that rhs).
+template <bool ELEM>
+__device__ inline void op_node(const LevelArgs &L, const VolDesc &vd, int x, int y, int z, int64_t n,
+                               const float *__restrict__ u, bool act, const float (&a)[8], const float (&b)[8],
+                               float (&out)[3]) {
+    out[0] = out[1] = out[2] = 0.f;
+    if (ELEM) {
+        if (!act) return;
+        float uu[27][3];
+        const double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        gather27<float, false>(vd, u, L.N, x, y, z, J, uu);
+        float f[3] = {0.f, 0.f, 0.f};
+        lsm_el::node_force<float>(uu, a, b, f);
+        out[0] = -f[0]; out[1] = -f[1]; out[2] = -f[2];
+    } else {
+#pragma unroll 1
+        for (int dz = -1; dz <= 1; dz++) {
+            int zz = wrap1(z + dz, vd.nz);
+#pragma unroll 1
+            for (int dy = -1; dy <= 1; dy++) {
+                int yy = wrap1(y + dy, vd.ny);
+#pragma unroll
+                for (int dx = -1; dx <= 1; dx++) {
+                    int xx = wrap1(x + dx, vd.nx);
+                    int s = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+                    int64_t nb = node_index(vd, xx, yy, zz);
+                    float u0 = u[nb], u1 = u[L.N + nb], u2 = u[2 * L.N + nb];
+#pragma unroll
+                    for (int e = 0; e < 3; e++) {
+                        const float *Ar = L.A + (int64_t)(9 * s + 3 * e) * L.N + n;
+                        out[e] += Ar[0] * u0 + Ar[L.N] * u1 + Ar[2 * L.N] * u2;
+                    }
+                }
+            }
+        }
+    }
+}
+
+__device__ inline bool sys_active(const SysState *sys, int v, int R, int r) {
+    return !sys || sys[v * R + r].active;
+}
+
+// FIRST: r = b, x = 0, d = D^-1 b / theta  (start of a smoothing pass from x = 0)
+__global__ __launch_bounds__(TILE) void k_cheb_first(LevelArgs L, int R, const SysState *__restrict__ sys,
+                                                     const float *__restrict__ bv, float *__restrict__ xv,
+                                                     float *__restrict__ rv, float *__restrict__ dv) {
+    for (int t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
+        const int v = L.tvol[t];
+        const VolDesc vd = L.vols[v];
+        int x, y, z;
+        bool ok;
+        tile_node(vd, L.tstart[t], x, y, z, ok);
+        if (!ok) continue;
+        const int64_t n = node_index(vd, x, y, z);
+        float c1, c2;
+        cheb_coef(L.lam[v], L.ratio, -1, c1, c2);
+        {  // one right-hand side per grid row
+            const int r = blockIdx.y;
+            if (!sys_active(sys, v, R, r)) continue;
+            const int64_t base = (int64_t)r * 3 * L.N;
+            float bb[3], zz[3];
+            for (int e = 0; e < 3; e++) bb[e] = bv[base + e * L.N + n];
+            apply_minv(L.minv, L.N, n, bb, zz);
+            for (int e = 0; e < 3; e++) {
+                rv[base + e * L.N + n] = bb[e];
+                xv[base + e * L.N + n] = 0.f;
+                dv[base + e * L.N + n] = c2 * zz[e];
+            }
+        }
+    }
+}
+
+// STEP k: x += d; r -= A d; (NEWD) d' = c1 d + c2 D^-1 r.  k = -1 restarts
+// the recurrence (d' = D^-1 r / theta) after a coarse correction d.
+template <bool ELEM, bool NEWD>
+__global__ __launch_bounds__(TILE) void k_cheb_step(LevelArgs L, int R, const SysState *__restrict__ sys, int k,
+                                                    float *__restrict__ xv, float *__restrict__ rv,
+                                                    const float *__restrict__ dv, float *__restrict__ dout) {
+    for (int t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
+        const int v = L.tvol[t];
+        const VolDesc vd = L.vols[v];
+        int x, y, z;
+        bool ok;
+        tile_node(vd, L.tstart[t], x, y, z, ok);
+        const int64_t n = node_index(vd, x, y, z);
+        float a[8], b[8];
+        bool act = false;
+        if (ELEM && ok) act = gather_k(vd, L.ka, L.kb, x, y, z, a, b);
+        float c1 = 0.f, c2 = 0.f;
+        if (NEWD) cheb_coef(L.lam[v], L.ratio, k, c1, c2);
+        {  // one right-hand side per grid row
+            const int r = blockIdx.y;
+            if (!sys_active(sys, v, R, r)) continue;
+            const int64_t base = (int64_t)r * 3 * L.N;
+            float q[3];
+            op_node<ELEM>(L, vd, x, y, z, n, dv + base, act, a, b, q);
+            if (!ok) continue;
+            float rr[3], dd[3];
+            for (int e = 0; e < 3; e++) {
+                const int64_t i = base + e * L.N + n;
+                dd[e] = dv[i];
+                xv[i] += dd[e];
+                rr[e] = rv[i] - q[e];
+                rv[i] = rr[e];
+            }
+            if (NEWD) {
+                float zz[3];
+                apply_minv(L.minv, L.N, n, rr, zz);
+                for (int e = 0; e < 3; e++) dout[base + e * L.N + n] = c1 * dd[e] + c2 * zz[e];
+            }
+        }
+    }
+}
+
+// coarse node I: b = P^T r_fine; r = b; x = 0; d = D^-1 b / theta
+__global__ __launch_bounds__(TILE) void k_restrict_first(LevelArgs F, LevelArgs C, int R,
+                                                         const SysState *__restrict__ sys,
+                                                         const float *__restrict__ rf, float *__restrict__ bc,
+                                                         float *__restrict__ xc, float *__restrict__ rc,
+                                                         float *__restrict__ dc) {
+    for (int t = blockIdx.x; t < C.ntiles; t += gridDim.x) {
+        const int v = C.tvol[t];
+        const VolDesc cd = C.vols[v], fd = F.vols[v];
+        int X, Y, Z;
+        bool ok;
+        tile_node(cd, C.tstart[t], X, Y, Z, ok);
+        if (!ok) continue;
+        const int64_t I = node_index(cd, X, Y, Z);
+        int ix[3], iy[3], iz[3];
+        double wx[3], wy[3], wz[3];
+        int nx = axis_support(X, fd.nx, ix, wx), ny = axis_support(Y, fd.ny, iy, wy),
+            nz = axis_support(Z, fd.nz, iz, wz);
+        float c1, c2;
+        cheb_coef(C.lam[v], C.ratio, -1, c1, c2);
+        {  // one right-hand side per grid row
+            const int r = blockIdx.y;
+            if (!sys_active(sys, v, R, r)) continue;
+            const float *rr = rf + (int64_t)r * 3 * F.N;
+            float s[3] = {0.f, 0.f, 0.f};
+            for (int a = 0; a < nz; a++)
+                for (int b = 0; b < ny; b++)
+                    for (int c = 0; c < nx; c++) {
+                        float w = (float)(wz[a] * wy[b] * wx[c]);
+                        int64_t i = node_index(fd, ix[c], iy[b], iz[a]);
+                        s[0] += w * rr[i];
+                        s[1] += w * rr[F.N + i];
+                        s[2] += w * rr[2 * F.N + i];
+                    }
+            float zz[3];
+            apply_minv(C.minv, C.N, I, s, zz);
+            const int64_t base = (int64_t)r * 3 * C.N;
+            for (int e = 0; e < 3; e++) {
+                bc[base + e * C.N + I] = s[e];
+                rc[base + e * C.N + I] = s[e];
+                xc[base + e * C.N + I] = 0.f;
+                dc[base + e * C.N + I] = c2 * zz[e];
+            }
+        }
+    }
+}
+
+// fine node i: e = P (xc + dpend)
+__global__ __launch_bounds__(TILE) void k_prolong(LevelArgs F, LevelArgs C, int R, const SysState *__restrict__ sys,
+                                                  const float *__restrict__ xc, const float *__restrict__ dpend,
+                                                  float *__restrict__ ef) {
+    for (int t = blockIdx.x; t < F.ntiles; t += gridDim.x) {
+        const int v = F.tvol[t];
+        const VolDesc fd = F.vols[v], cd = C.vols[v];
+        int x, y, z;
+        bool ok;
+        tile_node(fd, F.tstart[t], x, y, z, ok);
+        if (!ok) continue;
+        const int64_t n = node_index(fd, x, y, z);
+        int jx[2], jy[2], jz[2];
+        double wx[2], wy[2], wz[2];
+        int nx = axis_parents(x, fd.nx, jx, wx), ny = axis_parents(y, fd.ny, jy, wy),
+            nz = axis_parents(z, fd.nz, jz, wz);
+        {  // one right-hand side per grid row
+            const int r = blockIdx.y;
+            if (!sys_active(sys, v, R, r)) continue;
+            const int64_t cb = (int64_t)r * 3 * C.N;
+            float s[3] = {0.f, 0.f, 0.f};
+            for (int a = 0; a < nz; a++)
+                for (int b = 0; b < ny; b++)
+                    for (int c = 0; c < nx; c++) {
+                        float w = (float)(wz[a] * wy[b] * wx[c]);
+                        int64_t J = node_index(cd, jx[c], jy[b], jz[a]);
+                        for (int e = 0; e < 3; e++) {
+                            float val = xc[cb + e * C.N + J];
+                            if (dpend) val += dpend[cb + e * C.N + J];
+                            s[e] += w * val;
+                        }
+                    }
+            const int64_t fb = (int64_t)r * 3 * F.N;
+            for (int e = 0; e < 3; e++) ef[fb + e * F.N + n] = s[e];
+        }
+    }
+}
+
+// level 0: z += d (the pending last update); partials of r.z per (tile, rhs)
+__global__ __launch_bounds__(TILE) void k_final0(LevelArgs L, int R, const SysState *__restrict__ sys,
+                                                 const float *__restrict__ rcg, float *__restrict__ z,
+                                                 const float *__restrict__ d, double *__restrict__ prz) {
+    __shared__ double sh[TILE / 64];
+    for (int t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
+        const int v = L.tvol[t];
+        const VolDesc vd = L.vols[v];
+        int x, y, zc;
+        bool ok;
+        tile_node(vd, L.tstart[t], x, y, zc, ok);
+        const int64_t n = node_index(vd, x, y, zc);
+        {  // one right-hand side per grid row
+            const int r = blockIdx.y;
+            if (!sys_active(sys, v, R, r)) {
+                if (prz && threadIdx.x == 0) prz[(int64_t)t * R + r] = 0.0;
+                continue;
+            }
+            double dot = 0.0;
+            if (ok) {
+                const int64_t base = (int64_t)r * 3 * L.N;
+                for (int e = 0; e < 3; e++) {
+                    const int64_t i = base + e * L.N + n;
+                    float zv = z[i] + d[i];
+                    z[i] = zv;
+                    dot += (double)rcg[i] * (double)zv;
+                }
+            }
+            if (prz) {
+                double s = block_sum(dot, sh);
+                if (threadIdx.x == 0) prz[(int64_t)t * R + r] = s;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------- spectral bounds ---
+// Power iteration on D^-1 A (right-hand side slot 0 of the level's work
+// vectors): vout = D^-1 A vin / |vin|, per-tile partials of |vout|^2.
+__global__ __launch_bounds__(TILE) void k_pow_init(LevelArgs L, float *__restrict__ v) {
+    for (int t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
+        const VolDesc vd = L.vols[L.tvol[t]];
+        int x, y, z;
+        bool ok;
+        tile_node(vd, L.tstart[t], x, y, z, ok);
+        if (!ok) continue;
+        const int64_t n = node_index(vd, x, y, z);
+        for (int e = 0; e < 3; e++) {
+            uint32_t h = (uint32_t)(n * 3 + e) * 2654435761u;
+            h ^= h >> 15; h *= 2246822519u; h ^= h >> 13;
+            v[e * L.N + n] = (float)(h & 0xFFFF) / 65536.f - 0.5f;
+        }
+    }
+}
+
+template <bool ELEM>
+__global__ __launch_bounds__(TILE) void k_pow_step(LevelArgs L, const float *__restrict__ vin, float *__restrict__ vout,
+                                                   const double *__restrict__ nin, double *__restrict__ pnorm) {
+    __shared__ double sh[TILE / 64];
+    for (int t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
+        const int v = L.tvol[t];
+        const VolDesc vd = L.vols[v];
+        int x, y, z;
+        bool ok;
+        tile_node(vd, L.tstart[t], x, y, z, ok);
+        const int64_t n = node_index(vd, x, y, z);
+        float a[8], b[8];
+        bool act = false;
+        if (ELEM && ok) act = gather_k(vd, L.ka, L.kb, x, y, z, a, b);
+        float q[3];
+        op_node<ELEM>(L, vd, x, y, z, n, vin, act, a, b, q);
+        double d = 0.0;
+        if (ok) {
+            const float sc = nin && nin[v] > 0.0 ? (float)(1.0 / nin[v]) : 1.f;
+            float qs[3] = {q[0] * sc, q[1] * sc, q[2] * sc}, zz[3];
+            apply_minv(L.minv, L.N, n, qs, zz);
+            for (int e = 0; e < 3; e++) {
+                vout[e * L.N + n] = zz[e];
+                d += (double)zz[e] * zz[e];
+            }
+        }
+        double s = block_sum(d, sh);
+        if (threadIdx.x == 0) pnorm[t] = s;
+    }
+}
+
+// per volume: nrm = sqrt(sum of the tile partials)
+__global__ __launch_bounds__(256) void k_vol_norm(const VolDesc *__restrict__ vols, const double *__restrict__ pnorm,
+                                                  double *__restrict__ nrm) {
+    const VolDesc vd = vols[blockIdx.x];
+    __shared__ double sh[4];
+    double g = 0.0;
+    for (int k = threadIdx.x; k < vd.ntile; k += blockDim.x) g += pnorm[vd.tile0 + k];
+    double s = block_sum(g, sh);
+    if (threadIdx.x == 0) nrm[blockIdx.x] = sqrt(s);
+}
+
+// lam = min(Gershgorin bound, safety * power estimate)
+__global__ void k_pow_lam(int nvol, const double *__restrict__ nrm, float *__restrict__ lam, float safety) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < nvol && nrm[v] > 0.0) lam[v] = fminf(lam[v], safety * (float)nrm[v]);
+}
+
+LevelArgs args_of(const MGLevel &l, const Batch &B, float ratio) {
+    LevelArgs a{};
+    a.vols = l.d_vols; a.tvol = l.d_tvol; a.tstart = l.d_tstart; a.ntiles = l.ntiles; a.N = l.N;
+    a.A = l.A; a.ka = B.d_ka32; a.kb = B.d_kb32; a.minv = l.minv; a.lam = l.lam; a.ratio = ratio;
+    return a;
+}
+
+template <bool NEWD>
+void launch_step(hipStream_t st, const LevelArgs &L, bool elem, int R, const SysState *sys, int k, float *x, float *r,
+                 const float *d, float *dout) {
+    int g = grid_for(L.ntiles);
+    if (elem)
+        hipLaunchKernelGGL((k_cheb_step<true, NEWD>), dim3(g, R), dim3(TILE), 0, st, L, R, sys, k, x, r, d, dout);
+    else
+        hipLaunchKernelGGL((k_cheb_step<false, NEWD>), dim3(g, R), dim3(TILE), 0, st, L, R, sys, k, x, r, d, dout);
+}
+
+// V-cycle from level l, whose b, r, x, dA hold a started smoothing pass
+// (r = b, x = 0, dA = d_0).  Returns the pending last update of x.
+float *vcycle(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, int l) {
+    MGLevel &lv = mg.lv[l];
+    const bool coarsest = l + 1 == (int)mg.lv.size();
+    const bool elem = l == 0;
+    const MGParams &p = mg.prm;
+    LevelArgs L = args_of(lv, B, coarsest ? p.coarse_ratio : p.ratio);
+    float *dA = lv.dA, *dB = lv.dB;
+    if (coarsest) {
+        for (int k = 0; k + 1 < p.coarse_deg; k++) {
+            launch_step<true>(st, L, elem, mg.R, sys, k, lv.x, lv.r, dA, dB);
+            std::swap(dA, dB);
+        }
+        return dA;
+    }
+    // pre-smoothing: nu updates, the last one with the residual update
+    for (int k = 0; k + 1 < p.nu; k++) {
+        launch_step<true>(st, L, elem, mg.R, sys, k, lv.x, lv.r, dA, dB);
+        std::swap(dA, dB);
+    }
+    launch_step<false>(st, L, elem, mg.R, sys, 0, lv.x, lv.r, dA, nullptr);
+    // coarse correction
+    MGLevel &cv = mg.lv[l + 1];
+    const bool c_coarsest = l + 2 == (int)mg.lv.size();
+    LevelArgs C = args_of(cv, B, c_coarsest ? p.coarse_ratio : p.ratio);
+    hipLaunchKernelGGL(k_restrict_first, dim3(grid_for(C.ntiles), mg.R), dim3(TILE), 0, st, L, C, mg.R, sys, lv.r, cv.b,
+                       cv.x, cv.r, cv.dA);
+    float *pend = vcycle(st, B, mg, sys, l + 1);
+    hipLaunchKernelGGL(k_prolong, dim3(grid_for(L.ntiles), mg.R), dim3(TILE), 0, st, L, C, mg.R, sys, cv.x,
+                       (const float *)pend, dA);
+    // post-smoothing: restart from the corrected x, nu updates
+    launch_step<true>(st, L, elem, mg.R, sys, -1, lv.x, lv.r, dA, dB);
+    std::swap(dA, dB);
+    for (int k = 0; k + 1 < p.nu; k++) {
+        launch_step<true>(st, L, elem, mg.R, sys, k, lv.x, lv.r, dA, dB);
+        std::swap(dA, dB);
+    }
+    return dA;
+}
+
+int alloc_level_vectors(MGLevel &l, int R, bool bx) {
+    size_t n = (size_t)R * 3 * l.N;
+    if (bx) { RC(dalloc(&l.b, n)); RC(dalloc(&l.x, n)); l.own_bx = true; }
+    RC(dalloc(&l.r, n)); RC(dalloc(&l.dA, n)); RC(dalloc(&l.dB, n));
+    return PSX_OK;
+}
+
+int upload_level(hipStream_t st, MGLevel &l) {
+    RC(dalloc(&l.d_vols, l.vols.size()));
+    RC(dalloc(&l.d_tvol, l.tvol.size()));
+    RC(dalloc(&l.d_tstart, l.tstart.size()));
+    PSX_HIP(hipMemcpyAsync(l.d_vols, l.vols.data(), sizeof(VolDesc) * l.vols.size(), hipMemcpyHostToDevice, st));
+    PSX_HIP(hipMemcpyAsync(l.d_tvol, l.tvol.data(), sizeof(int32_t) * l.tvol.size(), hipMemcpyHostToDevice, st));
+    PSX_HIP(hipMemcpyAsync(l.d_tstart, l.tstart.data(), sizeof(int32_t) * l.tstart.size(), hipMemcpyHostToDevice,
+                           st));
+    return PSX_OK;
+}
+
+int env_int(const char *name, int def) {
+    const char *e = std::getenv(name);
+    return e ? std::atoi(e) : def;
+}
+
+float env_float(const char *name, float def) {
+    const char *e = std::getenv(name);
+    return e ? (float)std::atof(e) : def;
+}
+
+}  // namespace
+
+MGParams mg_params_from_env() {
+    MGParams p;
+    p.levels = std::max(1, env_int("PSX_MG_LEVELS", p.levels));
+    p.nu = std::max(1, env_int("PSX_MG_NU", p.nu));
+    p.coarse_deg = std::max(1, env_int("PSX_MG_CDEG", p.coarse_deg));
+    p.ratio = std::max(1.5f, env_float("PSX_MG_RATIO", p.ratio));
+    p.coarse_ratio = std::max(1.5f, env_float("PSX_MG_CRATIO", p.coarse_ratio));
+    p.power_steps = std::max(0, env_int("PSX_MG_POWER", p.power_steps));
+    return p;
+}
+
+void mg_free(MG &mg) {
+    for (auto &l : mg.lv) {
+        void *bufs[] = {l.d_vols, l.d_tvol, l.d_tstart, l.A, l.lam, l.r, l.dA, l.dB};
+        for (void *b : bufs)
+            if (b) (void)hipFree(b);
+        if (l.own_bx) { (void)hipFree(l.b); (void)hipFree(l.x); }
+        if (l.own_minv && l.minv) (void)hipFree(l.minv);
+    }
+    mg = MG();
+}
+
+int mg_build(hipStream_t st, const Batch &B, int R, const MGParams &prm, MG &mg) {
+    mg_free(mg);
+    mg.prm = prm;
+    mg.R = R;
+    const int nvol = (int)B.vols.size();
+    // number of levels: coarsen while every axis of every volume keeps >= 3 coarse nodes
+    int L = 1;
+    {
+        std::vector<VolDesc> cur = B.vols;
+        while (L < prm.levels) {
+            bool can = true;
+            for (auto &d : cur) can &= d.nx >= 6 && d.ny >= 6 && d.nz >= 6;
+            if (!can) break;
+            for (auto &d : cur) { d.nx = (d.nx + 1) / 2; d.ny = (d.ny + 1) / 2; d.nz = (d.nz + 1) / 2; }
+            L++;
+        }
+    }
+    mg.lv.resize(L);
+    // level 0: the batch grid
+    MGLevel &l0 = mg.lv[0];
+    l0.vols = B.vols; l0.tvol = B.tvol; l0.tstart = B.tstart; l0.ntiles = B.ntiles; l0.N = B.Ntot;
+    l0.minv = B.d_minv32;
+    // coarse geometry
+    for (int l = 1; l < L; l++) {
+        MGLevel &c = mg.lv[l];
+        const MGLevel &f = mg.lv[l - 1];
+        int64_t off = 0;
+        int tile = 0;
+        for (int v = 0; v < nvol; v++) {
+            VolDesc d{};
+            d.nx = (f.vols[v].nx + 1) / 2; d.ny = (f.vols[v].ny + 1) / 2; d.nz = (f.vols[v].nz + 1) / 2;
+            int64_t N = (int64_t)d.nx * d.ny * d.nz;
+            d.off = off; d.tile0 = tile; d.ntile = (int)psx::ceil_div(N, TILE);
+            for (int k = 0; k < d.ntile; k++) { c.tvol.push_back(v); c.tstart.push_back(k * TILE); }
+            tile += d.ntile;
+            off += N;
+            c.vols.push_back(d);
+        }
+        c.N = off;
+        c.ntiles = tile;
+    }
+    for (int l = 0; l < L; l++) RC(upload_level(st, mg.lv[l]));
+    // operators: A0 assembled in FP64, Galerkin products down the levels
+    double *Af = nullptr, *Ac = nullptr, *pmax = nullptr;
+    int maxtiles = 0;
+    for (auto &l : mg.lv) maxtiles = std::max(maxtiles, l.ntiles);
+    RC(dalloc(&pmax, maxtiles));
+    RC(dalloc(&Af, (size_t)NSTEN * l0.N));
+    hipLaunchKernelGGL(k_assemble_fine, dim3(grid_for(l0.ntiles)), dim3(256), 0, st, l0.d_vols, l0.d_tvol,
+                       l0.d_tstart, l0.ntiles, B.d_ka, B.d_kb, Af, l0.N);
+    for (int l = 0; l < L; l++) {
+        MGLevel &lv = mg.lv[l];
+        RC(dalloc(&lv.lam, nvol));
+        if (l == 0) {
+            hipLaunchKernelGGL(k_level_prep, dim3(grid_for(lv.ntiles)), dim3(256), 0, st, lv.d_vols, lv.d_tvol,
+                               lv.d_tstart, lv.ntiles, (const double *)Af, lv.N, (const double *)B.d_minv,
+                               (float *)nullptr, (float *)nullptr, pmax);
+        } else {
+            RC(dalloc(&lv.A, (size_t)NSTEN * lv.N));
+            RC(dalloc(&lv.minv, (size_t)6 * lv.N));
+            lv.own_minv = true;
+            hipLaunchKernelGGL(k_level_prep, dim3(grid_for(lv.ntiles)), dim3(256), 0, st, lv.d_vols, lv.d_tvol,
+                               lv.d_tstart, lv.ntiles, (const double *)Af, lv.N, (const double *)nullptr, lv.A,
+                               lv.minv, pmax);
+        }
+        hipLaunchKernelGGL(k_vol_max, dim3(nvol), dim3(256), 0, st, lv.d_vols, (const double *)pmax, lv.lam);
+        PSX_HIP(hipGetLastError());
+        if (l + 1 < L) {
+            MGLevel &cv = mg.lv[l + 1];
+            RC(dalloc(&Ac, (size_t)NSTEN * cv.N));
+            hipLaunchKernelGGL(k_galerkin, dim3(grid_for(cv.ntiles)), dim3(256), 0, st, lv.d_vols, cv.d_vols,
+                               cv.d_tvol, cv.d_tstart, cv.ntiles, (const double *)Af, lv.N, Ac, cv.N);
+            PSX_HIP(hipGetLastError());
+            PSX_HIP(hipStreamSynchronize(st));
+            (void)hipFree(Af);
+            Af = Ac;
+            Ac = nullptr;
+        }
+    }
+    PSX_HIP(hipStreamSynchronize(st));
+    (void)hipFree(Af);
+    (void)hipFree(pmax);
+    if (env_int("PSX_MG_DEBUG", 0)) {
+        for (int l = 0; l < L; l++) {
+            std::vector<float> lam(nvol);
+            PSX_HIP(hipMemcpy(lam.data(), mg.lv[l].lam, sizeof(float) * nvol, hipMemcpyDeviceToHost));
+            std::fprintf(stderr, "mg level %d: N=%lld vol0 %dx%dx%d lam0=%g\n", l, (long long)mg.lv[l].N,
+                         mg.lv[l].vols[0].nx, mg.lv[l].vols[0].ny, mg.lv[l].vols[0].nz, lam[0]);
+        }
+    }
+    for (int l = 0; l < L; l++) RC(alloc_level_vectors(mg.lv[l], R, l > 0));
+    // sharpen the bounds: power iteration on D^-1 A per volume and level
+    if (prm.power_steps > 0) {
+        double *pn = nullptr, *n0 = nullptr, *n1 = nullptr;
+        RC(dalloc(&pn, maxtiles)); RC(dalloc(&n0, nvol)); RC(dalloc(&n1, nvol));
+        for (int l = 0; l < L; l++) {
+            MGLevel &lv = mg.lv[l];
+            LevelArgs A = args_of(lv, B, prm.ratio);
+            const int g = grid_for(lv.ntiles);
+            float *va = lv.dA, *vb = lv.dB;
+            hipLaunchKernelGGL(k_pow_init, dim3(g), dim3(TILE), 0, st, A, va);
+            for (int it = 0; it < prm.power_steps; it++) {
+                if (l == 0)
+                    hipLaunchKernelGGL(k_pow_step<true>, dim3(g), dim3(TILE), 0, st, A, (const float *)va, vb,
+                                       it ? (const double *)n0 : (const double *)nullptr, pn);
+                else
+                    hipLaunchKernelGGL(k_pow_step<false>, dim3(g), dim3(TILE), 0, st, A, (const float *)va, vb,
+                                       it ? (const double *)n0 : (const double *)nullptr, pn);
+                hipLaunchKernelGGL(k_vol_norm, dim3(nvol), dim3(256), 0, st, lv.d_vols, (const double *)pn, n1);
+                std::swap(va, vb);
+                std::swap(n0, n1);
+            }
+            hipLaunchKernelGGL(k_pow_lam, dim3((unsigned)psx::ceil_div(nvol, 64)), dim3(64), 0, st, nvol,
+                               (const double *)n0, lv.lam, prm.power_safety);
+            PSX_HIP(hipGetLastError());
+        }
+        PSX_HIP(hipStreamSynchronize(st));
+        (void)hipFree(pn); (void)hipFree(n0); (void)hipFree(n1);
+    }
+    if (env_int("PSX_MG_DEBUG", 0)) {
+        for (int l = 0; l < L; l++) {
+            std::vector<float> lam(nvol);
+            PSX_HIP(hipMemcpy(lam.data(), mg.lv[l].lam, sizeof(float) * nvol, hipMemcpyDeviceToHost));
+            std::fprintf(stderr, "mg level %d: lam0 after power iteration %g\n", l, lam[0]);
+        }
+    }
+    mg.built = true;
+    return PSX_OK;
+}
+
+void mg_apply(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, const float *r, float *z, double *prz) {
+    MGLevel &l0 = mg.lv[0];
+    const bool only = mg.lv.size() == 1;
+    LevelArgs L = args_of(l0, B, only ? mg.prm.coarse_ratio : mg.prm.ratio);
+    l0.b = const_cast<float *>(r);
+    l0.x = z;
+    hipLaunchKernelGGL(k_cheb_first, dim3(grid_for(L.ntiles), mg.R), dim3(TILE), 0, st, L, mg.R, sys, (const float *)r, z,
+                       l0.r, l0.dA);
+    float *pend = vcycle(st, B, mg, sys, 0);
+    hipLaunchKernelGGL(k_final0, dim3(grid_for(L.ntiles), mg.R), dim3(TILE), 0, st, L, mg.R, sys, r, z,
+                       (const float *)pend, prz);
+}
+
+}  // namespace lsmk
