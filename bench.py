@@ -10,9 +10,11 @@ the collide kernel, timed with HIP events on the library's own stream.
 
 Also reported (objects in the same line):
   lsm          64^3 single-solid sphere pack (porosity 0.35), all six load
-               cases to rtol 1e-10: plain FP64 PCG one case at a time versus
-               the batched FP32-matvec/FP64-refinement solver; moduli error;
-               batched sub-volumes and adjoint gradients per second.
+               cases at rtol 1e-10 and 1e-5: plain FP64 PCG one case at a time
+               versus the batched FP32 multigrid-PCG / FP64-refinement solver,
+               with the moduli error; 64 uneven two-component sub-volumes in
+               one batch; adjoint gradients of (K, G) at 128^3 with a
+               lognormal stiffness field, and Gauss-Newton calibration steps.
   cpu_baseline the oracle (C restatement of the reference, 1 thread) timed on
                this host over a bounded sample of the same LBM workload.
 
@@ -101,53 +103,91 @@ def cpu_baseline(ind, inp, blue, budget_s=12.0):
             "sample": f"{n} steps of the same 256^3 two-phase MRT workload, oracle in C, 1 thread"}
 
 
-def run_lsm(quick=False):
+def _plain_rate(ind, cases, rtol, pars=(1.0, 0.1), maxit=200000):
+    """Plain FP64 GPU path: block-Jacobi PCG, one load case per solve."""
+    from porescale_amd.lsm import LSM
+    with LSM(ind, *pars, precision="f64") as s:
+        s.solve(cases[0], rtol, maxit)  # warm-up (allocation, graph build)
+        t0 = time.perf_counter()
+        its = [s.solve(c, rtol, maxit)[0] for c in cases]
+        return time.perf_counter() - t0, its
+
+
+def run_lsm(full=False):
+    """LSM rows: six-case moduli at 64^3 (plain FP64 vs batched FP32+MG, at
+    rtol 1e-10 and 1e-5), uneven sub-volumes in one batch, adjoint gradients
+    at 128^3 with a lognormal stiffness field and Gauss-Newton calibration."""
     import numpy as np
     from porescale_amd import geometry as geo
+    from porescale_amd.autograd import calibrate
     from porescale_amd.lsm import LSM, batch_moduli
     res = {}
     ind = geo.sphere_pack((64, 64, 64), 0.35, 6.0, seed=0)
-    # plain FP64: block-Jacobi PCG, one load case per solve
-    with LSM(ind, 1.0, 0.1, precision="f64") as s:
-        s.solve(0, 1e-10, 50000)
-        t0 = time.perf_counter()
-        its = []
-        for case in range(6):
-            it, rel = s.solve(case, 1e-10, 50000)
-            its.append(it)
-        t_plain = time.perf_counter() - t0
-        _, K64, G64 = s.moduli(1e-10, 50000)
-    # batched: six cases per launch, FP32 matvec + FP64 refinement
+    cfg2 = {"cfg": "64^3 sphere pack (r=6, porosity 0.35, seed 0), alpha=1, beta=0.1, 6 load cases"}
+    for rtol in (1e-10, 1e-5):
+        t_plain, its = _plain_rate(ind, range(6), rtol)
+        with LSM(ind, 1.0, 0.1, precision="f32") as s:
+            s.moduli(rtol, 200000)
+            t0 = time.perf_counter()
+            C6, K, G = s.moduli(rtol, 200000)
+            t_b = time.perf_counter() - t0
+        with LSM(ind, 1.0, 0.1, precision="f64") as s:
+            _, K64, G64 = s.moduli(1e-12, 200000)
+        cfg2[f"rtol_{rtol:g}"] = {
+            "plain_fp64_solves_per_s": 6 / t_plain, "plain_fp64_iters": its,
+            "batched_mg_solves_per_s": 6 / t_b, "speedup": t_plain / t_b,
+            "K_rel_err_vs_fp64": abs(K - K64) / K64, "G_rel_err_vs_fp64": abs(G - G64) / G64,
+            "C11": C6[0, 0], "C44": C6[3, 3], "K": K, "G": G}
     with LSM(ind, 1.0, 0.1, precision="f32") as s:
-        s.moduli(1e-10, 50000)
-        t0 = time.perf_counter()
-        C6, K, G = s.moduli(1e-10, 50000)
-        t_batch = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        s.adjoint(1.0, 0.0, 1e-10, 50000)
-        t_adj = time.perf_counter() - t0
         t64, t32 = s.profile(20)
-        nsolid = s.nsolid_nodes
-    res["cfg"] = "64^3 sphere pack (r=6, porosity 0.35), alpha=1, beta=0.1, 6 load cases, rtol 1e-10"
-    res["plain_fp64_solves_per_s"] = 6 / t_plain
-    res["plain_fp64_iters"] = its
-    res["batched_solves_per_s"] = 6 / t_batch
-    res["speedup_vs_plain_fp64"] = t_plain / t_batch
-    res["grads_per_s"] = 1.0 / t_adj
-    res["K_rel_err_vs_fp64"] = abs(K - K64) / abs(K64)
-    res["G_rel_err_vs_fp64"] = abs(G - G64) / abs(G64)
-    res["K"], res["G"] = K, G
-    res["matvec_ms"] = {"fp64": t64, "fp32": t32, "solid_nodes": nsolid}
-    if not quick:
-        rng = np.random.default_rng(0)
-        sizes = rng.integers(32, 97, size=64)
-        vols = [geo.two_component(geo.sphere_pack((int(L), int(L), int(L)), 0.35, 5.0, seed=100 + i), 0.7,
-                                  seed=200 + i, radius=4.0) for i, L in enumerate(sizes)]
-        t0 = time.perf_counter()
-        Cb, KG, ms, it = batch_moduli(vols, 1.0, 0.1, 0.3, 0.02, precision="f32", rtol=1e-10)
-        t_b = time.perf_counter() - t0
-        res["batch"] = {"volumes": len(vols), "sizes": [int(x) for x in sizes], "systems": 6 * len(vols),
-                        "seconds": t_b, "solves_per_s": 6 * len(vols) / t_b, "iterations": it}
+        cfg2["matvec_ms"] = {"fp64": t64, "fp32": t32, "solid_nodes": s.nsolid_nodes}
+    res["moduli_64"] = cfg2
+    # uneven sub-volumes, two components, one batch
+    rng = np.random.default_rng(0)
+    sizes = rng.integers(32, 97, size=64)
+    vols = [geo.two_component(geo.sphere_pack((int(L), int(L), int(L)), 0.35, 5.0, seed=100 + i), 0.7,
+                              seed=200 + i, radius=4.0) for i, L in enumerate(sizes)]
+    pars = (1.0, 0.1, 0.3, 0.02)
+    t0 = time.perf_counter()
+    Cb, KG, ms, it = batch_moduli(vols, *pars, precision="f32", rtol=1e-10)
+    t_b = time.perf_counter() - t0
+    nsamp = 8
+    t0 = time.perf_counter()
+    batch_moduli(vols[:nsamp], *pars, precision="f32", rtol=1e-10)
+    t_bs = time.perf_counter() - t0
+    t_ps = 0.0
+    for v in vols[:nsamp]:
+        t_ps += _plain_rate(v, range(6), 1e-10, pars)[0]
+    res["batch_64"] = {"volumes": len(vols), "sizes": [int(x) for x in sizes], "systems": 6 * len(vols),
+                       "nodes": int(sum(int(x) ** 3 for x in sizes)), "seconds": t_b,
+                       "solves_per_s": 6 * len(vols) / t_b, "iterations": it,
+                       "sample": f"first {nsamp} volumes, 6 cases, rtol 1e-10",
+                       "sample_plain_fp64_solves_per_s": 6 * nsamp / t_ps,
+                       "sample_batched_mg_solves_per_s": 6 * nsamp / t_bs, "sample_speedup": t_ps / t_bs}
+    # adjoint at 128^3: two components, lognormal stiffness (sigma 0.3, seed 1)
+    n5 = 128
+    ind5 = geo.two_component(geo.sphere_pack((n5, n5, n5), 0.35, 6.0, seed=5), 0.3, seed=6, radius=4.0)
+    stiff = np.exp(0.3 * np.random.default_rng(1).standard_normal(ind5.shape))
+    adj = {"cfg": "128^3 two-component sphere pack, lognormal stiffness sigma 0.3 seed 1, rtol 1e-10"}
+    for prec in ("f64", "f32"):
+        with LSM(ind5, *pars, precision=prec, stiff=stiff) as s:
+            t0 = time.perf_counter()
+            _, K5, G5 = s.moduli(1e-10, 200000)
+            t_f = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            s.adjoint(1.0, 0.0, 1e-10, 200000)
+            t_a = time.perf_counter() - t0
+        adj[prec] = {"forward_s": t_f, "adjoint_s": t_a, "grads_per_s": 1.0 / (t_f + t_a), "K": K5, "G": G5}
+    adj["speedup"] = adj["f32"]["grads_per_s"] / adj["f64"]["grads_per_s"]
+    steps = 50 if full else 5
+    tgt = np.array([adj["f32"]["K"], adj["f32"]["G"]]) * np.array([1.05, 0.97])
+    t0 = time.perf_counter()
+    hist, losses = calibrate(ind5, np.array(pars), tgt, steps=steps, rtol=1e-10, precision="f32", stiff=stiff)
+    t_c = time.perf_counter() - t0
+    adj["calibration"] = {"steps": steps, "seconds": t_c, "s_per_step": t_c / steps,
+                          "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
+                          "target_KG": tgt.tolist(), "fitted_ab": hist[-1].tolist()}
+    res["adjoint_128"] = adj
     return res
 
 
@@ -185,6 +225,7 @@ def main():
     ap.add_argument("--no-lsm", dest="lsm", action="store_false")
     ap.add_argument("--no-cpu", dest="cpu", action="store_false", default=True)
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB")
+    ap.add_argument("--full", action="store_true", help="50-step calibration in the LSM rows")
     args = ap.parse_args()
     ws = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or ws != 1:
@@ -233,7 +274,7 @@ def main():
         "lbm_diag": {"permr": lbm["permr"], "permb": lbm["permb"], "sat_b": lbm["sat_b"]},
     }
     if args.lsm:
-        out["lsm"] = run_lsm()
+        out["lsm"] = run_lsm(args.full)
         log(f"LSM {json.dumps(out['lsm'])[:400]}")
     if args.cpu:
         out["cpu_baseline"] = cpu_baseline(ind, inp, blue)
