@@ -136,3 +136,49 @@ def test_element_matrices_structure(mats):
         u[3 * p] = x[0]
     assert u @ KA @ u == pytest.approx(3.0) and u @ KB @ u == pytest.approx(8.0)
     assert (SA @ u)[0] == pytest.approx(3.0) and (SB @ u)[0] == pytest.approx(8.0)
+
+
+def _node_stencil_from_header():
+    """The generated node_stencil routine of lsm_element.h as a function of
+    (ka[8], kb[8]) -> (27, 3, 3) blocks, read from the header text."""
+    import re
+    text = open(os.path.join(ob.ROOT, "porescale_amd", "csrc", "lsm_element.h")).read()
+    body = text[text.index("void node_stencil("):]
+    body = body[:body.index("\n}\n")]
+    lines = re.findall(r"out\[(\d+) \* stride\] = (.*);", body)
+    assert len(lines) == 243 and sorted(int(k) for k, _ in lines) == list(range(243))
+    terms = []
+    for k, expr in lines:
+        tl = []
+        for coef, arr, p in re.findall(r"T\(([-0-9.e]+)\) \* (ka|kb)\[(\d)\]", expr):
+            tl.append((float(coef), arr, int(p)))
+        assert expr == "T(0)" or len(tl) == expr.count("*")
+        terms.append((int(k), tl))
+
+    def stencil(ka8, kb8):
+        out = np.zeros(243)
+        for k, tl in terms:
+            out[k] = sum(c * (ka8[p] if arr == "ka" else kb8[p]) for c, arr, p in tl)
+        return out.reshape(27, 3, 3)
+    return stencil
+
+
+def test_generated_node_stencil_equals_element_assembly(mats):
+    """Row blocks K(n, n + offset) of the assembled operator (used to build
+    the multigrid hierarchy) against a direct sum of the element matrices of
+    the 8 cubes around the node, with random per-cube scales."""
+    KA, KB = mats[0], mats[1]
+    stencil = _node_stencil_from_header()
+    rng = np.random.default_rng(4)
+    for _ in range(3):
+        ka8, kb8 = rng.random(8), rng.random(8)
+        ref = np.zeros((27, 3, 3))
+        for p in range(8):          # node is corner p of the cube at node - CORNER[p]
+            for q in range(8):      # neighbour corner q of that cube
+                off = CORNER[q] - CORNER[p]
+                s = (off[0] + 1) + 3 * (off[1] + 1) + 9 * (off[2] + 1)
+                ref[s] += ka8[p] * KA[3 * p:3 * p + 3, 3 * q:3 * q + 3] + kb8[p] * KB[3 * p:3 * p + 3, 3 * q:3 * q + 3]
+        np.testing.assert_allclose(stencil(ka8, kb8), ref, rtol=0, atol=1e-14)
+    # the rows of a homogeneous solid sum to zero (rigid translations)
+    st = stencil(np.ones(8), np.full(8, 0.1))
+    np.testing.assert_allclose(st.sum(axis=0), 0.0, atol=1e-13)

@@ -187,3 +187,20 @@ def test_cyclic_axis_permutation_maps_all_six_cases():
     assert Kr == pytest.approx(K, rel=1e-10) and Gr == pytest.approx(G, rel=1e-10)
     # a non-trivial geometry: the six cases really differ
     assert abs(C[0, 0] - C[1, 1]) > 1e-4 * C[0, 0]
+
+
+def test_multigrid_preconditioner_cuts_iterations(monkeypatch):
+    """The FP32 inner solves use the multigrid V-cycle; block Jacobi
+    (PSX_MG=0) must need several times more iterations for the same answer."""
+    ind = geo.two_component(geo.sphere_pack((24, 20, 22), 0.35, 3.0, seed=3), 0.3, seed=4, radius=2.0)
+    out = {}
+    for mg in ("0", "1"):
+        monkeypatch.setenv("PSX_MG", mg)
+        _say(f"case multigrid PSX_MG={mg}")
+        with LSM(ind, 1.0, 0.1, 0.3, 0.02, precision="f32") as s:
+            it, rel = s.solve("yz", 1e-10, 50000)
+            _, m = s.calc_stress("yz")
+        assert rel <= 1e-10
+        out[mg] = (it, m[1, 2])
+    assert out["1"][0] * 3 < out["0"][0]
+    assert out["1"][1] == pytest.approx(out["0"][1], rel=1e-9)
