@@ -18,7 +18,8 @@ Also reported (objects in the same line):
   cpu_baseline the oracle (C restatement of the reference, 1 thread) timed on
                this host over a bounded sample of the same LBM workload.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-lsm] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-lsm] [--no-cpu] [--full]
+    python bench.py --lsm-only         # the LSM rows alone (profiling)
     python bench.py --calib-copy MIB   # plain float32 elementwise kernel (torch.neg)
                                        # over MIB MiB in and out, ten launches:
                                        # scale factor for FETCH_SIZE/WRITE_SIZE
@@ -226,6 +227,7 @@ def main():
     ap.add_argument("--no-cpu", dest="cpu", action="store_false", default=True)
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB")
     ap.add_argument("--full", action="store_true", help="50-step calibration in the LSM rows")
+    ap.add_argument("--lsm-only", action="store_true", help="only the LSM rows (for profiling them)")
     args = ap.parse_args()
     ws = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or ws != 1:
@@ -235,6 +237,9 @@ def main():
         sys.exit(2)
     if args.calib_copy:
         calib_copy(args.calib_copy)
+        return
+    if args.lsm_only:
+        print(json.dumps({"lsm": run_lsm(args.full)}), flush=True)
         return
     from porescale_amd import _lib
     dev, ncu, gb = _lib.device_info()

@@ -35,6 +35,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <type_traits>
 #include <stdexcept>
 #include <vector>
@@ -665,6 +667,95 @@ namespace {
 
 const int CASE_PD[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 2}, {2, 0}, {0, 1}};
 
+// (Re)uploads the tile lists and volume descriptors of the batch.
+int upload_tiles(Batch &B, hipStream_t st) {
+    if (B.d_vols) (void)hipFree(B.d_vols);
+    if (B.d_tvol) (void)hipFree(B.d_tvol);
+    if (B.d_tstart) (void)hipFree(B.d_tstart);
+    B.d_vols = nullptr; B.d_tvol = nullptr; B.d_tstart = nullptr;
+    RC(dalloc(&B.d_vols, B.vols.size()));
+    RC(dalloc(&B.d_tvol, B.tvol.size()));
+    RC(dalloc(&B.d_tstart, B.tstart.size()));
+    PSX_HIP(hipMemcpyAsync(B.d_vols, B.vols.data(), sizeof(VolDesc) * B.vols.size(), hipMemcpyHostToDevice, st));
+    PSX_HIP(hipMemcpyAsync(B.d_tvol, B.tvol.data(), sizeof(int32_t) * B.tvol.size(), hipMemcpyHostToDevice, st));
+    PSX_HIP(hipMemcpyAsync(B.d_tstart, B.tstart.data(), sizeof(int32_t) * B.tstart.size(), hipMemcpyHostToDevice, st));
+    PSX_HIP(hipStreamSynchronize(st));
+    return PSX_OK;
+}
+
+// Tile-shape autotuner: times the FP32 operator (the inner solver's
+// matvec) on this batch for each candidate tile shape and keeps the
+// fastest.  Results are cached per set of volume dimensions for the life
+// of the process.  PSX_LSM_TILE=linear or "lbx,lby,lbz" (log2 brick sides)
+// forces a shape.
+int autotune_tiles(Batch &B, hipStream_t st) {
+    static std::mutex mu;
+    static std::map<std::vector<int>, int> cache;
+    std::vector<int> key;
+    for (auto &d : B.vols) { key.push_back(d.nx); key.push_back(d.ny); key.push_back(d.nz); }
+    int forced = -1;
+    if (const char *e = std::getenv("PSX_LSM_TILE")) {
+        int a, b, c;
+        if (std::strcmp(e, "linear") == 0) forced = 0;
+        else if (std::sscanf(e, "%d,%d,%d", &a, &b, &c) == 3 && a >= 0 && b >= 0 && c >= 0 && a + b + c == 8)
+            forced = tile_shape_code(a, b, c);
+    }
+    int best = forced;
+    if (best < 0) {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) best = it->second;
+    }
+    if (best < 0) {
+        const int cand[] = {0, tile_shape_code(3, 3, 2), tile_shape_code(4, 2, 2), tile_shape_code(2, 3, 3),
+                            tile_shape_code(4, 3, 1), tile_shape_code(5, 2, 1), tile_shape_code(3, 2, 3),
+                            tile_shape_code(6, 1, 1)};
+        const int R = B.Ntot * MAXR * 3 * 4 < ((int64_t)512 << 20) ? MAXR : 1;
+        float *u = nullptr, *y = nullptr;
+        RC(dalloc(&u, (size_t)R * 3 * B.Ntot));
+        RC(dalloc(&y, (size_t)R * 3 * B.Ntot));
+        int64_t n = (int64_t)R * 3 * B.Ntot;
+        hipLaunchKernelGGL(k_zero<float>, dim3((unsigned)psx::ceil_div(n, 256)), dim3(256), 0, st, u, n);
+        hipEvent_t e0, e1;
+        PSX_HIP(hipEventCreate(&e0));
+        PSX_HIP(hipEventCreate(&e1));
+        float best_ms = 1e30f;
+        best = 0;
+        for (int shape : cand) {
+            make_tiles(B, shape);
+            RC(upload_tiles(B, st));
+            auto launch = [&]() {
+                hipLaunchKernelGGL((k_apply<float, false, false, false>), dim3(grid_for(B.ntiles)), dim3(TILE), 0, st,
+                                   B.d_vols, B.d_tvol, B.d_tstart, B.ntiles, R, (const float *)B.d_ka32,
+                                   (const float *)B.d_kb32, (const float *)u, y, B.Ntot, (const SysState *)nullptr,
+                                   (const Jump *)nullptr, (double *)nullptr);
+            };
+            launch();
+            PSX_HIP(hipEventRecord(e0, st));
+            for (int k = 0; k < 5; k++) launch();
+            PSX_HIP(hipEventRecord(e1, st));
+            PSX_HIP(hipEventSynchronize(e1));
+            PSX_HIP(hipGetLastError());
+            float ms = 0.f;
+            PSX_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (std::getenv("PSX_MG_DEBUG"))
+                std::fprintf(stderr, "tile shape %d: %d tiles, %.4f ms per apply (R=%d)\n", shape, B.ntiles, ms / 5, R);
+            if (ms < best_ms) { best_ms = ms; best = shape; }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipFree(u);
+        (void)hipFree(y);
+        std::lock_guard<std::mutex> lk(mu);
+        cache[key] = best;
+    }
+    if (B.shape != best || (!B.vols.empty() && B.vols[0].pad_ != best)) {
+        make_tiles(B, best);
+        RC(upload_tiles(B, st));
+    }
+    return PSX_OK;
+}
+
 // Builds the batch description and uploads voxel scales.
 int batch_build(Batch &B, hipStream_t st, const psx_lsm_params &p, int nvol, const int32_t *dims,
                 const int32_t *ind, const double *stiff) {
@@ -715,6 +806,7 @@ int batch_build(Batch &B, hipStream_t st, const psx_lsm_params &p, int nvol, con
     PSX_HIP(hipMemcpyAsync(B.d_kb, B.kb_h.data(), sizeof(double) * off, hipMemcpyHostToDevice, st));
     PSX_HIP(hipMemcpyAsync(B.d_ka32, ka32.data(), sizeof(float) * off, hipMemcpyHostToDevice, st));
     PSX_HIP(hipMemcpyAsync(B.d_kb32, kb32.data(), sizeof(float) * off, hipMemcpyHostToDevice, st));
+    RC(autotune_tiles(B, st));
     int g = grid_for(B.ntiles);
     hipLaunchKernelGGL(k_diag_inv<double>, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles,
                        B.d_ka, B.d_kb, B.d_minv, B.Ntot);

@@ -95,7 +95,34 @@ __device__ __forceinline__ T block_sum(T v, T *sh) {
     return s;  // valid in thread 0
 }
 
+// Tile shapes.  VolDesc::pad_ == 0: a tile is 256 consecutive nodes (x
+// fastest) and `start` its first node.  Otherwise pad_ = 1 | lbx << 4 |
+// lby << 8 | lbz << 12 describes a brick of 2^lbx x 2^lby x 2^lbz = 256
+// nodes and `start` is the brick index (x fastest) within the volume.
+inline int tile_shape_code(int lbx, int lby, int lbz) { return 1 | lbx << 4 | lby << 8 | lbz << 12; }
+
+inline int64_t tiles_for(const VolDesc &d, int shape) {
+    if (!(shape & 1)) return ((int64_t)d.nx * d.ny * d.nz + 255) / 256;
+    const int lbx = (shape >> 4) & 15, lby = (shape >> 8) & 15, lbz = (shape >> 12) & 15;
+    const int64_t nbx = (d.nx + (1 << lbx) - 1) >> lbx, nby = (d.ny + (1 << lby) - 1) >> lby,
+                  nbz = (d.nz + (1 << lbz) - 1) >> lbz;
+    return nbx * nby * nbz;
+}
+
 __device__ __forceinline__ void tile_node(const VolDesc &vd, int start, int &x, int &y, int &z, bool &ok) {
+    if (vd.pad_ & 1) {
+        const int s = vd.pad_;
+        const int lbx = (s >> 4) & 15, lby = (s >> 8) & 15, lbz = (s >> 12) & 15;
+        const int nbx = (vd.nx + (1 << lbx) - 1) >> lbx, nby = (vd.ny + (1 << lby) - 1) >> lby;
+        const int bxi = start % nbx, r2 = start / nbx, byi = r2 % nby, bzi = r2 / nby;
+        const int tid = (int)threadIdx.x;
+        x = (bxi << lbx) + (tid & ((1 << lbx) - 1));
+        y = (byi << lby) + ((tid >> lbx) & ((1 << lby) - 1));
+        z = (bzi << lbz) + (tid >> (lbx + lby));
+        ok = x < vd.nx && y < vd.ny && z < vd.nz;
+        if (!ok) x = y = z = 0;
+        return;
+    }
     int n = start + (int)threadIdx.x;
     int64_t N = (int64_t)vd.nx * vd.ny * vd.nz;
     ok = n < N;
@@ -108,6 +135,7 @@ __device__ __forceinline__ void tile_node(const VolDesc &vd, int start, int &x, 
 
 
 struct Batch {
+    int shape = 0;  // tile shape of every volume (VolDesc::pad_)
     std::vector<VolDesc> vols;
     std::vector<int32_t> tvol, tstart;
     int64_t Ntot = 0;
@@ -122,6 +150,26 @@ struct Batch {
     std::vector<uint8_t> comp_h;
     std::vector<double> stiff_h;
 };
+
+// Rebuilds the tile lists of every volume for a tile shape (host side).
+inline void make_tiles(Batch &B, int shape) {
+    B.shape = shape;
+    B.tvol.clear();
+    B.tstart.clear();
+    int tile = 0;
+    for (size_t v = 0; v < B.vols.size(); v++) {
+        VolDesc &d = B.vols[v];
+        d.pad_ = shape;
+        d.tile0 = tile;
+        d.ntile = (int)tiles_for(d, shape);
+        for (int k = 0; k < d.ntile; k++) {
+            B.tvol.push_back((int32_t)v);
+            B.tstart.push_back(shape & 1 ? k : k * 256);
+        }
+        tile += d.ntile;
+    }
+    B.ntiles = tile;
+}
 
 // Workgroups for a tile loop: at most 8 per CU, one per tile.
 inline int grid_for(int ntiles) {
