@@ -641,6 +641,18 @@ struct Solver {
     Jump *d_jump = nullptr;
     Jump jump_h{};
     MG mg;  // multigrid preconditioner of the FP32 inner solves (built on first use)
+    // instantiated iteration graphs, reused while the captured arguments match
+    struct GraphKey {
+        const void *X = nullptr;
+        double tol2 = 0.0;
+        int maxit = 0, chunk = 0;
+        bool mg = false;
+        bool operator==(const GraphKey &o) const {
+            return X == o.X && tol2 == o.tol2 && maxit == o.maxit && chunk == o.chunk && mg == o.mg;
+        }
+    };
+    hipGraphExec_t exec[2] = {nullptr, nullptr};  // [FP64, FP32]
+    GraphKey exec_key[2];
 };
 
 }  // namespace
@@ -847,6 +859,8 @@ void solver_free(Solver &S) {
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     mg_free(S.mg);
+    for (auto &e : S.exec)
+        if (e) (void)hipGraphExecDestroy(e);
     S = Solver();
 }
 
@@ -923,17 +937,31 @@ void cg_start(hipStream_t st, const Batch &B, Solver &S, SysState *sys, T *X, T 
 // and replayed.
 template <typename T>
 int cg_run(hipStream_t st, const Batch &B, Solver &S, SysState *sys, T *X, T *Rv, T *Z, T *P, T *Qv, const T *ka,
-           const T *kb, const T *minv, double tol, int maxit, int chunk, int use_graph, bool mg = false) {
+           const T *kb, const T *minv, double tol, int maxit, int chunk, int use_graph, bool mg = false,
+           int host_limit = -1) {
+    if (host_limit < 0) host_limit = maxit;
     double tol2 = tol * tol;
     std::vector<SysState> hs(S.S);
     hipGraphExec_t exec = nullptr;
     if (use_graph) {
-        hipGraph_t graph;
-        PSX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < chunk; k++) cg_iteration<T>(st, B, S, sys, X, Rv, Z, P, Qv, ka, kb, minv, tol2, maxit, mg);
-        PSX_HIP(hipStreamEndCapture(st, &graph));
-        PSX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-        (void)hipGraphDestroy(graph);
+        // the chunk of iterations is captured once per solver and argument set
+        const int slot = std::is_same<T, float>::value ? 1 : 0;
+        Solver::GraphKey key;
+        key.X = X; key.tol2 = tol2; key.maxit = maxit; key.chunk = chunk; key.mg = mg;
+        if (!S.exec[slot] || !(S.exec_key[slot] == key)) {
+            if (S.exec[slot]) (void)hipGraphExecDestroy(S.exec[slot]);
+            S.exec[slot] = nullptr;
+            hipGraph_t graph;
+            PSX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < chunk; k++)
+                cg_iteration<T>(st, B, S, sys, X, Rv, Z, P, Qv, ka, kb, minv, tol2, maxit, mg);
+            PSX_HIP(hipStreamEndCapture(st, &graph));
+            hipError_t ie = hipGraphInstantiate(&S.exec[slot], graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            PSX_HIP(ie);
+            S.exec_key[slot] = key;
+        }
+        exec = S.exec[slot];
     }
     int done_iters = 0;
     while (true) {
@@ -948,9 +976,8 @@ int cg_run(hipStream_t st, const Batch &B, Solver &S, SysState *sys, T *X, T *Rv
         PSX_HIP(hipStreamSynchronize(st));
         bool any = false;
         for (auto &s : hs) any |= s.active != 0;
-        if (!any || done_iters >= maxit) break;
+        if (!any || done_iters >= host_limit) break;
     }
-    if (exec) (void)hipGraphExecDestroy(exec);
     return PSX_OK;
 }
 
@@ -959,7 +986,7 @@ int cg_run(hipStream_t st, const Batch &B, Solver &S, SysState *sys, T *X, T *Rv
 // x += ||r|| d, until ||r|| <= tol ||b||.
 int solve_batch(hipStream_t st, const Batch &B, Solver &S, bool f32, double tol, int maxit, int use_graph,
                 std::vector<SysState> *out_state, int *out_iters) {
-    const int chunk = 16;
+    int chunk = 16;
     if (const char *e = std::getenv("PSX_LSM_GRAPH")) use_graph = std::atoi(e);  // 0: plain launches (profiling)
     int total_iters = 0;
     if (!f32) {
@@ -984,11 +1011,14 @@ int solve_batch(hipStream_t st, const Batch &B, Solver &S, bool f32, double tol,
     int gb = (int)psx::ceil_div(n, 256);
     int g = grid_for(B.ntiles);
     hipLaunchKernelGGL(k_zero<double>, dim3(gb), dim3(256), 0, st, S.X, n);
-    const double inner_tol = 1e-5;
+    double inner_tol = 1e-5;
+    if (const char *e = std::getenv("PSX_LSM_INNER_TOL")) inner_tol = std::atof(e);
     // multigrid-preconditioned inner solves (PSX_MG=0 selects block Jacobi)
     const char *mge = std::getenv("PSX_MG");
     const bool use_mg = !mge || std::atoi(mge) != 0;
     if (use_mg && !S.mg.built) RC(mg_build(st, B, S.R, mg_params_from_env(), S.mg));
+    if (use_mg) chunk = 4;  // few iterations per inner solve: check convergence more often
+    if (const char *e = std::getenv("PSX_LSM_CHUNK")) chunk = std::max(1, std::atoi(e));
     std::vector<SysState> ho(S.S);
     const int max_outer = 16;
     for (int outer = 0; outer <= max_outer; outer++) {
@@ -1012,7 +1042,7 @@ int solve_batch(hipStream_t st, const Batch &B, Solver &S, bool f32, double tol,
                            S.sys);
         PSX_HIP(hipGetLastError());
         RC(cg_run<float>(st, B, S, S.sys, S.X32, S.R32, S.Z32, S.P32, S.Q32, B.d_ka32, B.d_kb32, B.d_minv32,
-                         inner_tol, maxit - total_iters, chunk, use_graph, use_mg));
+                         inner_tol, maxit, chunk, use_graph, use_mg, maxit - total_iters));
         std::vector<SysState> hs(S.S);
         PSX_HIP(hipMemcpy(hs.data(), S.sys, sizeof(SysState) * S.S, hipMemcpyDeviceToHost));
         int mx = 0;
