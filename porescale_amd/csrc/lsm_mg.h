@@ -33,7 +33,8 @@ struct MGLevel {
 
 struct MGParams {
     int levels = 4;        // at most this many levels (including level 0)
-    int nu = 2;            // Chebyshev degree of each smoothing pass
+    int nu = 3;            // Chebyshev degree of each smoothing pass (levels >= 1)
+    int nu0 = 1;           // the same on the finest level (its matvec is the costliest)
     int coarse_deg = 4;    // Chebyshev degree on the coarsest level
     float ratio = 30.f;    // smoothing interval [lam/ratio, lam]
     float coarse_ratio = 200.f;
@@ -54,7 +55,7 @@ void mg_free(MG &mg);
 // z = M^-1 r for every active system; r, z are [R][3][Ntot] FP32 on the
 // batch grid.  prz (may be null) receives the (tile, rhs) partials of r.z.
 void mg_apply(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, const float *r, float *z, double *prz);
-// Parameters from the environment (PSX_MG_LEVELS, PSX_MG_NU, PSX_MG_CDEG,
+// Parameters from the environment (PSX_MG_LEVELS, PSX_MG_NU, PSX_MG_NU0, PSX_MG_CDEG,
 // PSX_MG_RATIO, PSX_MG_CRATIO, PSX_MG_POWER); defaults otherwise.
 MGParams mg_params_from_env();
 

@@ -606,8 +606,9 @@ float *vcycle(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, int l
         }
         return dA;
     }
+    const int nu = l == 0 ? p.nu0 : p.nu;
     // pre-smoothing: nu updates, the last one with the residual update
-    for (int k = 0; k + 1 < p.nu; k++) {
+    for (int k = 0; k + 1 < nu; k++) {
         launch_step<true>(st, L, elem, mg.R, sys, k, lv.x, lv.r, dA, dB);
         std::swap(dA, dB);
     }
@@ -624,7 +625,7 @@ float *vcycle(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, int l
     // post-smoothing: restart from the corrected x, nu updates
     launch_step<true>(st, L, elem, mg.R, sys, -1, lv.x, lv.r, dA, dB);
     std::swap(dA, dB);
-    for (int k = 0; k + 1 < p.nu; k++) {
+    for (int k = 0; k + 1 < nu; k++) {
         launch_step<true>(st, L, elem, mg.R, sys, k, lv.x, lv.r, dA, dB);
         std::swap(dA, dB);
     }
@@ -665,6 +666,7 @@ MGParams mg_params_from_env() {
     MGParams p;
     p.levels = std::max(1, env_int("PSX_MG_LEVELS", p.levels));
     p.nu = std::max(1, env_int("PSX_MG_NU", p.nu));
+    p.nu0 = std::max(1, env_int("PSX_MG_NU0", p.nu0));
     p.coarse_deg = std::max(1, env_int("PSX_MG_CDEG", p.coarse_deg));
     p.ratio = std::max(1.5f, env_float("PSX_MG_RATIO", p.ratio));
     p.coarse_ratio = std::max(1.5f, env_float("PSX_MG_CRATIO", p.coarse_ratio));
