@@ -16,7 +16,10 @@
 //                FP32 mode (lbm_fast.hip): per (q,node) T = (fr+fb) - tr_q*rr0
 //                and R = fr/(fr+fb) as floats, post-collision, in AoSoA blocks
 //                of 64 nodes, streamed by a pull through src[18][nl] (source
-//                node of each incoming link, -1 for bounce-back).  T keeps the
+//                node of each incoming link, -1 for bounce-back), held on the
+//                device as int16 rank differences within x-planes plus
+//                per-plane bases (36 B per node instead of 72) when every
+//                difference fits, as the int32 table otherwise.  T keeps the
 //                small deviation of the total population (momentum is exact to
 //                FP32 ulps of the deviation) and R is exactly 1 or 0 for
 //                single-colour populations, so pure-phase nodes keep exactly
@@ -58,6 +61,8 @@ struct psx_lbm {
     LbmConst c{};
     hipStream_t stream = nullptr;
     int32_t *d_dst = nullptr, *d_vox = nullptr;
+    int16_t *d_src16 = nullptr;  // FP32 path: compressed pull table (null: use d_dst)
+    int32_t *d_pbase = nullptr;  // [3][nx] per-plane bases of d_src16
     void *d_pop[2] = {nullptr, nullptr};
     void *d_phi = nullptr;
     double *d_partial = nullptr, *d_red = nullptr;
@@ -179,8 +184,8 @@ int launch_density(psx_lbm *h, int buf) {
     int64_t nb = psx::ceil_div(h->nl, 256);
     if (nb == 0) return PSX_OK;
     if (h->f32) {
-        PSX_HIP(psx_lbm_fast::density(h->c, (const float *)h->d_pop[buf], h->d_dst, h->d_vox, (double *)h->d_phi,
-                                      h->xcd, h->stream));
+        PSX_HIP(psx_lbm_fast::density(h->c, (const float *)h->d_pop[buf], h->d_dst, h->d_src16, h->d_pbase,
+                                      h->d_vox, (double *)h->d_phi, h->xcd, h->stream));
     } else {
         StoreF64 s{(double *)h->d_pop[buf], h->nl};
         hipLaunchKernelGGL(k_density<StoreF64>, dim3((unsigned)nb), dim3(256), 0, h->stream, h->c, s,
@@ -195,8 +200,8 @@ int launch_collide(psx_lbm *h) {
     if (nb == 0) return PSX_OK;
     int a = h->cur, b = 1 - h->cur;
     if (h->f32) {
-        PSX_HIP(psx_lbm_fast::collide(h->c, (const float *)h->d_pop[a], (float *)h->d_pop[b], h->d_dst, h->d_vox,
-                                      (const double *)h->d_phi, h->xcd, h->stream));
+        PSX_HIP(psx_lbm_fast::collide(h->c, (const float *)h->d_pop[a], (float *)h->d_pop[b], h->d_dst, h->d_src16,
+                                      h->d_pbase, h->d_vox, (const double *)h->d_phi, h->xcd, h->stream));
     } else {
         StoreF64 si{(double *)h->d_pop[a], h->nl}, so{(double *)h->d_pop[b], h->nl};
         hipLaunchKernelGGL(k_collide<StoreF64>, dim3((unsigned)nb), dim3(256), 0, h->stream, h->c, si, so,
@@ -255,6 +260,12 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
                 }
             }
     h->nl = nl;
+    if (h->f32 && (int64_t)aos_pad(nl > 0 ? nl : 1) * 2 * Q >= (int64_t)INT32_MAX) {
+        psx::set_error("psx_lbm_create: %lld fluid nodes exceed the FP32 path's 32-bit population index",
+                       (long long)nl);
+        delete h;
+        return PSX_EINVAL;
+    }
     h->dst_h.assign(18 * (size_t)nl, 0);
     std::vector<int32_t> vox(nl);
     for (int64_t n = 0; n < nl; n++) {
@@ -278,6 +289,39 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
             }
         h->dst_h.swap(src);  // the FP32 path keeps src in place of dst
     }
+    // Compressed pull table (FP32 path): the source of direction q lies in
+    // plane X - c_qx; its index is n + pbase[c_qx + 1][X] + d with d the
+    // difference of the two nodes' ranks within their planes, which fits
+    // int16 on porous media (adjacent planes are alike).  Kept only if every
+    // residual fits; PSX_LBM_IDX16=0 disables it.
+    std::vector<int16_t> s16;
+    std::vector<int32_t> pbase;
+    bool use16 = false;
+    if (h->f32 && nl > 0) {
+        const char *e16 = std::getenv("PSX_LBM_IDX16");
+        use16 = !e16 || std::atoi(e16) != 0;
+    }
+    if (use16) {
+        std::vector<int64_t> pstart(nx + 1, 0);
+        for (int64_t n = 0; n < nl; n++) pstart[h->icor[3 * n]]++;  // counts of plane x-1 at index x
+        for (int X = 0; X < nx; X++) pstart[X + 1] += pstart[X];
+        pbase.assign(3 * (size_t)nx, 0);
+        for (int cx = -1; cx <= 1; cx++)
+            for (int X = 0; X < nx; X++)
+                pbase[(size_t)(cx + 1) * nx + X] = (int32_t)(pstart[((X - cx) % nx + nx) % nx] - pstart[X]);
+        s16.assign(18 * (size_t)nl, 0);
+        for (int q = 1; q < Q && use16; q++) {
+            const int cx = H_IV[q][0];
+            for (int64_t n = 0; n < nl; n++) {
+                int32_t sidx = h->dst_h[(size_t)(q - 1) * nl + n];
+                if (sidx < 0) { s16[(size_t)(q - 1) * nl + n] = (int16_t)-32768; continue; }
+                const int X = h->icor[3 * n] - 1;
+                int64_t d = (int64_t)sidx - n - pbase[(size_t)(cx + 1) * nx + X];
+                if (d < -32767 || d > 32767) { use16 = false; break; }
+                s16[(size_t)(q - 1) * nl + n] = (int16_t)d;
+            }
+        }
+    }
     fill_const(h);
     PSX_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     size_t popb = h->f32 ? (size_t)aos_pad(nl > 0 ? nl : 1) * 2 * Q * sizeof(float)
@@ -287,6 +331,12 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
     PSX_HIP(hipMemsetAsync(h->d_pop[0], 0, popb, h->stream));
     PSX_HIP(hipMalloc(&h->d_dst, sizeof(int32_t) * 18 * (size_t)(nl > 0 ? nl : 1)));
     PSX_HIP(hipMalloc(&h->d_vox, sizeof(int32_t) * (size_t)(nl > 0 ? nl : 1)));
+    if (use16) {
+        PSX_HIP(hipMalloc(&h->d_src16, sizeof(int16_t) * 18 * (size_t)nl));
+        PSX_HIP(hipMalloc(&h->d_pbase, sizeof(int32_t) * 3 * (size_t)nx));
+        PSX_HIP(hipMemcpyAsync(h->d_src16, s16.data(), sizeof(int16_t) * 18 * nl, hipMemcpyHostToDevice, h->stream));
+        PSX_HIP(hipMemcpyAsync(h->d_pbase, pbase.data(), sizeof(int32_t) * 3 * nx, hipMemcpyHostToDevice, h->stream));
+    }
     size_t phib = (size_t)nv * sizeof(double);
     PSX_HIP(hipMalloc(&h->d_phi, phib));
     int64_t nb = std::max<int64_t>(1, psx::ceil_div(nl, 256));
@@ -308,7 +358,8 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
 extern "C" void psx_lbm_destroy(psx_lbm *h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    void *bufs[] = {h->d_pop[0], h->d_pop[1], h->d_dst, h->d_vox, h->d_phi, h->d_partial, h->d_red};
+    void *bufs[] = {h->d_pop[0], h->d_pop[1], h->d_dst, h->d_vox, h->d_phi, h->d_partial, h->d_red, h->d_src16,
+                    h->d_pbase};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (h->stream) (void)hipStreamDestroy(h->stream);

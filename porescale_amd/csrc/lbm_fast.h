@@ -4,13 +4,16 @@
 
 namespace psx_lbm_fast {
 
-// Pull + collide over all nodes; `in`/`out` are AoSoA (T, R) blocks, src is
-// [18][nl] (-1 = bounce-back).
+// Pull + collide over all nodes; `in`/`out` are AoSoA (T, R) blocks.  The
+// pull table is src16 (int16 residuals, -32768 = bounce-back) with the
+// per-plane bases pbase[3][nx] when src16 is not null, else src [18][nl]
+// (-1 = bounce-back).
 hipError_t collide(const psx_lbm_dev::LbmConst &c, const float *in, float *out, const int32_t *src,
-                   const int32_t *vox, const double *phi, int xcd, hipStream_t st);
+                   const int16_t *src16, const int32_t *pbase, const int32_t *vox, const double *phi, int xcd,
+                   hipStream_t st);
 // Pulled densities -> colour field phi at every node's voxel.
-hipError_t density(const psx_lbm_dev::LbmConst &c, const float *in, const int32_t *src, const int32_t *vox,
-                   double *phi, int xcd, hipStream_t st);
+hipError_t density(const psx_lbm_dev::LbmConst &c, const float *in, const int32_t *src, const int16_t *src16,
+                   const int32_t *pbase, const int32_t *vox, double *phi, int xcd, hipStream_t st);
 // check_state partial sums (k_diag) of the pulled state.
 hipError_t diag(const psx_lbm_dev::LbmConst &c, const float *in, const int32_t *src, const int32_t *vox,
                 const double *phi, double *partial, hipStream_t st);

@@ -17,6 +17,47 @@ using namespace psx_lbm_dev;
 
 namespace {
 
+// x component of c_q, as a compile-time function of q
+__host__ __device__ constexpr int cxq(int q) {
+    return q == 1 || q == 7 || q == 9 || q == 11 || q == 13 ? 1 : q == 2 || q == 8 || q == 10 || q == 12 || q == 14 ? -1 : 0;
+}
+
+// Pull table in one of two forms.  I16: src = i + pb[cx_q + 1] + d with d
+// the int16 residual (rank difference of source and node within their
+// x-planes) and pb the per-plane bases of the node's plane; d = -32768 marks
+// a bounce-back link.  Otherwise the int32 source index (-1: bounce-back).
+struct PullIdx {
+    const int32_t *__restrict__ src;
+    const int16_t *__restrict__ src16;
+    const int32_t *__restrict__ pbase;  // [3][nx]
+};
+
+template <bool I16>
+__device__ __forceinline__ void load_bases(const LbmConst &c, const PullIdx &P, int32_t v, int32_t (&pb)[3]) {
+    if (I16) {
+        const int X = v / (c.ny * c.nz);
+        pb[0] = P.pbase[X]; pb[1] = P.pbase[c.nx + X]; pb[2] = P.pbase[2 * c.nx + X];
+    }
+}
+
+// slot (direction k, node j) that direction q of node i pulls from
+template <bool I16>
+__device__ __forceinline__ void pull_from(const PullIdx &P, const int32_t (&pb)[3], int64_t nl, int64_t i, int q,
+                                          int64_t &j, int &k) {
+    j = i;
+    k = q;
+    if (q == 0) return;
+    if (I16) {
+        int d = P.src16[(int64_t)(q - 1) * nl + i];
+        if (d == -32768) k = opp(q);
+        else j = i + pb[cxq(q) + 1] + d;
+    } else {
+        int32_t s = P.src[(int64_t)(q - 1) * nl + i];
+        if (s >= 0) j = s;
+        else k = opp(q);
+    }
+}
+
 __device__ __forceinline__ void put(const LbmConst &c, float *__restrict__ out, int q, int64_t j, double fr,
                                     double fb) {
     double f = fr + fb;
@@ -141,28 +182,28 @@ __device__ __forceinline__ void collide_fast(const LbmConst &c, double t1, doubl
     f[17] = f[17] - mb17 - F12 * (-bf[1] + bf[2]);
 }
 
+template <bool I16>
 __global__ __launch_bounds__(BLOCK) void k_collide_f32(const LbmConst c, const float *__restrict__ in,
-                                                       float *__restrict__ out, const int32_t *__restrict__ src,
+                                                       float *__restrict__ out, const PullIdx P,
                                                        const int32_t *__restrict__ vox,
                                                        const double *__restrict__ phi, int xcd) {
     int64_t b = xcd ? psx::xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
     int64_t i = b * blockDim.x + threadIdx.x;
     const int64_t nl = c.nl;
     if (i >= nl) return;
+    const int32_t v = vox[i];
+    int32_t pb[3];
+    load_bases<I16>(c, P, v, pb);
     double f[Q];
-    int64_t at[Q];  // where direction q is pulled from
+    int32_t at[Q];  // where direction q is pulled from (AoSoA index; < 2^31 for nl < 5.6e7)
     double rr = 0.0, rb = 0.0;
     bool allred = true, allblue = true;
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-        int64_t j = i;
-        int k = q;
-        if (q > 0) {
-            int32_t s = src[(int64_t)(q - 1) * nl + i];
-            if (s >= 0) j = s;
-            else k = opp(q);
-        }
-        at[q] = aos(k, j);
+        int64_t j;
+        int k;
+        pull_from<I16>(P, pb, nl, i, q, j, k);
+        at[q] = (int32_t)aos(k, j);
         double t = c.w0[q] + (double)in[at[q]];
         float r = in[at[q] + Q * AOS];
         double a = (double)r * t;
@@ -197,7 +238,7 @@ __global__ __launch_bounds__(BLOCK) void k_collide_f32(const LbmConst c, const f
     } else {
         mode = 2;
         int x, y, z;
-        vox_xyz(c, vox[i], x, y, z);
+        vox_xyz(c, v, x, y, z);
         double Fs[3];
         calc_fs<double, false>(c, phi, x, y, z, Cg, Fs);
         r = rr + rb;
@@ -244,50 +285,58 @@ __global__ __launch_bounds__(BLOCK) void k_collide_f32(const LbmConst c, const f
     }
 }
 
+template <bool I16>
 __global__ __launch_bounds__(BLOCK) void k_density_f32(const LbmConst c, const float *__restrict__ in,
-                                                       const int32_t *__restrict__ src,
-                                                       const int32_t *__restrict__ vox, double *__restrict__ phi,
-                                                       int xcd) {
+                                                       const PullIdx P, const int32_t *__restrict__ vox,
+                                                       double *__restrict__ phi, int xcd) {
     int64_t b = xcd ? psx::xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
     int64_t i = b * blockDim.x + threadIdx.x;
     const int64_t nl = c.nl;
     if (i >= nl) return;
+    const int32_t v = vox[i];
+    int32_t pb[3];
+    load_bases<I16>(c, P, v, pb);
     double rr = 0.0, rb = 0.0;
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-        int64_t j = i;
-        int k = q;
-        if (q > 0) {
-            int32_t s = src[(int64_t)(q - 1) * nl + i];
-            if (s >= 0) j = s;
-            else k = opp(q);
-        }
+        int64_t j;
+        int k;
+        pull_from<I16>(P, pb, nl, i, q, j, k);
         int64_t a = aos(k, j);
         double t = c.w0[q] + (double)in[a];
         double rt = (double)in[a + Q * AOS] * t;
         rr += rt;
         rb += t - rt;
     }
-    phi[vox[i]] = (rr - rb) / (rr + rb);
+    phi[v] = (rr - rb) / (rr + rb);
 }
 
 }  // namespace
 
 namespace psx_lbm_fast {
 
-hipError_t collide(const LbmConst &c, const float *in, float *out, const int32_t *src, const int32_t *vox,
-                   const double *phi, int xcd, hipStream_t st) {
+hipError_t collide(const LbmConst &c, const float *in, float *out, const int32_t *src, const int16_t *src16,
+                   const int32_t *pbase, const int32_t *vox, const double *phi, int xcd, hipStream_t st) {
     int64_t nb = psx::ceil_div(c.nl, BLOCK);
     if (nb == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_collide_f32, dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, out, src, vox, phi, xcd);
+    // the collide kernel is latency-bound (occupancy 3): it takes the int32
+    // table, whose source indices need no dependent load of the plane base
+    (void)src16;
+    (void)pbase;
+    const PullIdx P{src, nullptr, nullptr};
+    hipLaunchKernelGGL(k_collide_f32<false>, dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, out, P, vox, phi, xcd);
     return hipGetLastError();
 }
 
-hipError_t density(const LbmConst &c, const float *in, const int32_t *src, const int32_t *vox, double *phi, int xcd,
-                   hipStream_t st) {
+hipError_t density(const LbmConst &c, const float *in, const int32_t *src, const int16_t *src16,
+                   const int32_t *pbase, const int32_t *vox, double *phi, int xcd, hipStream_t st) {
     int64_t nb = psx::ceil_div(c.nl, BLOCK);
     if (nb == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_density_f32, dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, src, vox, phi, xcd);
+    const PullIdx P{src, src16, pbase};
+    if (src16)
+        hipLaunchKernelGGL(k_density_f32<true>, dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, P, vox, phi, xcd);
+    else
+        hipLaunchKernelGGL(k_density_f32<false>, dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, P, vox, phi, xcd);
     return hipGetLastError();
 }
 

@@ -169,3 +169,22 @@ def test_fp32_state_round_trip_and_first_step():
         np.testing.assert_allclose(a, b, rtol=0, atol=2e-8)
     for a, b in zip(out["f32"][1], out["f64"][1]):
         np.testing.assert_allclose(a, b, rtol=0, atol=5e-8)
+
+
+def test_fp32_compressed_pull_table_is_exact(monkeypatch):
+    """The int16 pull table (rank differences within x-planes) addresses
+    exactly the populations the int32 table does: states bit-identical."""
+    ind = geo.sphere_pack((20, 14, 16), 0.4, 3.0, seed=17)
+    nz, ny, nx = ind.shape
+    inp = pio.LBMInput(B=0.7, sig=0.02, vir=0.1, vib=0.2, wet=0.3, ncx=nx, ncy=ny, ncz=nz, dt=1.0,
+                       bfr=[1e-5, 0, 0], bfb=[1e-5, 0, 0])
+    state = ob.equilibrium_state(inp, ob.fluid_order_mask(ind, geo.phase_blobs(ind, 0.5, 5, radius=2.5)))
+    out = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("PSX_LBM_IDX16", flag)
+        _say(f"case fp32 pull table int16={flag}")
+        with LBM(inp, ind, precision="f32") as lb:
+            lb.set_state(*state)
+            lb.step(40)
+            out[flag] = lb.get_state()
+    assert np.array_equal(out["0"][0], out["1"][0]) and np.array_equal(out["0"][1], out["1"][1])
