@@ -906,9 +906,15 @@ void cg_iteration(hipStream_t st, const Batch &B, Solver &S, SysState *sys, T *X
     launch_apply<T, false, false, true>(st, B, S.R, ka, kb, P, Qv, sys, S.d_jump, S.ppq);
     hipLaunchKernelGGL(k_sys_reduce, dim3(S.S), dim3(256), 0, st, B.d_vols, S.R, 0, S.ppq, (const double *)nullptr,
                        sys, tol2, maxit);
-    hipLaunchKernelGGL(k_cg_update<T>, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles, S.R,
-                       sys, X, Rv, Z, P, Qv, mg ? (const T *)nullptr : minv, B.Ntot, S.prz, S.prr, 0);
-    if (mg) mg_precond<T>(st, B, S, sys, Rv, Z);
+    if (mg) {  // the update is fused into the V-cycle's first kernel
+        if constexpr (std::is_same<T, float>::value) {
+            const CGUpdate upd{X, P, Qv, S.prr};
+            mg_apply(st, B, S.mg, sys, Rv, Z, S.prz, &upd);
+        }
+    } else {
+        hipLaunchKernelGGL(k_cg_update<T>, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles, S.R,
+                           sys, X, Rv, Z, P, Qv, minv, B.Ntot, S.prz, S.prr, 0);
+    }
     hipLaunchKernelGGL(k_sys_reduce, dim3(S.S), dim3(256), 0, st, B.d_vols, S.R, 1, S.prz, S.prr, sys, tol2, maxit);
     hipLaunchKernelGGL(k_cg_pupdate<T>, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles, S.R,
                        sys, P, Z, B.Ntot, 0);

@@ -52,9 +52,19 @@ struct MG {
 // Builds the hierarchy for the batch (R right-hand sides per volume).
 int mg_build(hipStream_t st, const Batch &B, int R, const MGParams &prm, MG &mg);
 void mg_free(MG &mg);
+// The PCG update that mg_apply can fold into its first kernel:
+// X += alpha P, r -= alpha Q (alpha per system), partials of r.r into prr.
+struct CGUpdate {
+    float *X;
+    const float *P, *Q;
+    double *prr;
+};
+
 // z = M^-1 r for every active system; r, z are [R][3][Ntot] FP32 on the
 // batch grid.  prz (may be null) receives the (tile, rhs) partials of r.z.
-void mg_apply(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, const float *r, float *z, double *prz);
+// With upd, r is first updated in place as described above.
+void mg_apply(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, const float *r, float *z, double *prz,
+              const CGUpdate *upd = nullptr);
 // Parameters from the environment (PSX_MG_LEVELS, PSX_MG_NU, PSX_MG_NU0, PSX_MG_CDEG,
 // PSX_MG_RATIO, PSX_MG_CRATIO, PSX_MG_POWER); defaults otherwise.
 MGParams mg_params_from_env();

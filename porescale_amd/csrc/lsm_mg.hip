@@ -343,6 +343,54 @@ __global__ __launch_bounds__(TILE) void k_cheb_first(LevelArgs L, int R, const S
     }
 }
 
+// The CG update fused with FIRST on level 0: x_cg += alpha p, r_cg -= alpha q
+// (partials of r_cg.r_cg into prr), then r = r_cg, x = 0, d = D^-1 r_cg / theta.
+__global__ __launch_bounds__(TILE) void k_cg_update_first(LevelArgs L, int R, const SysState *__restrict__ sys,
+                                                          float *__restrict__ xcg, float *__restrict__ rcg,
+                                                          const float *__restrict__ pcg, const float *__restrict__ qcg,
+                                                          double *__restrict__ prr, float *__restrict__ xv,
+                                                          float *__restrict__ rv, float *__restrict__ dv) {
+    __shared__ double sh[TILE / 64];
+    for (int t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
+        const int v = L.tvol[t];
+        const VolDesc vd = L.vols[v];
+        int x, y, z;
+        bool ok;
+        tile_node(vd, L.tstart[t], x, y, z, ok);
+        const int64_t n = node_index(vd, x, y, z);
+        float c1, c2;
+        cheb_coef(L.lam[v], L.ratio, -1, c1, c2);
+        const int r = blockIdx.y;
+        if (!sys_active(sys, v, R, r)) {
+            if (threadIdx.x == 0) prr[(int64_t)t * R + r] = 0.0;
+            continue;
+        }
+        const float al = (float)sys[v * R + r].alpha;
+        const int64_t base = (int64_t)r * 3 * L.N;
+        double drr = 0.0;
+        if (ok) {
+            float bb[3], zz[3];
+            for (int e = 0; e < 3; e++) {
+                const int64_t i = base + e * L.N + n;
+                xcg[i] += al * pcg[i];
+                float rn = rcg[i] - al * qcg[i];
+                rcg[i] = rn;
+                bb[e] = rn;
+                drr += (double)rn * (double)rn;
+            }
+            apply_minv(L.minv, L.N, n, bb, zz);
+            for (int e = 0; e < 3; e++) {
+                const int64_t i = base + e * L.N + n;
+                rv[i] = bb[e];
+                xv[i] = 0.f;
+                dv[i] = c2 * zz[e];
+            }
+        }
+        double s = block_sum(drr, sh);
+        if (threadIdx.x == 0) prr[(int64_t)t * R + r] = s;
+    }
+}
+
 // STEP k: x += d; r -= A d; (NEWD) d' = c1 d + c2 D^-1 r.  k = -1 restarts
 // the recurrence (d' = D^-1 r / theta) after a coarse correction d.
 template <bool ELEM, bool NEWD>
@@ -815,14 +863,19 @@ int mg_build(hipStream_t st, const Batch &B, int R, const MGParams &prm, MG &mg)
     return PSX_OK;
 }
 
-void mg_apply(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, const float *r, float *z, double *prz) {
+void mg_apply(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, const float *r, float *z, double *prz,
+              const CGUpdate *upd) {
     MGLevel &l0 = mg.lv[0];
     const bool only = mg.lv.size() == 1;
     LevelArgs L = args_of(l0, B, only ? mg.prm.coarse_ratio : mg.prm.ratio);
     l0.b = const_cast<float *>(r);
     l0.x = z;
-    hipLaunchKernelGGL(k_cheb_first, dim3(grid_for(L.ntiles), mg.R), dim3(TILE), 0, st, L, mg.R, sys, (const float *)r, z,
-                       l0.r, l0.dA);
+    if (upd)
+        hipLaunchKernelGGL(k_cg_update_first, dim3(grid_for(L.ntiles), mg.R), dim3(TILE), 0, st, L, mg.R, sys, upd->X,
+                           const_cast<float *>(r), upd->P, upd->Q, upd->prr, z, l0.r, l0.dA);
+    else
+        hipLaunchKernelGGL(k_cheb_first, dim3(grid_for(L.ntiles), mg.R), dim3(TILE), 0, st, L, mg.R, sys,
+                           (const float *)r, z, l0.r, l0.dA);
     float *pend = vcycle(st, B, mg, sys, 0);
     hipLaunchKernelGGL(k_final0, dim3(grid_for(L.ntiles), mg.R), dim3(TILE), 0, st, L, mg.R, sys, r, z,
                        (const float *)pend, prz);

@@ -188,3 +188,26 @@ def test_fp32_compressed_pull_table_is_exact(monkeypatch):
             lb.step(40)
             out[flag] = lb.get_state()
     assert np.array_equal(out["0"][0], out["1"][0]) and np.array_equal(out["0"][1], out["1"][1])
+
+
+def test_fp64_multipore_selects_iipore_bitexact():
+    """Two pore families (codes 1 and 2) with iipore = 2: only code-2 voxels
+    are fluid (L1:1046-1072).  The other family enters the colour stencil as
+    `wet`, the defined form of L1:2673 whose reference reads cf(0) there."""
+    ind = geo.sphere_pack((12, 14, 10), 0.55, 2.5, seed=21)
+    fam = geo.phase_blobs(ind, 0.6, seed=4, radius=2.0)
+    ind2 = np.where(fam, 2, ind).astype(np.int32)
+    nz, ny, nx = ind.shape
+    inp = pio.LBMInput(B=0.7, sig=0.02, vir=0.1, vib=0.2, wet=0.4, iipore=2, npores=2, sat=0.4, ncx=nx, ncy=ny,
+                       ncz=nz, dt=1.0, bfr=[0, 0, 1e-5], bfb=[0, 0, 1e-5])
+    _say("case multipore iipore=2")
+    o = ob.OracleLBM(inp, ind2)
+    o.init_random(inp.sat)
+    o.step(20)
+    with LBM(inp, ind2) as lb:
+        assert lb.nl == int((ind2 == 2).sum()) == o.nl
+        lb.initdistr(inp.sat)
+        lb.step(20)
+        fr, fb = lb.get_state()
+    ofr, ofb = o.state()
+    assert np.array_equal(fr, ofr) and np.array_equal(fb, ofb)
