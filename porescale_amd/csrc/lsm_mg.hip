@@ -433,6 +433,56 @@ __global__ __launch_bounds__(TILE) void k_cheb_step(LevelArgs L, int R, const Sy
     }
 }
 
+// Level-0 restart that also ends the V-cycle (smoothing degree 1 on the
+// finest level): x += e; r -= A e; z = x + D^-1 r / theta, written to x (the
+// caller's z), with the (tile, rhs) partials of r_cg.z into prz.
+__global__ __launch_bounds__(TILE) void k_restart_final0(LevelArgs L, int R, const SysState *__restrict__ sys,
+                                                         float *__restrict__ xv, float *__restrict__ rv,
+                                                         const float *__restrict__ ev, const float *__restrict__ rcg,
+                                                         double *__restrict__ prz) {
+    __shared__ double sh[TILE / 64];
+    for (int t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
+        const int v = L.tvol[t];
+        const VolDesc vd = L.vols[v];
+        int x, y, z;
+        bool ok;
+        tile_node(vd, L.tstart[t], x, y, z, ok);
+        const int64_t n = node_index(vd, x, y, z);
+        const int r = blockIdx.y;
+        if (!sys_active(sys, v, R, r)) {
+            if (prz && threadIdx.x == 0) prz[(int64_t)t * R + r] = 0.0;
+            continue;
+        }
+        float a[8], b[8];
+        const bool act = ok && gather_k(vd, L.ka, L.kb, x, y, z, a, b);
+        float c1, c2;
+        cheb_coef(L.lam[v], L.ratio, -1, c1, c2);
+        const int64_t base = (int64_t)r * 3 * L.N;
+        float q[3];
+        op_node<true>(L, vd, x, y, z, n, ev + base, act, a, b, q);
+        double dot = 0.0;
+        if (ok) {
+            float rr[3], xx[3], zz[3];
+            for (int e = 0; e < 3; e++) {
+                const int64_t i = base + e * L.N + n;
+                xx[e] = xv[i] + ev[i];
+                rr[e] = rv[i] - q[e];
+            }
+            apply_minv(L.minv, L.N, n, rr, zz);
+            for (int e = 0; e < 3; e++) {
+                const int64_t i = base + e * L.N + n;
+                const float zf = xx[e] + c2 * zz[e];
+                xv[i] = zf;
+                dot += (double)rcg[i] * (double)zf;
+            }
+        }
+        if (prz) {
+            double s = block_sum(dot, sh);
+            if (threadIdx.x == 0) prz[(int64_t)t * R + r] = s;
+        }
+    }
+}
+
 // coarse node I: b = P^T r_fine; r = b; x = 0; d = D^-1 b / theta
 __global__ __launch_bounds__(TILE) void k_restrict_first(LevelArgs F, LevelArgs C, int R,
                                                          const SysState *__restrict__ sys,
@@ -640,7 +690,8 @@ void launch_step(hipStream_t st, const LevelArgs &L, bool elem, int R, const Sys
 
 // V-cycle from level l, whose b, r, x, dA hold a started smoothing pass
 // (r = b, x = 0, dA = d_0).  Returns the pending last update of x.
-float *vcycle(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, int l) {
+float *vcycle(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, int l, const float *rcg = nullptr,
+              double *prz = nullptr, bool *finished = nullptr) {
     MGLevel &lv = mg.lv[l];
     const bool coarsest = l + 1 == (int)mg.lv.size();
     const bool elem = l == 0;
@@ -671,6 +722,12 @@ float *vcycle(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, int l
     hipLaunchKernelGGL(k_prolong, dim3(grid_for(L.ntiles), mg.R), dim3(TILE), 0, st, L, C, mg.R, sys, cv.x,
                        (const float *)pend, dA);
     // post-smoothing: restart from the corrected x, nu updates
+    if (l == 0 && nu == 1 && finished) {  // the restart is the last update: it also forms z and r.z
+        hipLaunchKernelGGL(k_restart_final0, dim3(grid_for(L.ntiles), mg.R), dim3(TILE), 0, st, L, mg.R, sys, lv.x,
+                           lv.r, (const float *)dA, rcg, prz);
+        *finished = true;
+        return nullptr;
+    }
     launch_step<true>(st, L, elem, mg.R, sys, -1, lv.x, lv.r, dA, dB);
     std::swap(dA, dB);
     for (int k = 0; k + 1 < nu; k++) {
@@ -876,9 +933,11 @@ void mg_apply(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, const
     else
         hipLaunchKernelGGL(k_cheb_first, dim3(grid_for(L.ntiles), mg.R), dim3(TILE), 0, st, L, mg.R, sys,
                            (const float *)r, z, l0.r, l0.dA);
-    float *pend = vcycle(st, B, mg, sys, 0);
-    hipLaunchKernelGGL(k_final0, dim3(grid_for(L.ntiles), mg.R), dim3(TILE), 0, st, L, mg.R, sys, r, z,
-                       (const float *)pend, prz);
+    bool finished = false;
+    float *pend = vcycle(st, B, mg, sys, 0, r, prz, &finished);
+    if (!finished)
+        hipLaunchKernelGGL(k_final0, dim3(grid_for(L.ntiles), mg.R), dim3(TILE), 0, st, L, mg.R, sys, r, z,
+                           (const float *)pend, prz);
 }
 
 }  // namespace lsmk
