@@ -111,6 +111,13 @@ int psx_lbm_get_fields(psx_lbm *h, double *rr, double *rb, double *cf);
 int psx_lbm_step(psx_lbm *h, int64_t nsteps);
 /* check_state for iteration it1 with itw steps per iteration. */
 int psx_lbm_check_state(psx_lbm *h, int64_t it1, int64_t itw, psx_lbm_diag *out);
+/* The fluid fields of save_results (L1:2067-2142) on the voxel grid, x
+   fastest (Fortran order of ind), nvox = ncx*ncy*ncz entries each:
+   u [nvox][3] velocity (what velocity_field lists), color [nvox] colour
+   field (color_field), pressure [nvox] (pressure<it1>), phase [nvox] the
+   ind_red_blue code (L1:2080-2088).  Solid voxels get u = 0, color = 1.2,
+   pressure = 0, phase = 0.  Any output may be NULL. */
+int psx_lbm_fields(psx_lbm *h, double *u, double *color, double *pressure, int32_t *phase);
 int psx_lbm_sync(psx_lbm *h);
 /* Runs nsteps and times each kernel with HIP events on the handle's stream.
    ms[0] = total wall ms (event to event), ms[1] = mean collide kernel ms,

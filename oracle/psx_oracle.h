@@ -69,6 +69,9 @@ void or_lbm_densities(or_lbm *h);
 void or_lbm_step(or_lbm *h, int64_t nsteps);
 void or_lbm_check_state(const or_lbm *h, int64_t it1, int64_t itw, or_lbm_diag *d);
 void or_lbm_get_S(const or_lbm *h, double *Sr, double *Sb);
+/* save_results fields (L1:2067-2155) on the voxel grid, i fastest: u (3 per
+   voxel), colour, pressure, phase code; any pointer may be NULL */
+void or_lbm_save_fields(const or_lbm *h, double *u, double *color, double *pressure, int32_t *phase);
 /* single-node kernels exposed for unit tests */
 void or_collision(double t1, double t2, double t3, double *f, double *pcf, double r,
                   const double *bf, const double *Fp, const double *S, double dt);

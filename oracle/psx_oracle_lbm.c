@@ -450,6 +450,45 @@ void or_lbm_get_fields(const or_lbm *h, double *rr, double *rb, double *cf) {
     if (cf) memcpy(cf, h->cf, h->nl * sizeof(double));
 }
 
+/* save_results field pass (L1:2067-2155): per voxel, i fastest, the fluid
+   velocity (L1:2107-2130: both colour branches use the red body force), the
+   colour cf (1.2 at non-fluid voxels), the pressure rr*cr2 + rb*cb2 - cr2
+   (0 at non-fluid voxels) and the phase code INDSRB (L1:2093-2101). */
+void or_lbm_save_fields(const or_lbm *h, double *u, double *color, double *pressure, int32_t *phase) {
+    const int nx = h->nx, ny = h->ny, nz = h->nz;
+    const double dt = h->p.dt, dti = 1.0 / dt, cr2 = h->p.cr2, cb2 = h->cb2;
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                const int64_t a = at3(i, j, k, nx, ny);
+                const int32_t l = h->mijk[a];
+                double uu[3] = {0.0, 0.0, 0.0}, col = 1.2, pr = 0.0;
+                int32_t ph = 0;
+                if (l > 0) {
+                    const int64_t n = l - 1;
+                    const double cf = h->cf[n];
+                    if (cf > 0.9) ph = 1;
+                    else if (cf < -0.9) ph = 2;
+                    else if (cf >= 0.0 && cf <= 0.9) ph = 6;
+                    else ph = 7;
+                    double Cg[3], Fs[3], f[19], bfi[3];
+                    calc_fs(h, &h->icor[3 * n], Cg, Fs);
+                    for (int q = 0; q < 19; q++) f[q] = h->fr[19 * n + q] + h->fb[19 * n + q];
+                    const double ri = dti / (h->rr[n] + h->rb[n]);
+                    for (int d = 0; d < 3; d++) bfi[d] = dt * (h->p.bfr[d] + Fs[d]) * 0.5;
+                    uu[0] = (f[1] - f[2] + f[7] - f[8] + f[9] - f[10] + f[11] - f[12] + f[13] - f[14] + bfi[0]) * ri;
+                    uu[1] = (f[3] - f[4] + f[7] + f[8] - f[9] - f[10] + f[15] - f[16] + f[17] - f[18] + bfi[1]) * ri;
+                    uu[2] = (f[5] - f[6] + f[11] + f[12] - f[13] - f[14] + f[15] + f[16] - f[17] - f[18] + bfi[2]) * ri;
+                    col = cf;
+                    pr = (h->rr[n] * cr2 + h->rb[n] * cb2) - cr2 * 1.0;
+                }
+                if (u) for (int d = 0; d < 3; d++) u[3 * a + d] = uu[d];
+                if (color) color[a] = col;
+                if (pressure) pressure[a] = pr;
+                if (phase) phase[a] = ph;
+            }
+}
+
 /* density / colour field pass (L1:3436-3447, 3934-3945) */
 void or_lbm_densities(or_lbm *h) {
     for (int64_t i = 0; i < h->nl; i++) {

@@ -65,7 +65,7 @@ class LSMParams(C.Structure):
 EXPORTS = (
     "psx_last_error", "psx_device_info",
     "psx_lbm_create", "psx_lbm_destroy", "psx_lbm_counts", "psx_lbm_tables", "psx_lbm_rates",
-    "psx_lbm_init_random", "psx_lbm_set_state", "psx_lbm_get_state", "psx_lbm_get_fields",
+    "psx_lbm_init_random", "psx_lbm_set_state", "psx_lbm_get_state", "psx_lbm_get_fields", "psx_lbm_fields",
     "psx_lbm_step", "psx_lbm_check_state", "psx_lbm_sync", "psx_lbm_profile", "psx_lbm_footprint",
     "psx_lsm_create", "psx_lsm_destroy", "psx_lsm_counts", "psx_lsm_set_u", "psx_lsm_get_u",
     "psx_lsm_forces", "psx_lsm_verlet", "psx_lsm_solve", "psx_lsm_stress", "psx_lsm_moduli",
@@ -90,6 +90,7 @@ def _declare(L):
     L.psx_lbm_set_state.argtypes = [_vp, _vp, _vp]
     L.psx_lbm_get_state.argtypes = [_vp, _vp, _vp]
     L.psx_lbm_get_fields.argtypes = [_vp, _vp, _vp, _vp]
+    L.psx_lbm_fields.argtypes = [_vp, _vp, _vp, _vp, _vp]
     L.psx_lbm_step.argtypes = [_vp, _i64]
     L.psx_lbm_check_state.argtypes = [_vp, _i64, _i64, C.POINTER(LBMDiag)]
     L.psx_lbm_sync.argtypes = [_vp]

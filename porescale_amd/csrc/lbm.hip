@@ -534,6 +534,57 @@ extern "C" int psx_lbm_get_fields(psx_lbm *h, double *rr, double *rb, double *cf
     PSX_GUARD_END
 }
 
+extern "C" int psx_lbm_fields(psx_lbm *h, double *u, double *color, double *pressure, int32_t *phase) {
+    PSX_CHECK_ARG(h, "psx_lbm_fields: NULL handle");
+    PSX_GUARD_BEGIN
+    const int64_t nl = h->nl, nv = (int64_t)h->nx * h->ny * h->nz;
+    // solid voxels (L1:2133-2136); fluid voxels are overwritten below
+    for (int64_t a = 0; a < nv; a++) {
+        if (u) u[3 * a] = u[3 * a + 1] = u[3 * a + 2] = 0.0;
+        if (color) color[a] = 1.2;
+        if (pressure) pressure[a] = 0.0;
+        if (phase) phase[a] = 0;
+    }
+    if (nl == 0) return PSX_OK;
+    std::vector<double> out((size_t)5 * nl);
+    std::vector<int32_t> ph((size_t)nl);
+    double *d_out = nullptr;
+    int32_t *d_ph = nullptr;
+    hipError_t e = hipMalloc((void **)&d_out, sizeof(double) * out.size());
+    if (e == hipSuccess) e = hipMalloc((void **)&d_ph, sizeof(int32_t) * ph.size());
+    if (e == hipSuccess) {
+        const unsigned nb = (unsigned)psx::ceil_div(nl, 256);
+        if (h->f32) {
+            e = psx_lbm_fast::fields(h->c, (const float *)h->d_pop[h->cur], h->d_dst, h->d_vox,
+                                     (const double *)h->d_phi, d_out, d_ph, h->stream);
+        } else {
+            StoreF64 st{(double *)h->d_pop[h->cur], nl};
+            hipLaunchKernelGGL(k_fields<StoreF64>, dim3(nb), dim3(256), 0, h->stream, h->c, st, h->d_vox,
+                               (const double *)h->d_phi, d_out, d_ph);
+            e = hipGetLastError();
+        }
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out.data(), d_out, sizeof(double) * out.size(), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(ph.data(), d_ph, sizeof(int32_t) * ph.size(), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (d_out) (void)hipFree(d_out);
+    if (d_ph) (void)hipFree(d_ph);
+    PSX_HIP(e);
+    for (int64_t i = 0; i < nl; i++) {
+        const int32_t *ic = &h->icor[3 * i];
+        const int64_t a = (int64_t)(ic[0] - 1) + (int64_t)h->nx * ((ic[1] - 1) + (int64_t)h->ny * (ic[2] - 1));
+        if (u)
+            for (int d = 0; d < 3; d++) u[3 * a + d] = out[(size_t)d * nl + i];
+        if (color) color[a] = out[(size_t)3 * nl + i];
+        if (pressure) pressure[a] = out[(size_t)4 * nl + i];
+        if (phase) phase[a] = ph[i];
+    }
+    return PSX_OK;
+    PSX_GUARD_END
+}
+
 extern "C" int psx_lbm_init_random(psx_lbm *h, double sat) {
     PSX_CHECK_ARG(h, "psx_lbm_init_random: NULL handle");
     PSX_GUARD_BEGIN

@@ -423,6 +423,46 @@ __global__ __launch_bounds__(256) void k_density(const LbmConst c, const St in,
     phi[vox[i]] = (typename St::phi_t)((rr - rb) / (rr + rb));
 }
 
+// Per-node output of save_results (L1:2074-2142): out[0..2] velocity, out[3]
+// colour, out[4] pressure, each [nl]; phase = the ind_red_blue code
+// (1 red, 2 blue, 6 red side of the interface, 7 blue side).  Both colour
+// branches of the reference use the red body force bfr.
+template <class St>
+__global__ __launch_bounds__(256) void k_fields(const LbmConst c, const St in, const int32_t *__restrict__ vox,
+                                                const typename St::phi_t *__restrict__ phi,
+                                                double *__restrict__ out, int32_t *__restrict__ phase) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nl = c.nl;
+    if (i >= nl) return;
+    double f[Q], rr, rb;
+    {
+        double a, b;
+        load_pop(in, c, 0, i, rr, rb);
+        f[0] = rr + rb;
+#pragma unroll
+        for (int q = 1; q < Q; q++) {
+            load_pop(in, c, q, i, a, b);
+            f[q] = a + b;
+            rr = rr + a;
+            rb = rb + b;
+        }
+    }
+    const double cf = (rr - rb) / (rr + rb);
+    int x, y, z;
+    vox_xyz(c, vox[i], x, y, z);
+    double Cg[3], Fs[3], bfi[3];
+    calc_fs(c, phi, x, y, z, Cg, Fs);
+    const double ri = c.dti / (rr + rb);
+#pragma unroll
+    for (int d = 0; d < 3; d++) bfi[d] = c.dt * (c.bfr[d] + Fs[d]) * 0.5;
+    out[i] = (f[1] - f[2] + f[7] - f[8] + f[9] - f[10] + f[11] - f[12] + f[13] - f[14] + bfi[0]) * ri;
+    out[nl + i] = (f[3] - f[4] + f[7] + f[8] - f[9] - f[10] + f[15] - f[16] + f[17] - f[18] + bfi[1]) * ri;
+    out[2 * nl + i] = (f[5] - f[6] + f[11] + f[12] - f[13] - f[14] + f[15] + f[16] - f[17] - f[18] + bfi[2]) * ri;
+    out[3 * nl + i] = cf;
+    out[4 * nl + i] = (rr * c.cr2 + rb * c.cb2) - c.cr2 * 1.0;
+    phase[i] = cf > 0.9 ? 1 : cf < -0.9 ? 2 : (cf >= 0.0 && cf <= 0.9) ? 6 : 7;
+}
+
 // check_state partial sums; see NDIAG layout in diag_finish().
 constexpr int NDIAG = 36;
 constexpr int BLOCK = 256;

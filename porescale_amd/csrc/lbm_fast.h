@@ -18,4 +18,8 @@ hipError_t density(const psx_lbm_dev::LbmConst &c, const float *in, const int32_
 hipError_t diag(const psx_lbm_dev::LbmConst &c, const float *in, const int32_t *src, const int32_t *vox,
                 const double *phi, double *partial, hipStream_t st);
 
+// save_results fields (k_fields) of the pulled state: out [5][nl], phase [nl].
+hipError_t fields(const psx_lbm_dev::LbmConst &c, const float *in, const int32_t *src, const int32_t *vox,
+                  const double *phi, double *out, int32_t *phase, hipStream_t st);
+
 }  // namespace psx_lbm_fast

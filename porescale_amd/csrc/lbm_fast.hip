@@ -349,4 +349,13 @@ hipError_t diag(const LbmConst &c, const float *in, const int32_t *src, const in
     return hipGetLastError();
 }
 
+hipError_t fields(const LbmConst &c, const float *in, const int32_t *src, const int32_t *vox, const double *phi,
+                  double *out, int32_t *phase, hipStream_t st) {
+    int64_t nb = psx::ceil_div(c.nl, BLOCK);
+    if (nb == 0) return hipSuccess;
+    PullF32 ld{in, src, c.nl};
+    hipLaunchKernelGGL(k_fields<PullF32>, dim3((unsigned)nb), dim3(BLOCK), 0, st, c, ld, vox, phi, out, phase);
+    return hipGetLastError();
+}
+
 }  // namespace psx_lbm_fast

@@ -127,6 +127,16 @@ module porescale
        real(c_double) :: rr(*), rb(*), cf(*)
      end function psx_lbm_get_fields
 
+     ! save_results fields on the voxel grid (x fastest): u(3, nvox),
+     ! color(nvox), pressure(nvox), phase(nvox)
+     function psx_lbm_fields(h, u, color, pressure, phase) bind(C, name='psx_lbm_fields')
+       import :: c_int, c_int32_t, c_double, c_ptr
+       integer(c_int) :: psx_lbm_fields
+       type(c_ptr), value :: h
+       real(c_double) :: u(*), color(*), pressure(*)
+       integer(c_int32_t) :: phase(*)
+     end function psx_lbm_fields
+
      function psx_lbm_step(h, nsteps) bind(C, name='psx_lbm_step')
        import :: c_int, c_int64_t, c_ptr
        integer(c_int) :: psx_lbm_step

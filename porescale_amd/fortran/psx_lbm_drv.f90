@@ -7,7 +7,10 @@
 ! icon = 1 restarts from iteration2 (:2845-2850).  Runs the 2 pre-steps and the
 ! itm x itw majority-rule steps (:3454-3947), rewrites iteration2 after every
 ! itw steps (:3951-3961) and appends one output.m row per check_state
-! (:3274-3282).  settings_19 receives the node and table counts.
+! (:3274-3282).  Every its-th iteration it writes the fluid part of
+! save_results (:2067-2155): velocity_field, color_field, pressure<it1> and
+! a block appended to ind_red_blue.  settings_19 receives the node and table
+! counts.
 !
 ! Optional lines after line 30 (ignored by the reference, which stops reading
 ! at line 30) select the GPU options:
@@ -26,7 +29,7 @@ program psx_lbm_drv
   real(8) :: cr2, rr0, rb0, B, sig, vir, vib, wet, eps, sat, ee, lsize
   real(8) :: bfr(3), bfb(3), Em, nu, rs, dt, dl, lam
   integer :: idir, itm, itw, its, mtype, icon, iipore, npores, ncx, ncy, ncz
-  integer :: prec, coll, ios, u, it1, itms, j, i
+  integer :: prec, coll, ios, u, it1, itms, j, i, is
   integer(c_int32_t), allocatable :: ind(:, :, :)
   real(8), allocatable :: fr(:, :), fb(:, :)
   integer(c_int64_t) :: counts(5), nl
@@ -169,7 +172,9 @@ program psx_lbm_drv
 
   ! two pre-steps (lsm_lbm_static_alpha_beta_multipore.f:3454-3621)
   call psx_check(psx_lbm_step(h, 2_c_int64_t), 'psx_lbm_step')
+  is = 0
   do it1 = itms, itm
+     is = is + 1
      call psx_check(psx_lbm_step(h, int(itw, c_int64_t)), 'psx_lbm_step')
      call psx_check(psx_lbm_get_state(h, fr, fb), 'psx_lbm_get_state')
      open(newunit=u, file='iteration2', status='replace')
@@ -177,6 +182,10 @@ program psx_lbm_drv
      write(u, *) ((fr(j, i), j = 1, 19), i = 1, nl)
      write(u, *) ((fb(j, i), j = 1, 19), i = 1, nl)
      close(u)
+     if (is == its) then
+        is = 0
+        call save_fields(it1)
+     end if
      call psx_check(psx_lbm_check_state(h, int(it1, c_int64_t), int(itw, c_int64_t), d), &
           'psx_lbm_check_state')
      if (d%isneg /= 0) write(*, *) 'ATTENTION!!!, there are negative populations'
@@ -191,6 +200,49 @@ program psx_lbm_drv
   call psx_lbm_destroy(h)
 
 contains
+
+  ! The fluid files of save_results (lsm_lbm_static_alpha_beta_multipore.f:
+  ! 2067-2155) in their list-directed layout; the pressure file is named
+  ! pressure<it1> and ind_red_blue receives one more block.
+  subroutine save_fields(it)
+    integer, intent(in) :: it
+    real(c_double), allocatable :: uv(:, :), col(:), pre(:)
+    integer(c_int32_t), allocatable :: ph(:, :, :)
+    integer :: uo, a, nv, jj, kk
+    character(len=32) :: fname, rowfmt
+    nv = ncx * ncy * ncz
+    allocate(uv(3, nv), col(nv), pre(nv), ph(ncx, ncy, ncz))
+    call psx_check(psx_lbm_fields(h, uv, col, pre, ph), 'psx_lbm_fields')
+    open(newunit=uo, file='velocity_field', status='replace')
+    do a = 1, nv
+       write(uo, *) uv(:, a)
+    end do
+    close(uo)
+    open(newunit=uo, file='color_field', status='replace')
+    do a = 1, nv
+       write(uo, *) col(a)
+    end do
+    close(uo)
+    write(fname, '(A, I0)') 'pressure', it
+    open(newunit=uo, file=trim(fname), status='replace')
+    do a = 1, nv
+       write(uo, *) pre(a)
+    end do
+    close(uo)
+    write(rowfmt, '(A, I0, A)') '(', ncx, 'I2)'
+    open(newunit=uo, file='ind_red_blue', position='append')
+    write(uo, 4579) it * itw
+    write(uo, *) 'a=['
+    do kk = 1, ncz
+       do jj = 1, ncy
+          write(uo, rowfmt) ph(:, jj, kk)
+       end do
+    end do
+    write(uo, *) '];'
+    write(uo, *) 'clm'
+    close(uo)
+4579 format(///, 5X, 'ITER=', I7, ';')
+  end subroutine save_fields
 
   subroutine mrt(visc, dt_, S)
     real(8), intent(in) :: visc, dt_

@@ -271,6 +271,54 @@ def read_output_m(path: str) -> np.ndarray:
     return np.array(rows)
 
 
+def read_ind_red_blue(path: str, ncx: int, ncy: int, ncz: int):
+    """Blocks of the phase file save_results appends to
+    (lsm_lbm_static_alpha_beta_multipore.f:2143-2155): a list of
+    (iteration, codes (ncz, ncy, ncx) int32), in file order."""
+    blocks = []
+    with open(path) as fh:
+        lines = fh.read().splitlines()
+    k = 0
+    while k < len(lines):
+        s = lines[k].strip()
+        if s.startswith("ITER="):
+            it = int(s[5:].rstrip(";"))
+            if lines[k + 1].strip() != "a=[":
+                raise ValueError(f"{path}: line {k + 2}: expected 'a=['")
+            rows = lines[k + 2:k + 2 + ncy * ncz]
+            if len(rows) != ncy * ncz:
+                raise ValueError(f"{path}: block ITER={it} is truncated")
+            a = np.array([[int(r[2 * i:2 * i + 2]) for i in range(ncx)] for r in rows], dtype=np.int32)
+            blocks.append((it, a.reshape(ncz, ncy, ncx)))
+            k += 2 + ncy * ncz
+        else:
+            k += 1
+    return blocks
+
+
+def write_save_fields(directory: str, it1: int, itw: int, fields: dict) -> None:
+    """The fluid files of save_results (lsm_lbm_static_alpha_beta_multipore.f:
+    2067-2155) from the arrays ``LBM.save_fields`` returns: velocity_field
+    (3 values a line), color_field, pressure<it1> (one value a line, voxels x
+    fastest) and one block appended to ind_red_blue."""
+    u = np.asarray(fields["velocity"]).reshape(-1, 3)
+    with open(os.path.join(directory, "velocity_field"), "w") as fh:
+        for row in u:
+            fh.write(" " + " ".join(repr(float(x)) for x in row) + "\n")
+    for name, key in (("color_field", "color"), (f"pressure{int(it1)}", "pressure")):
+        with open(os.path.join(directory, name), "w") as fh:
+            for x in np.asarray(fields[key]).reshape(-1):
+                fh.write(f" {float(x)!r}\n")
+    ph = np.asarray(fields["phase"])
+    ncz, ncy, ncx = ph.shape
+    with open(os.path.join(directory, "ind_red_blue"), "a") as fh:
+        fh.write(f"\n\n\n     ITER={int(it1) * int(itw):7d};\n a=[\n")
+        for kk in range(ncz):
+            for jj in range(ncy):
+                fh.write("".join(f"{int(x):2d}" for x in ph[kk, jj]) + "\n")
+        fh.write(" ];\n clm\n")
+
+
 def read_velocities(path: str, n: int) -> np.ndarray:
     with open(path) as fh:
         rows = [[int(line[3 * c:3 * c + 3]) for c in range(n)] for line in fh.readlines()[:3]]

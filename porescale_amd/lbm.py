@@ -15,7 +15,7 @@ from typing import Optional, Sequence
 import numpy as np
 
 from . import _lib
-from .io import LBMInput
+from .io import LBMInput, write_save_fields
 
 __all__ = ["LBM", "trt_rates", "mrt_rates"]
 
@@ -130,6 +130,17 @@ class LBM:
         _lib.check(self._L.psx_lbm_get_fields(self._h, _lib.ptr(rr), _lib.ptr(rb), _lib.ptr(cf)))
         return rr, rb, cf
 
+    def save_fields(self):
+        """The fluid fields save_results writes (L1:2067-2142), as arrays
+        (ncz, ncy, ncx) of the voxel grid: ``velocity`` (.., 3), ``color``,
+        ``pressure`` (float64) and ``phase`` (int32 ind_red_blue codes)."""
+        nz, ny, nx = self.shape
+        u = np.zeros((nz, ny, nx, 3))
+        col, pre = np.zeros((nz, ny, nx)), np.zeros((nz, ny, nx))
+        ph = np.zeros((nz, ny, nx), np.int32)
+        _lib.check(self._L.psx_lbm_fields(self._h, _lib.ptr(u), _lib.ptr(col), _lib.ptr(pre), _lib.ptr(ph)))
+        return {"velocity": u, "color": col, "pressure": pre, "phase": ph}
+
     def step(self, nsteps: int = 1):
         """``nsteps`` iterations of the majorityrule time loop body."""
         _lib.check(self._L.psx_lbm_step(self._h, int(nsteps)))
@@ -148,14 +159,21 @@ class LBM:
         _lib.check(self._L.psx_lbm_profile(self._h, int(nsteps), _lib.ptr(ms)))
         return tuple(ms)
 
-    def run(self, itm: int, itw: int, prestep: bool = True, icon: int = 0):
+    def run(self, itm: int, itw: int, prestep: bool = True, icon: int = 0, save_dir=None, its: int = 1):
         """The majorityrule schedule: 2 pre-steps, then itm*itw steps with a
-        check_state after every itw; returns the output.m rows."""
+        check_state after every itw; returns the output.m rows.  With
+        ``save_dir``, every its-th iteration writes the save_results files
+        there (L1:3633, 3963-3967)."""
         if prestep:
             self.step(2)
         rows = []
         it0 = 1
+        isv = 0
         for it1 in range(it0, itm + 1):
+            isv += 1
             self.step(itw)
+            if save_dir is not None and isv == its:
+                isv = 0
+                write_save_fields(save_dir, it1, itw, self.save_fields())
             rows.append(list(self.check_state(it1, itw).outm))
         return np.array(rows)

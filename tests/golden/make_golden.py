@@ -10,6 +10,9 @@ Fixtures (compressed .npz, inputs and reference outputs only):
   lbm_blobs.npz   L1, blob start injected through iteration2 (icon=1),
                   18x16x14 pack, wetting 0.7071, anisotropic body forces,
                   unequal viscosities, 2 + 3*20 steps, output.m rows.
+  lbm_blobs_fields.npz  the save_results files of the lbm_blobs run's last
+                  iteration: velocity_field, color_field, pressure<it1> and
+                  the last ind_red_blue block.
   lsm_xx.npz      XX_P.f, two-component 10x12x14 pack, 100 Verlet steps,
                   nthread=3: r, v, a and stress_tensor moduli.
   lsm_yz.npz      YZ_P.f, same geometry, nthread=1.
@@ -60,8 +63,10 @@ def main():
     fr, fb, outm, it1 = ob.run_ref_lbm(inp0, ind0)
     np.savez_compressed(os.path.join(HERE, "lbm_icon0.npz"), fr=fr, fb=fb, outm=outm, steps=7)
     fr0, fb0 = ob.equilibrium_state(inp1, ob.fluid_order_mask(ind1, blue))
-    fr, fb, outm, it1 = ob.run_ref_lbm(inp1, ind1, restart=(fr0, fb0))
+    fr, fb, outm, it1, fl = ob.run_ref_lbm(inp1, ind1, restart=(fr0, fb0), fields=True)
     np.savez_compressed(os.path.join(HERE, "lbm_blobs.npz"), fr=fr, fb=fb, outm=outm, steps=62)
+    np.savez_compressed(os.path.join(HERE, "lbm_blobs_fields.npz"), it1=it1, iter=fl["iter"],
+                        velocity=fl["velocity"], color=fl["color"], pressure=fl["pressure"], phase=fl["phase"])
     ind = lsm_geometry()
     nz, ny, nx = ind.shape
     for case, nth in (("xx", 3), ("yz", 1)):

@@ -71,6 +71,7 @@ def lib():
         L.or_lbm_step.argtypes = [vp, C.c_int64]
         L.or_lbm_check_state.argtypes = [vp, C.c_int64, C.c_int64, C.POINTER(LBMDiag)]
         L.or_lbm_get_S.argtypes = [vp, _P(np.float64), _P(np.float64)]
+        L.or_lbm_save_fields.argtypes = [vp, vp, vp, vp, vp]
         L.or_collision.argtypes = [C.c_double] * 3 + [_P(np.float64), _P(np.float64), C.c_double,
                                                      _P(np.float64), _P(np.float64), _P(np.float64),
                                                      C.c_double]
@@ -159,6 +160,15 @@ class OracleLBM:
         rr, rb, cf = (np.zeros(self.nl) for _ in range(3))
         self.L.or_lbm_get_fields(self.h, _ptr(rr), _ptr(rb), _ptr(cf))
         return rr, rb, cf
+
+    def save_fields(self):
+        """save_results fields on the voxel grid (nz, ny, nx[, 3])."""
+        nz, ny, nx = self._ind.shape
+        u = np.zeros((nz, ny, nx, 3))
+        col, pre = np.zeros((nz, ny, nx)), np.zeros((nz, ny, nx))
+        ph = np.zeros((nz, ny, nx), np.int32)
+        self.L.or_lbm_save_fields(self.h, _ptr(u), _ptr(col), _ptr(pre), _ptr(ph))
+        return {"velocity": u, "color": col, "pressure": pre, "phase": ph}
 
     def step(self, n=1):
         self.L.or_lbm_step(self.h, int(n))
@@ -290,8 +300,10 @@ def _run(exe: str, d: str, timeout: float) -> None:
                    stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
 
 
-def run_ref_lbm(inp: pio.LBMInput, ind: np.ndarray, restart=None, timeout=600.0):
-    """Run the flang-compiled L1 program; returns (fr, fb, output.m rows, it1).
+def run_ref_lbm(inp: pio.LBMInput, ind: np.ndarray, restart=None, timeout=600.0, fields=False):
+    """Run the flang-compiled L1 program; returns (fr, fb, output.m rows, it1)
+    and, with ``fields``, the last save_results files as a dict (velocity
+    (nv, 3), color (nv,), pressure (nv,), voxels i fastest).
 
     ``restart`` = (fr, fb) injects an initial state through iteration2 (icon=1);
     the program then runs 2 pre-steps plus (itm - 1) * itw steps.
@@ -306,9 +318,26 @@ def run_ref_lbm(inp: pio.LBMInput, ind: np.ndarray, restart=None, timeout=600.0)
         _run("l1_ref", d, timeout)
         it1, fr, fb = pio.read_lbm_restart(os.path.join(d, "iteration2"), nl)
         outm = pio.read_output_m(os.path.join(d, "output.m"))
-        return fr, fb, outm, it1
+        if not fields:
+            return fr, fb, outm, it1
+        return fr, fb, outm, it1, read_field_files(d, it1, ind.shape)
     finally:
         shutil.rmtree(d, ignore_errors=True)
+
+
+def read_field_files(d: str, it1: int, shape) -> dict:
+    """velocity_field, color_field, pressure<it1> and the last ind_red_blue
+    block as written by save_results; ``shape`` = (ncz, ncy, ncx)."""
+    with open(os.path.join(d, "velocity_field")) as fh:
+        vel = pio.parse_fortran_numbers(fh.read()).reshape(-1, 3)
+    with open(os.path.join(d, "color_field")) as fh:
+        col = pio.parse_fortran_numbers(fh.read())
+    with open(os.path.join(d, f"pressure{it1}")) as fh:
+        pre = pio.parse_fortran_numbers(fh.read())
+    ncz, ncy, ncx = shape
+    it, ph = pio.read_ind_red_blue(os.path.join(d, "ind_red_blue"), ncx, ncy, ncz)[-1]
+    return {"velocity": vel.reshape(ncz, ncy, ncx, 3), "color": col.reshape(shape),
+            "pressure": pre.reshape(shape), "phase": ph, "iter": it}
 
 
 def run_ref_lsm(case: str, ind: np.ndarray, inp: pio.LSMInput, restart=None, timeout=600.0):

@@ -59,10 +59,18 @@ def test_lbm_driver_restart_matches_reference_golden():
         it1, fr, fb = pio.read_lbm_restart(os.path.join(d, "iteration2"), fr0.shape[0])
         rows = pio.read_output_m(os.path.join(d, "output.m"))
         settings = open(os.path.join(d, "settings_19")).read()
+        fl = ob.read_field_files(d, it1, ind.shape)
+        nz, ny, nx = ind.shape
+        blocks = pio.read_ind_red_blue(os.path.join(d, "ind_red_blue"), nx, ny, nz)
     finally:
         shutil.rmtree(d, ignore_errors=True)
     assert it1 == inp.itm
     assert np.array_equal(fr, g["fr"]) and np.array_equal(fb, g["fb"])
+    # save_results every its = 1 iterations: the last files are the reference's
+    gf = np.load(os.path.join(GOLD, "lbm_blobs_fields.npz"))
+    for k in ("velocity", "color", "pressure", "phase"):
+        assert np.array_equal(fl[k], gf[k]), k
+    assert [b[0] for b in blocks] == [inp.itw * (i + 1) for i in range(inp.itm)]
     assert rows.shape == g["outm"].shape
     np.testing.assert_allclose(rows, g["outm"], rtol=1e-7, atol=1e-15, equal_nan=True)
     assert f"{fr0.shape[0]}" in settings

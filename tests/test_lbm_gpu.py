@@ -211,3 +211,65 @@ def test_fp64_multipore_selects_iipore_bitexact():
         fr, fb = lb.get_state()
     ofr, ofb = o.state()
     assert np.array_equal(fr, ofr) and np.array_equal(fb, ofb)
+
+
+def test_fields_fp64_bitexact_vs_reference_files():
+    """psx_lbm_fields from the blob run's final state against the reference's
+    own save_results files (velocity_field, color_field, pressure3,
+    ind_red_blue)."""
+    _, (ind, inp, blue) = lbm_cases()
+    g = np.load(os.path.join(GOLD, "lbm_blobs.npz"))
+    f = np.load(os.path.join(GOLD, "lbm_blobs_fields.npz"))
+    _say("case save_results fields 18x16x14 fp64")
+    with LBM(inp, ind) as lb:
+        lb.set_state(g["fr"], g["fb"])
+        got = lb.save_fields()
+    for k in ("velocity", "color", "pressure", "phase"):
+        assert np.array_equal(got[k], f[k]), k
+
+
+def test_run_writes_save_results_files(tmp_path):
+    """LBM.run(save_dir=...) writes the reference's files each its-th
+    iteration; the last ones equal the reference's."""
+    _, (ind, inp, blue) = lbm_cases()
+    f = np.load(os.path.join(GOLD, "lbm_blobs_fields.npz"))
+    _say("case run() with save_results 18x16x14 fp64")
+    with LBM(inp, ind) as lb:
+        lb.set_state(*ob.equilibrium_state(inp, ob.fluid_order_mask(ind, blue)))
+        lb.run(inp.itm, inp.itw, save_dir=str(tmp_path), its=1)
+    back = ob.read_field_files(str(tmp_path), inp.itm, ind.shape)
+    for k in ("velocity", "color", "pressure", "phase"):
+        assert np.array_equal(back[k], f[k]), k
+    assert sorted(p.name for p in tmp_path.glob("pressure*")) == [f"pressure{i}" for i in range(1, inp.itm + 1)]
+
+
+def test_fields_after_gpu_steps_match_oracle():
+    """Fields after 25 GPU steps: FP64 bit-identical to the oracle's
+    save_results; FP32 storage from the same start within storage precision."""
+    ind = geo.sphere_pack((20, 18, 16), 0.45, 3.0, seed=31)
+    nz, ny, nx = ind.shape
+    inp = pio.LBMInput(B=0.7, sig=0.02, vir=0.1, vib=0.2, wet=-0.4, idir=1, ncx=nx, ncy=ny, ncz=nz, dt=1.0,
+                       bfr=[2e-5, 0, 1e-6], bfb=[1e-5, 0, 0])
+    state = ob.equilibrium_state(inp, ob.fluid_order_mask(ind, geo.phase_blobs(ind, 0.45, 8, radius=2.5)))
+    o = ob.OracleLBM(inp, ind)
+    o.set_state(*state)
+    o.step(25)
+    want = o.save_fields()
+    _say("case fields after 25 steps 20x18x16 fp64")
+    with LBM(inp, ind) as lb:
+        lb.set_state(*state)
+        lb.step(25)
+        got = lb.save_fields()
+    for k in ("velocity", "color", "pressure", "phase"):
+        assert np.array_equal(got[k], want[k]), k
+    _say("case fields from the oracle state 20x18x16 fp32")
+    with LBM(inp, ind, precision="f32") as lb:
+        lb.set_state(*o.state())
+        g32 = lb.save_fields()
+    # FP32 storage rounds each population to ~6e-8 relative (|f| < 0.4)
+    np.testing.assert_allclose(g32["velocity"], want["velocity"], rtol=0, atol=1e-7)
+    np.testing.assert_allclose(g32["color"], want["color"], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(g32["pressure"], want["pressure"], rtol=0, atol=1e-7)
+    cf = want["color"]
+    near = (np.abs(cf - 0.9) < 1e-5) | (np.abs(cf + 0.9) < 1e-5) | (np.abs(cf) < 1e-5)
+    assert np.array_equal(g32["phase"][~near], want["phase"][~near])

@@ -121,12 +121,15 @@ def test_lbm_fresh_case_vs_reference_binary():
     nz, ny, nx = ind.shape
     inp = pio.LBMInput(B=0.5, sig=0.05, vir=0.05, vib=0.2, wet=-0.3, idir=1, itm=2, itw=7, sat=0.3,
                        ncx=nx, ncy=ny, ncz=nz, dt=1.0, bfr=[2e-5, 0, 0], bfb=[3e-5, 1e-6, 0])
-    fr, fb, outm, _ = ob.run_ref_lbm(inp, ind)
+    fr, fb, outm, _, fl = ob.run_ref_lbm(inp, ind, fields=True)
     o = ob.OracleLBM(inp, ind)
     o.init_random(inp.sat)
     o.step(2 + 2 * 7)
     ofr, ofb = o.state()
     assert np.array_equal(ofr, fr) and np.array_equal(ofb, fb)
+    got = o.save_fields()
+    for k in ("velocity", "color", "pressure", "phase"):
+        assert np.array_equal(got[k], fl[k]), k
     icor, ips, ipb2, ipb4 = o.tables()
     assert len(ips) + len(ipb2) + len(ipb4) == 18 * o.nl
 
@@ -145,3 +148,45 @@ def test_lsm_fresh_case_vs_reference_binary():
     o.verlet(60)
     orr, ov, oa = o.state()
     assert np.array_equal(orr, r) and np.array_equal(ov, v) and np.array_equal(oa, a)
+
+
+def test_lbm_save_fields_bitexact_vs_reference_files():
+    """save_results of the reference (velocity_field, color_field,
+    pressure<it1>, ind_red_blue) after the blob run, from its final state."""
+    _, (ind, inp, blue) = lbm_cases()
+    g, f = _gold("lbm_blobs.npz"), _gold("lbm_blobs_fields.npz")
+    o = ob.OracleLBM(inp, ind)
+    o.set_state(g["fr"], g["fb"])
+    got = o.save_fields()
+    for k in ("velocity", "color", "pressure", "phase"):
+        assert np.array_equal(got[k], f[k]), k
+    assert int(f["iter"]) == inp.itm * inp.itw
+    solid = ind != inp.iipore
+    assert (got["color"][solid] == 1.2).all() and (got["velocity"][solid] == 0).all()
+    assert (got["pressure"][solid] == 0).all() and (got["phase"][solid] == 0).all()
+    # phase codes follow the colour thresholds; pressure is cr2*(rho - 1) for
+    # equal sound speeds (cr2 = cb2 = 1/3 in this case)
+    cf = got["color"][~solid]
+    ph = got["phase"][~solid]
+    expect = np.where(cf > 0.9, 1, np.where(cf < -0.9, 2, np.where(cf >= 0, 6, 7)))
+    assert np.array_equal(ph, expect)
+    rr, rb, _ = o.fields()
+    icor = o.tables()[0] - 1
+    p = got["pressure"][icor[:, 2], icor[:, 1], icor[:, 0]]
+    np.testing.assert_allclose(p, inp.cr2 * (rr + rb - 1.0), rtol=0, atol=1e-15)
+
+
+def test_save_fields_files_round_trip(tmp_path):
+    """The Python writer's files parse back to the same arrays, and
+    ind_red_blue keeps one block per save."""
+    _, (ind, inp, blue) = lbm_cases()
+    f = _gold("lbm_blobs_fields.npz")
+    fl = {k: f[k] for k in ("velocity", "color", "pressure", "phase")}
+    pio.write_save_fields(str(tmp_path), 2, inp.itw, fl)
+    pio.write_save_fields(str(tmp_path), 3, inp.itw, fl)
+    back = ob.read_field_files(str(tmp_path), 3, ind.shape)
+    for k in ("velocity", "color", "pressure", "phase"):
+        assert np.array_equal(back[k], fl[k]), k
+    nz, ny, nx = ind.shape
+    blocks = pio.read_ind_red_blue(str(tmp_path / "ind_red_blue"), nx, ny, nz)
+    assert [b[0] for b in blocks] == [2 * inp.itw, 3 * inp.itw]
