@@ -19,6 +19,10 @@
  *   - LSM tables (counts), Verlet trajectories and stress moduli: against the
  *     flang-compiled XX/YZ binaries, plus the closed form C11=3a+8b,
  *     C12=a-4b, C44=a+6b on an all-solid block.
+ *   - save_results fields, the coupled solid (fluid force, lsm_step) and
+ *     calc_ab: bit-exact against the files the flang-compiled L1 writes
+ *     (velocity_field, color_field, pressure*, ind_red_blue, LSM_disf*,
+ *     LSM_velocity_field, LSM_force_field, LSM_fluid_force, alpha_beta).
  */
 #ifndef PSX_ORACLE_H
 #define PSX_ORACLE_H
@@ -72,6 +76,9 @@ void or_lbm_get_S(const or_lbm *h, double *Sr, double *Sb);
 /* save_results fields (L1:2067-2155) on the voxel grid, i fastest: u (3 per
    voxel), colour, pressure, phase code; any pointer may be NULL */
 void or_lbm_save_fields(const or_lbm *h, double *u, double *color, double *pressure, int32_t *phase);
+/* LSM-LBM coupling force (L1:3849-3879) of the last step's post-collision
+   populations: af (3,ns) in the L1 solid numbering (ns = or_lbm_count 4) */
+void or_lbm_fluid_force(const or_lbm *h, double rsi, double *af);
 /* single-node kernels exposed for unit tests */
 void or_collision(double t1, double t2, double t3, double *f, double *pcf, double r,
                   const double *bf, const double *Fp, const double *S, double dt);
@@ -80,7 +87,9 @@ void or_collmatr_trt(double sv, double *S);
 /* ------------------------------------------------------------------ LSM -- */
 typedef struct {
     int32_t nx, ny, nz;      /* voxel dims of the ind file                      */
-    int32_t pad_axis;        /* 0 = x (XX_P.f), 1 = y (YZ_P.f), 2 = z            */
+    int32_t pad_axis;        /* 0 = x (XX_P.f), 1 = y (YZ_P.f), 2 = z; -1 = the
+                                coupled model's fully periodic lattice (L1:501-
+                                1017; pass ind with pores as 1, solid as 0)    */
     int32_t jump_comp;       /* displacement component of the imposed jump       */
     int32_t nthread;         /* force-loop layer schedule (XX:378)              */
     double alf1, bet1, alf2, bet2;
@@ -107,6 +116,14 @@ void or_lsm_verlet(or_lsm *h, int64_t nsteps);
    Exxxx,Eyyxx,Eyyzz are moduli(1,1),(2,2),(3,3) and YZ's Eyzyz is moduli(2,3). */
 int64_t or_lsm_stress(const or_lsm *h, double *asigmap, double *astrain, double *moduli);
 void or_lsm_anf(double *anf /* 9x48 */, int32_t *nta /* 2x48 */);
+/* coupled model (pad_axis = -1): lsm_step (L1:1721-1903) with fluid force
+   af (3,ns) and damping damp (0.25 in L1) */
+void or_lsm_coupled_step(or_lsm *h, const double *af, double damp);
+/* calc_ab (L1:4008-4468): out[0..8] asigmap, [9..17] asigmam, [18..26]
+   astrain (column-major sums over solid cubes), [27] epsi, [28..36] alphai,
+   [37] ncub, [38..37+npores] betai; ind_raw = the codes of the ind file */
+void or_lsm_calc_ab(const or_lsm *h, const int32_t *ind_raw, int32_t iipore, int32_t npores, double pres,
+                    double *out);
 
 #ifdef __cplusplus
 }

@@ -489,6 +489,44 @@ void or_lbm_save_fields(const or_lbm *h, double *u, double *color, double *press
             }
 }
 
+/* LSM-LBM coupling force (L1:3849-3879): momentum given to the solid by
+   every bounce-back link, -2 (pcf_out - ppp fq) c_in rsi/dt^2, spread over
+   the link's 2 (edge, ipb2) or 4 (face, ipb4) solid nodes; af is (3,ns) in
+   the L1 solid numbering.  pcf are the post-collision populations of the
+   last step. */
+void or_lbm_fluid_force(const or_lbm *h, double rsi, double *af) {
+    const or_lbm_params *p = &h->p;
+    const double dt = p->dt, dti = 1.0 / dt, dt2 = dt * dt, ppp = 0.99;
+    const double rsdti2 = rsi * dti * dti;
+    const double c1d6 = 1.0 / 6.0, c1d12 = 1.0 / 12.0;
+    double fq[19];
+    fq[0] = p->rr0 * 0.0;
+    for (int q = 1; q < 19; q++) fq[q] = p->rr0 * (q < 7 ? p->cr2 * c1d6 * dt2 : p->cr2 * c1d12 * dt2);
+    memset(af, 0, 3 * (size_t)h->ns * sizeof(double));
+    for (int pass = 0; pass < 2; pass++) {
+        const int64_t nlink = pass == 0 ? h->npb2 : h->npb4;
+        const int w = pass == 0 ? 4 : 6, nsol = pass == 0 ? 2 : 4;
+        const double share = pass == 0 ? 0.5 : 0.25;
+        const int32_t *tab = pass == 0 ? h->ipb2 : h->ipb4;
+        for (int64_t s = 0; s < nlink; s++) {
+            const int32_t *e = &tab[w * s];
+            const int64_t n = e[0] - 1;
+            const int qb = e[1] - 1, q = h->niv[qb];
+            const double x = (h->pcfr[19 * n + q] + h->pcfb[19 * n + q]) - ppp * fq[qb];
+            double floc[3];
+            for (int d = 0; d < 3; d++) floc[d] = -2.0 * x * (double)IV[qb][d] * rsdti2;
+            for (int m = 0; m < nsol; m++) {
+                /* a wall voxel that is a second pore family (ind > 1 with
+                   another iipore) has no solid corner node: id 0, where the
+                   reference writes afdat(1:3,0) out of bounds; nothing here */
+                if (e[2 + m] <= 0) continue;
+                double *a = &af[3 * (int64_t)(e[2 + m] - 1)];
+                for (int d = 0; d < 3; d++) a[d] = a[d] + floc[d] * share;
+            }
+        }
+    }
+}
+
 /* density / colour field pass (L1:3436-3447, 3934-3945) */
 void or_lbm_densities(or_lbm *h) {
     for (int64_t i = 0; i < h->nl; i++) {

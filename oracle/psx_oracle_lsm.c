@@ -324,11 +324,12 @@ or_lsm *or_lsm_create(const or_lsm_params *p, const int32_t *ind_in, const int32
     h->r = (double *)calloc(3 * (ns + 1), sizeof(double));
     h->v = (double *)calloc(3 * (ns + 1), sizeof(double));
     h->a = (double *)calloc(3 * (ns + 1), sizeof(double));
-    /* glued pairs and initial jump (XX:1366-1375) */
+    /* glued pairs and initial jump (XX:1366-1375); none on L1's fully
+       periodic lattice (pad_axis < 0) */
     int o1 = (h->P + 1) % 3, o2 = (h->P + 2) % 3;
-    h->pairs = (int32_t *)malloc(2 * (int64_t)(h->G[o1] * h->G[o2] + 1) * sizeof(int32_t));
+    h->pairs = (int32_t *)malloc(2 * (int64_t)(h->P >= 0 ? h->G[o1] * h->G[o2] + 1 : 1) * sizeof(int32_t));
     h->npair = 0;
-    for (int s = 1; s <= h->G[o1]; s++)
+    for (int s = 1; h->P >= 0 && s <= h->G[o1]; s++)
         for (int t = 1; t <= h->G[o2]; t++) {
             int c1[3], c2[3];
             c1[h->P] = 1; c2[h->P] = h->G[h->P] - 1;
@@ -532,10 +533,162 @@ int64_t or_lsm_stress(const or_lsm *h, double *asig, double *astr, double *modul
             if (astr) astr[d + 3 * c] = st[c][d];
         }
     double vcell = (double)((int32_t)(h->n[0] * h->n[1] * h->n[2]));
-    double rnp = (double)h->n[h->P];
+    double rnp = h->P >= 0 ? (double)h->n[h->P] : 1.0;
     double den = vcell * (h->p.gstrain * 2.0) / rnp;
     if (moduli)
         for (int c = 0; c < 3; c++)
             for (int d = 0; d < 3; d++) moduli[d + 3 * c] = sp[c][d] / den;
     return ncub;
+}
+
+/* lsm_step (L1:1721-1903): the coupled model's velocity Verlet with the
+   fluid force af (3,ns) held over the step and damping `damp` (0.25,
+   L1:1888). */
+void or_lsm_coupled_step(or_lsm *h, const double *af, double damp) {
+    const double dt = h->p.dt, rsi = 1.0 / h->p.rs;
+    const int64_t n3 = 3 * h->ns;
+    for (int64_t i = 0; i < n3; i++) h->v[i] = h->v[i] + h->a[i] * dt * 0.5 + af[i] * dt * 0.5;
+    for (int64_t i = 0; i < n3; i++) h->r[i] = h->r[i] + h->v[i] * dt;
+    or_lsm_forces(h, h->r, h->a);
+    for (int64_t i = 0; i < n3; i++) h->a[i] = h->a[i] * rsi;
+    for (int64_t i = 0; i < n3; i++) h->a[i] = h->a[i] - damp * h->v[i];
+    for (int64_t i = 0; i < n3; i++) h->v[i] = h->v[i] + h->a[i] * dt * 0.5 + af[i] * dt * 0.5;
+}
+
+/* calc_ab (L1:4008-4468) on the fully periodic lattice (pad_axis < 0):
+   ind_raw = the voxel codes as read from `ind` (pore families 1..npores,
+   0 solid).  out: [0..8] asigmap, [9..17] asigmam, [18..26] astrain (sums
+   over the solid cubes, column-major), [27] epsi, [28..36] alphai,
+   [37] ncub, [38..] betai(npores). */
+void or_lsm_calc_ab(const or_lsm *h, const int32_t *ind_raw, int32_t iipore, int32_t npores, double pres,
+                    double *out) {
+    static const int icv[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0},
+                                  {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+    static const double sv[8][3] = {{-0.5, -0.5, -0.5}, {0.5, -0.5, -0.5}, {-0.5, 0.5, -0.5},
+                                    {0.5, 0.5, -0.5},   {-0.5, -0.5, 0.5}, {0.5, -0.5, 0.5},
+                                    {-0.5, 0.5, 0.5},   {0.5, 0.5, 0.5}};
+    const int nx = h->n[0], ny = h->n[1], nz = h->n[2];
+#define RAW(i, j, k) ind_raw[(int64_t)((i)-1) + (int64_t)nx * (((j)-1) + (int64_t)ny * ((k)-1))]
+#define NODE(c) h->mijks[gidx(h, (c)[0], (c)[1], (c)[2])]
+    const double vcell = (double)((int32_t)(nx * ny * nz));
+    double sbond[24], abond[72];
+    for (int l = 0; l < 24; l++) sbond[l] = h->csb[l] * h->p.alf1;
+    for (int l = 0; l < 72; l++) abond[l] = h->cab[l] * h->p.bet1;
+    double epsi = 0.0;
+    for (int i = 1; i <= nx; i++)
+        for (int j = 1; j <= ny; j++)
+            for (int k = 1; k <= nz; k++)
+                if (RAW(i, j, k) == iipore) epsi = epsi + 1.0;
+    epsi = epsi / vcell;
+    /* per-cube tensors summed in cube order; sp[c][r] = sigmap(r, c) */
+    double asp[3][3] = {{0}}, asm_[3][3] = {{0}}, ast[3][3] = {{0}};
+    int64_t ncub = 0;
+    for (int i = 1; i <= nx; i++)
+        for (int j = 1; j <= ny; j++)
+            for (int k = 1; k <= nz; k++) {
+                if (RAW(i, j, k) != 0) continue;
+                int icp[8][3];
+                for (int l = 0; l < 8; l++) {
+                    icp[l][0] = wrapn(i + icv[l][0], nx);
+                    icp[l][1] = wrapn(j + icv[l][1], ny);
+                    icp[l][2] = wrapn(k + icv[l][2], nz);
+                }
+                double sp[3][3] = {{0}}, sm[3][3] = {{0}}, st[3][3] = {{0}};
+                for (int l = 0; l < 8; l++)
+                    for (int m = 0; m < 3; m++)
+                        for (int n = 0; n < 3; n++) {
+                            const double *up = &h->r[3 * (NODE(icp[l]) - 1)];
+                            st[n][m] = st[n][m] + sv[l][m] * up[n] + sv[l][n] * up[m];
+                        }
+                for (int l = 0; l < 24; l++) {
+                    const int32_t *e = h->cbon[l];
+                    const double *r1 = &h->r[3 * (NODE(icp[e[0] - 1]) - 1)], *r2 = &h->r[3 * (NODE(icp[e[1] - 1]) - 1)];
+                    const double *xn = h->bn[e[2] - 1], *rn = h->vb[e[2] - 1];
+                    double rt[3], flin[3];
+                    for (int d = 0; d < 3; d++) rt[d] = r2[d] - r1[d] + rn[d];
+                    double dot = rt[0] * xn[0] + rt[1] * xn[1] + rt[2] * xn[2];
+                    for (int d = 0; d < 3; d++) flin[d] = sbond[l] * (dot * xn[d] - rn[d]);
+                    for (int c = 0; c < 3; c++) {
+                        if (xn[c] > 0)
+                            for (int d = 0; d < 3; d++) { sp[c][d] = sp[c][d] + flin[d]; sm[c][d] = sm[c][d] - flin[d]; }
+                        if (xn[c] < 0)
+                            for (int d = 0; d < 3; d++) { sp[c][d] = sp[c][d] - flin[d]; sm[c][d] = sm[c][d] + flin[d]; }
+                    }
+                }
+                for (int l = 0; l < 72; l++) {
+                    const int32_t *e = h->cang[l];
+                    const double *ri = &h->r[3 * (NODE(icp[e[0] - 1]) - 1)];
+                    const double *rj = &h->r[3 * (NODE(icp[e[1] - 1]) - 1)];
+                    const double *rk = &h->r[3 * (NODE(icp[e[2] - 1]) - 1)];
+                    const double *xnj = h->vn[e[3] - 1], *xrj = h->vx[e[3] - 1], *rnj = h->vv[e[3] - 1];
+                    const double *xnk = h->vn[e[4] - 1], *xrk = h->vx[e[4] - 1], *rnk = h->vv[e[4] - 1];
+                    const double *anf = h->anf[e[5] - 1];
+                    double rtj[3], rtk[3];
+                    for (int d = 0; d < 3; d++) { rtj[d] = rj[d] - ri[d] + rnj[d]; rtk[d] = rk[d] - ri[d] + rnk[d]; }
+                    double dfi = ((xrk[1] * rtk[2] - xrk[2] * rtk[1] - xrj[1] * rtj[2] + xrj[2] * rtj[1]) * anf[0] -
+                                  (xrk[0] * rtk[2] - xrk[2] * rtk[0] - xrj[0] * rtj[2] + xrj[2] * rtj[0]) * anf[1] +
+                                  (xrk[0] * rtk[1] - xrk[1] * rtk[0] - xrj[0] * rtj[1] + xrj[1] * rtj[0]) * anf[2]) *
+                                 abond[l];
+                    for (int c = 0; c < 3; c++) {
+                        if (xnj[c] > 0)
+                            for (int d = 0; d < 3; d++) { sp[c][d] = sp[c][d] - dfi * anf[3 + d]; sm[c][d] = sm[c][d] + dfi * anf[3 + d]; }
+                        if (xnj[c] < 0)
+                            for (int d = 0; d < 3; d++) { sp[c][d] = sp[c][d] + dfi * anf[3 + d]; sm[c][d] = sm[c][d] - dfi * anf[3 + d]; }
+                    }
+                    for (int c = 0; c < 3; c++) {
+                        if (xnk[c] > 0)
+                            for (int d = 0; d < 3; d++) { sp[c][d] = sp[c][d] - dfi * anf[6 + d]; sm[c][d] = sm[c][d] + dfi * anf[6 + d]; }
+                        if (xnk[c] < 0)
+                            for (int d = 0; d < 3; d++) { sp[c][d] = sp[c][d] + dfi * anf[6 + d]; sm[c][d] = sm[c][d] - dfi * anf[6 + d]; }
+                    }
+                }
+                for (int c = 0; c < 3; c++)
+                    for (int d = 0; d < 3; d++) {
+                        asp[c][d] = asp[c][d] + sp[c][d];
+                        asm_[c][d] = asm_[c][d] + sm[c][d];
+                        ast[c][d] = ast[c][d] + st[c][d] * 0.25;
+                    }
+                ncub++;
+            }
+    for (int c = 0; c < 3; c++)
+        for (int d = 0; d < 3; d++) {
+            out[d + 3 * c] = asp[c][d];
+            out[9 + d + 3 * c] = asm_[c][d];
+            out[18 + d + 3 * c] = ast[c][d];
+            out[28 + d + 3 * c] = -(c == d ? 1.0 : 0.0) * epsi + asp[c][d] / vcell / pres;
+        }
+    out[27] = epsi;
+    out[37] = (double)ncub;
+    /* betai: face-averaged solid displacement along the outward normal of
+       each pore family (L1:4384-4445) */
+    for (int iip = 1; iip <= npores; iip++) {
+        double b = 0.0;
+        for (int k = 1; k <= nz; k++)
+            for (int j = 1; j <= ny; j++)
+                for (int i = 1; i <= nx; i++) {
+                    if (RAW(i, j, k) != iip) continue;
+                    for (int l = 1; l <= 6; l++) {
+                        const int *c = IV[l];
+                        if (RAW(wrapn(i + c[0], nx), wrapn(j + c[1], ny), wrapn(k + c[2], nz)) != 0) continue;
+                        int lp[4][3];
+                        for (int m = 0; m < 4; m++) {
+                            int o1 = m & 1, o2 = m >> 1, off[3];
+                            if (c[0] != 0) { off[0] = (c[0] + 1) / 2; off[1] = o1; off[2] = o2; }
+                            else if (c[1] != 0) { off[0] = o1; off[1] = (c[1] + 1) / 2; off[2] = o2; }
+                            else { off[0] = o1; off[1] = o2; off[2] = (c[2] + 1) / 2; }
+                            lp[m][0] = wrapn(i + off[0], nx);
+                            lp[m][1] = wrapn(j + off[1], ny);
+                            lp[m][2] = wrapn(k + off[2], nz);
+                        }
+                        const double *r3 = &h->r[3 * (NODE(lp[0]) - 1)], *r4 = &h->r[3 * (NODE(lp[1]) - 1)];
+                        const double *r5 = &h->r[3 * (NODE(lp[2]) - 1)], *r6 = &h->r[3 * (NODE(lp[3]) - 1)];
+                        double up[3];
+                        for (int d = 0; d < 3; d++) up[d] = (r3[d] + r4[d] + r5[d] + r6[d]) * 0.25;
+                        b = b - up[0] * h->vv[l][0] - up[1] * h->vv[l][1] - up[2] * h->vv[l][2];
+                    }
+                }
+        out[38 + iip - 1] = b / vcell / pres;
+    }
+#undef RAW
+#undef NODE
 }

@@ -38,10 +38,12 @@
 #include <algorithm>
 #include <cmath>
 #include <type_traits>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <new>
 #include <stdexcept>
+#include <utility>
 #include <vector>
 
 #include "psx_common.h"
@@ -69,6 +71,16 @@ struct psx_lbm {
     int cur = 0;
     int xcd = 1;
     bool f32 = false;
+    // FP32 fused step (k_step_f32): nodes numbered brick by brick; perm maps
+    // that numbering to the reference's (empty: the reference's own order)
+    bool fused = false;
+    int box[3] = {0, 0, 0};
+    std::vector<int32_t> perm;
+    psx_lbm_fast::Brick *d_bricks = nullptr;
+    int nbricks = 0;
+    int32_t *d_hnode = nullptr;
+    int16_t *d_hcell = nullptr;
+    inline int64_t ref_node(int64_t i) const { return perm.empty() ? i : perm[(size_t)i]; }
 };
 
 namespace {
@@ -199,7 +211,10 @@ int launch_collide(psx_lbm *h) {
     int64_t nb = psx::ceil_div(h->nl, 256);
     if (nb == 0) return PSX_OK;
     int a = h->cur, b = 1 - h->cur;
-    if (h->f32) {
+    if (h->fused) {
+        PSX_HIP(psx_lbm_fast::step(h->c, (const float *)h->d_pop[a], (float *)h->d_pop[b], h->d_dst, h->d_vox,
+                                   h->d_bricks, h->nbricks, h->d_hnode, h->d_hcell, h->box, h->xcd, h->stream));
+    } else if (h->f32) {
         PSX_HIP(psx_lbm_fast::collide(h->c, (const float *)h->d_pop[a], (float *)h->d_pop[b], h->d_dst, h->d_src16,
                                       h->d_pbase, h->d_vox, (const double *)h->d_phi, h->xcd, h->stream));
     } else {
@@ -210,6 +225,79 @@ int launch_collide(psx_lbm *h) {
     }
     h->cur = b;
     return PSX_OK;
+}
+
+// Renumbers the FP32 nodes brick by brick (bricks of bd[0] x bd[1] x bd[2]
+// voxels in x-outer / z-fastest brick order, the reference's order inside a
+// brick), permutes the pull table and vox accordingly, and lists per brick
+// the fluid voxels of its halo box whose colour some brick node's calcFs
+// window reads.
+void build_bricks(psx_lbm *h, const int (&bd)[3], std::vector<int32_t> &vox,
+                  std::vector<psx_lbm_fast::Brick> &bricks, std::vector<int32_t> &hnode,
+                  std::vector<int16_t> &hcell) {
+    const int nx = h->nx, ny = h->ny, nz = h->nz;
+    const int64_t nl = h->nl;
+    const int nbx = (nx + bd[0] - 1) / bd[0], nby = (ny + bd[1] - 1) / bd[1], nbz = (nz + bd[2] - 1) / bd[2];
+    const int64_t nkey = (int64_t)nbx * nby * nbz;
+    auto key_of = [&](int64_t n) {
+        const int32_t *ic = &h->icor[3 * n];
+        return ((int64_t)((ic[0] - 1) / bd[0]) * nby + (ic[1] - 1) / bd[1]) * nbz + (ic[2] - 1) / bd[2];
+    };
+    std::vector<int64_t> start(nkey + 1, 0);
+    for (int64_t n = 0; n < nl; n++) start[key_of(n) + 1]++;
+    for (int64_t k = 0; k < nkey; k++) start[k + 1] += start[k];
+    std::vector<int64_t> fill(start.begin(), start.end() - 1);
+    h->perm.assign((size_t)nl, 0);
+    std::vector<int32_t> inv((size_t)nl);
+    for (int64_t n = 0; n < nl; n++) {  // stable: reference order inside a brick
+        const int64_t i = fill[key_of(n)]++;
+        h->perm[(size_t)i] = (int32_t)n;
+        inv[(size_t)n] = (int32_t)i;
+    }
+    std::vector<int32_t> src2(h->dst_h.size()), vox2((size_t)nl);
+    for (int q = 1; q < Q; q++)
+        for (int64_t i = 0; i < nl; i++) {
+            const int32_t sidx = h->dst_h[(size_t)(q - 1) * nl + h->perm[(size_t)i]];
+            src2[(size_t)(q - 1) * nl + i] = sidx >= 0 ? inv[(size_t)sidx] : -1;
+        }
+    for (int64_t i = 0; i < nl; i++) vox2[(size_t)i] = vox[(size_t)h->perm[(size_t)i]];
+    h->dst_h.swap(src2);
+    vox.swap(vox2);
+    const int lx = h->box[0], ly = h->box[1], lz = h->box[2];
+    std::vector<uint8_t> need((size_t)lx * ly * lz);
+    std::vector<std::pair<int32_t, int16_t>> ent;
+    for (int64_t k = 0; k < nkey; k++) {
+        if (start[k + 1] == start[k]) continue;
+        const int X = (int)(k / ((int64_t)nby * nbz)), Y = (int)((k / nbz) % nby), Z = (int)(k % nbz);
+        psx_lbm_fast::Brick B{};
+        B.nstart = (int32_t)start[k];
+        B.ncount = (int32_t)(start[k + 1] - start[k]);
+        B.x0 = X * bd[0]; B.y0 = Y * bd[1]; B.z0 = Z * bd[2];
+        std::fill(need.begin(), need.end(), 0);
+        for (int64_t i = start[k]; i < start[k + 1]; i++) {
+            const int32_t *ic = &h->icor[3 * (size_t)h->perm[(size_t)i]];
+            const int cx = ic[0] - 1 - B.x0 + 1, cy = ic[1] - 1 - B.y0 + 1, cz = ic[2] - 1 - B.z0 + 1;
+            for (int a = -1; a <= 1; a++)
+                for (int b = -1; b <= 1; b++)
+                    for (int e = -1; e <= 1; e++) need[(size_t)(cz + e) + lz * ((cy + b) + (size_t)ly * (cx + a))] = 1;
+        }
+        ent.clear();
+        for (int a = 0; a < lx; a++)
+            for (int b = 0; b < ly; b++)
+                for (int e = 0; e < lz; e++) {
+                    const size_t cell = (size_t)e + lz * (b + (size_t)ly * a);
+                    if (!need[cell]) continue;
+                    const int x = ((B.x0 - 1 + a) % nx + nx) % nx, y = ((B.y0 - 1 + b) % ny + ny) % ny,
+                              z = ((B.z0 - 1 + e) % nz + nz) % nz;
+                    const int32_t m = h->mijk[(size_t)x + (size_t)nx * (y + (size_t)ny * z)];
+                    if (m > 0) ent.push_back({inv[(size_t)m - 1], (int16_t)cell});
+                }
+        std::sort(ent.begin(), ent.end());
+        B.hstart = (int32_t)hnode.size();
+        B.hcount = (int32_t)ent.size();
+        for (auto &pr : ent) { hnode.push_back(pr.first); hcell.push_back(pr.second); }
+        bricks.push_back(B);
+    }
 }
 
 // host-side conversion of one population pair into FP32 storage (same
@@ -289,6 +377,30 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
             }
         h->dst_h.swap(src);  // the FP32 path keeps src in place of dst
     }
+    std::vector<psx_lbm_fast::Brick> bricks;
+    std::vector<int32_t> hnode;
+    std::vector<int16_t> hcell;
+    if (h->f32 && nl > 0) {  // opt-in: measured slower than the two-kernel step (occupancy 2)
+        const char *ef = std::getenv("PSX_LBM_FUSED");
+        h->fused = ef && std::atoi(ef) != 0;
+    }
+    if (h->fused) {
+        int bd[3] = {8, 8, 16};  // brick (x, y, z); PSX_LBM_BRICK="bx,by,bz"
+        if (const char *eb = std::getenv("PSX_LBM_BRICK")) {
+            if (std::sscanf(eb, "%d,%d,%d", &bd[0], &bd[1], &bd[2]) != 3 || bd[0] < 1 || bd[1] < 1 || bd[2] < 1) {
+                psx::set_error("psx_lbm_create: PSX_LBM_BRICK must be bx,by,bz");
+                delete h;
+                return PSX_EINVAL;
+            }
+        }
+        for (int d = 0; d < 3; d++) h->box[d] = bd[d] + 2;
+        if ((int64_t)h->box[0] * h->box[1] * h->box[2] * 8 > 65536) {
+            psx::set_error("psx_lbm_create: brick halo box exceeds 64 KB of LDS");
+            delete h;
+            return PSX_EINVAL;
+        }
+        build_bricks(h, bd, vox, bricks, hnode, hcell);
+    }
     // Compressed pull table (FP32 path): the source of direction q lies in
     // plane X - c_qx; its index is n + pbase[c_qx + 1][X] + d with d the
     // difference of the two nodes' ranks within their planes, which fits
@@ -297,7 +409,7 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
     std::vector<int16_t> s16;
     std::vector<int32_t> pbase;
     bool use16 = false;
-    if (h->f32 && nl > 0) {
+    if (h->f32 && !h->fused && nl > 0) {
         const char *e16 = std::getenv("PSX_LBM_IDX16");
         use16 = !e16 || std::atoi(e16) != 0;
     }
@@ -342,6 +454,18 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
     int64_t nb = std::max<int64_t>(1, psx::ceil_div(nl, 256));
     PSX_HIP(hipMalloc(&h->d_partial, sizeof(double) * NDIAG * nb));
     PSX_HIP(hipMalloc(&h->d_red, sizeof(double) * NDIAG));
+    if (h->fused) {
+        h->nbricks = (int)bricks.size();
+        PSX_HIP(hipMalloc(&h->d_bricks, sizeof(psx_lbm_fast::Brick) * std::max<size_t>(1, bricks.size())));
+        PSX_HIP(hipMalloc(&h->d_hnode, sizeof(int32_t) * std::max<size_t>(1, hnode.size())));
+        PSX_HIP(hipMalloc(&h->d_hcell, sizeof(int16_t) * std::max<size_t>(1, hcell.size())));
+        PSX_HIP(hipMemcpyAsync(h->d_bricks, bricks.data(), sizeof(psx_lbm_fast::Brick) * bricks.size(),
+                               hipMemcpyHostToDevice, h->stream));
+        PSX_HIP(hipMemcpyAsync(h->d_hnode, hnode.data(), sizeof(int32_t) * hnode.size(), hipMemcpyHostToDevice,
+                               h->stream));
+        PSX_HIP(hipMemcpyAsync(h->d_hcell, hcell.data(), sizeof(int16_t) * hcell.size(), hipMemcpyHostToDevice,
+                               h->stream));
+    }
     if (nl > 0) {
         PSX_HIP(hipMemcpyAsync(h->d_dst, h->dst_h.data(), sizeof(int32_t) * 18 * nl, hipMemcpyHostToDevice, h->stream));
         PSX_HIP(hipMemcpyAsync(h->d_vox, vox.data(), sizeof(int32_t) * nl, hipMemcpyHostToDevice, h->stream));
@@ -359,7 +483,7 @@ extern "C" void psx_lbm_destroy(psx_lbm *h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void *bufs[] = {h->d_pop[0], h->d_pop[1], h->d_dst, h->d_vox, h->d_phi, h->d_partial, h->d_red, h->d_src16,
-                    h->d_pbase};
+                    h->d_pbase, h->d_bricks, h->d_hnode, h->d_hcell};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -459,7 +583,8 @@ extern "C" int psx_lbm_set_state(psx_lbm *h, const double *fr, const double *fb)
                     if (sidx >= 0) j = sidx;
                     else k = opp(q);
                 }
-                f32_encode(h->c.w0[q], fr[Q * i + q], fb[Q * i + q], st[aos(k, j)], st[aos(Q + k, j)]);
+                const int64_t r = h->ref_node(i);
+                f32_encode(h->c.w0[q], fr[Q * r + q], fb[Q * r + q], st[aos(k, j)], st[aos(Q + k, j)]);
             }
         PSX_HIP(hipMemcpyAsync(h->d_pop[0], st.data(), sizeof(float) * st.size(), hipMemcpyHostToDevice, h->stream));
         PSX_HIP(hipStreamSynchronize(h->stream));
@@ -500,8 +625,9 @@ extern "C" int psx_lbm_get_state(psx_lbm *h, double *fr, double *fb) {
                 }
                 double f = h->c.w0[q] + (double)st[aos(k, j)];
                 double r = (double)st[aos(Q + k, j)];
-                fr[Q * i + q] = r * f;
-                fb[Q * i + q] = f - r * f;
+                const int64_t o = h->ref_node(i);
+                fr[Q * o + q] = r * f;
+                fb[Q * o + q] = f - r * f;
             }
     } else {
         std::vector<double> st((size_t)2 * Q * nl);
@@ -546,6 +672,10 @@ extern "C" int psx_lbm_fields(psx_lbm *h, double *u, double *color, double *pres
         if (phase) phase[a] = 0;
     }
     if (nl == 0) return PSX_OK;
+    if (h->fused) {  // the fused step keeps the colour field in LDS only
+        int rc = launch_density(h, h->cur);
+        if (rc) return rc;
+    }
     std::vector<double> out((size_t)5 * nl);
     std::vector<int32_t> ph((size_t)nl);
     double *d_out = nullptr;
@@ -573,7 +703,7 @@ extern "C" int psx_lbm_fields(psx_lbm *h, double *u, double *color, double *pres
     if (d_ph) (void)hipFree(d_ph);
     PSX_HIP(e);
     for (int64_t i = 0; i < nl; i++) {
-        const int32_t *ic = &h->icor[3 * i];
+        const int32_t *ic = &h->icor[3 * h->ref_node(i)];
         const int64_t a = (int64_t)(ic[0] - 1) + (int64_t)h->nx * ((ic[1] - 1) + (int64_t)h->ny * (ic[2] - 1));
         if (u)
             for (int d = 0; d < 3; d++) u[3 * a + d] = out[(size_t)d * nl + i];
@@ -609,7 +739,7 @@ extern "C" int psx_lbm_step(psx_lbm *h, int64_t nsteps) {
     for (int64_t s = 0; s < nsteps; s++) {
         int rc = launch_collide(h);
         if (rc) return rc;
-        rc = launch_density(h, h->cur);
+        if (!h->fused) rc = launch_density(h, h->cur);
         if (rc) return rc;
     }
     return PSX_OK;
@@ -630,7 +760,7 @@ extern "C" int psx_lbm_profile(psx_lbm *h, int64_t nsteps, double *ms) {
         int rc = launch_collide(h);
         if (rc) return rc;
         PSX_HIP(hipEventRecord(ev[2 * s + 1], h->stream));
-        rc = launch_density(h, h->cur);
+        if (!h->fused) rc = launch_density(h, h->cur);
         if (rc) return rc;
         PSX_HIP(hipEventRecord(ev[2 * s + 2], h->stream));
     }
@@ -663,6 +793,10 @@ extern "C" int psx_lbm_check_state(psx_lbm *h, int64_t it1, int64_t itw, psx_lbm
     double s[NDIAG] = {0};
     int64_t nb = psx::ceil_div(h->nl, 256);
     if (nb > 0) {
+        if (h->fused) {  // the fused step keeps the colour field in LDS only
+            int rc = launch_density(h, h->cur);
+            if (rc) return rc;
+        }
         if (h->f32) {
             PSX_HIP(psx_lbm_fast::diag(h->c, (const float *)h->d_pop[h->cur], h->d_dst, h->d_vox,
                                        (const double *)h->d_phi, h->d_partial, h->stream));

@@ -210,14 +210,10 @@ __device__ __forceinline__ void collide_node(double t1, double t2, double t3, do
     for (int q = 1; q < Q; q++) pcf[q] = f[q] - mb[q] + Fp[q];
 }
 
-// calcFs (L1:2626-2796) reading the phi grid around voxel (x,y,z).
-template <typename P, bool EXACT = true>
-__device__ __forceinline__ void calc_fs(const LbmConst &c, const P *__restrict__ phi, int x, int y, int z,
-                                        double *Cg, double *Fs) {
-    const int nx = c.nx, ny = c.ny, nz = c.nz;
-    int xs[3] = {x == 0 ? nx - 1 : x - 1, x, x == nx - 1 ? 0 : x + 1};
-    int ys[3] = {y == 0 ? ny - 1 : y - 1, y, y == ny - 1 ? 0 : y + 1};
-    int zs[3] = {z == 0 ? nz - 1 : z - 1, z, z == nz - 1 ? 0 : z + 1};
+// calcFs (L1:2626-2796) on the 3x3x3 window of the colour field around a
+// voxel; phi3(i, j, k) returns the value at offset (i-1, j-1, k-1).
+template <bool EXACT, class Phi3>
+__device__ __forceinline__ void calc_fs_window(const LbmConst &c, const Phi3 &phi3, double *Cg, double *Fs) {
     double cc[27];  // cc[i + 3j + 9k]
 #pragma unroll
     for (int k = 0; k < 3; k++)
@@ -225,7 +221,7 @@ __device__ __forceinline__ void calc_fs(const LbmConst &c, const P *__restrict__
         for (int j = 0; j < 3; j++)
 #pragma unroll
             for (int i = 0; i < 3; i++) {
-                double g = (double)phi[(int64_t)zs[k] + (int64_t)nz * (ys[j] + (int64_t)ny * xs[i])];
+                double g = phi3(i, j, k);
                 cc[i + 3 * j + 9 * k] = (g - c.wet) + c.wet;  // solid voxels hold wet
             }
 #define C3(i, j, k) cc[((i)-1) + 3 * ((j)-1) + 9 * ((k)-1)]
@@ -286,6 +282,20 @@ __device__ __forceinline__ void calc_fs(const LbmConst &c, const P *__restrict__
     Fs[0] = c.sig * (k2 - k1) * Cg[0];
     Fs[1] = c.sig * (k2 - k1) * Cg[1];
     Fs[2] = c.sig * (k2 - k1) * Cg[2];
+}
+
+// calcFs reading the periodic phi grid (z fastest) around voxel (x,y,z).
+template <typename P, bool EXACT = true>
+__device__ __forceinline__ void calc_fs(const LbmConst &c, const P *__restrict__ phi, int x, int y, int z,
+                                        double *Cg, double *Fs) {
+    const int nx = c.nx, ny = c.ny, nz = c.nz;
+    const int xs[3] = {x == 0 ? nx - 1 : x - 1, x, x == nx - 1 ? 0 : x + 1};
+    const int ys[3] = {y == 0 ? ny - 1 : y - 1, y, y == ny - 1 ? 0 : y + 1};
+    const int zs[3] = {z == 0 ? nz - 1 : z - 1, z, z == nz - 1 ? 0 : z + 1};
+    auto phi3 = [&](int i, int j, int k) -> double {
+        return (double)phi[(int64_t)zs[k] + (int64_t)nz * (ys[j] + (int64_t)ny * xs[i])];
+    };
+    calc_fs_window<EXACT>(c, phi3, Cg, Fs);
 }
 
 __device__ __forceinline__ void vox_xyz(const LbmConst &c, int32_t v, int &x, int &y, int &z) {

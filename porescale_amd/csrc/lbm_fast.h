@@ -4,6 +4,15 @@
 
 namespace psx_lbm_fast {
 
+// One brick of the fused step: its nodes are [nstart, nstart + ncount); the
+// colour field of its halo box is computed for the hcount nodes
+// hnode[hstart..] into LDS cells hcell[hstart..]; (x0, y0, z0) is the voxel
+// at box cell (1, 1, 1), i.e. the brick's first corner.
+struct Brick {
+    int32_t nstart, ncount, hstart, hcount;
+    int32_t x0, y0, z0, pad_;
+};
+
 // Pull + collide over all nodes; `in`/`out` are AoSoA (T, R) blocks.  The
 // pull table is src16 (int16 residuals, -32768 = bounce-back) with the
 // per-plane bases pbase[3][nx] when src16 is not null, else src [18][nl]
@@ -14,6 +23,11 @@ hipError_t collide(const psx_lbm_dev::LbmConst &c, const float *in, float *out, 
 // Pulled densities -> colour field phi at every node's voxel.
 hipError_t density(const psx_lbm_dev::LbmConst &c, const float *in, const int32_t *src, const int16_t *src16,
                    const int32_t *pbase, const int32_t *vox, double *phi, int xcd, hipStream_t st);
+// Fused pull + colour + collide step over bricks (k_step_f32): one launch
+// per time step.  box = the LDS box (brick + halo) dimensions (x, y, z).
+hipError_t step(const psx_lbm_dev::LbmConst &c, const float *in, float *out, const int32_t *src,
+                const int32_t *vox, const Brick *bricks, int nbricks, const int32_t *hnode, const int16_t *hcell,
+                const int (&box)[3], int xcd, hipStream_t st);
 // check_state partial sums (k_diag) of the pulled state.
 hipError_t diag(const psx_lbm_dev::LbmConst &c, const float *in, const int32_t *src, const int32_t *vox,
                 const double *phi, double *partial, hipStream_t st);

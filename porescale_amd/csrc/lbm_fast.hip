@@ -182,16 +182,13 @@ __device__ __forceinline__ void collide_fast(const LbmConst &c, double t1, doubl
     f[17] = f[17] - mb17 - F12 * (-bf[1] + bf[2]);
 }
 
-template <bool I16>
-__global__ __launch_bounds__(BLOCK) void k_collide_f32(const LbmConst c, const float *__restrict__ in,
-                                                       float *__restrict__ out, const PullIdx P,
-                                                       const int32_t *__restrict__ vox,
-                                                       const double *__restrict__ phi, int xcd) {
-    int64_t b = xcd ? psx::xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
-    int64_t i = b * blockDim.x + threadIdx.x;
+// Pull + collide + recolour of node i (voxel v) into its own slots of out.
+// fs(x, y, z, Cg, Fs) evaluates calcFs at the node's voxel.
+template <bool I16, class CalcFs>
+__device__ __forceinline__ void collide_pull(const LbmConst &c, const float *__restrict__ in,
+                                             float *__restrict__ out, const PullIdx &P, int64_t i, int32_t v,
+                                             const CalcFs &fs) {
     const int64_t nl = c.nl;
-    if (i >= nl) return;
-    const int32_t v = vox[i];
     int32_t pb[3];
     load_bases<I16>(c, P, v, pb);
     double f[Q];
@@ -240,7 +237,7 @@ __global__ __launch_bounds__(BLOCK) void k_collide_f32(const LbmConst c, const f
         int x, y, z;
         vox_xyz(c, v, x, y, z);
         double Fs[3];
-        calc_fs<double, false>(c, phi, x, y, z, Cg, Fs);
+        fs(x, y, z, Cg, Fs);
         r = rr + rb;
         double ci2 = (c.cr2 * rr + c.cb2 * rb) / r;
         t1 = 1.0 - 2.0 * ci2 * dt2;
@@ -285,15 +282,10 @@ __global__ __launch_bounds__(BLOCK) void k_collide_f32(const LbmConst c, const f
     }
 }
 
+// Colour field of node i from its pulled populations.
 template <bool I16>
-__global__ __launch_bounds__(BLOCK) void k_density_f32(const LbmConst c, const float *__restrict__ in,
-                                                       const PullIdx P, const int32_t *__restrict__ vox,
-                                                       double *__restrict__ phi, int xcd) {
-    int64_t b = xcd ? psx::xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
-    int64_t i = b * blockDim.x + threadIdx.x;
-    const int64_t nl = c.nl;
-    if (i >= nl) return;
-    const int32_t v = vox[i];
+__device__ __forceinline__ double colour_pull(const LbmConst &c, const float *__restrict__ in, const PullIdx &P,
+                                              int64_t i, int32_t v) {
     int32_t pb[3];
     load_bases<I16>(c, P, v, pb);
     double rr = 0.0, rb = 0.0;
@@ -301,14 +293,72 @@ __global__ __launch_bounds__(BLOCK) void k_density_f32(const LbmConst c, const f
     for (int q = 0; q < Q; q++) {
         int64_t j;
         int k;
-        pull_from<I16>(P, pb, nl, i, q, j, k);
+        pull_from<I16>(P, pb, c.nl, i, q, j, k);
         int64_t a = aos(k, j);
         double t = c.w0[q] + (double)in[a];
         double rt = (double)in[a + Q * AOS] * t;
         rr += rt;
         rb += t - rt;
     }
-    phi[v] = (rr - rb) / (rr + rb);
+    return (rr - rb) / (rr + rb);
+}
+
+template <bool I16>
+__global__ __launch_bounds__(BLOCK) void k_collide_f32(const LbmConst c, const float *__restrict__ in,
+                                                       float *__restrict__ out, const PullIdx P,
+                                                       const int32_t *__restrict__ vox,
+                                                       const double *__restrict__ phi, int xcd) {
+    int64_t b = xcd ? psx::xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+    int64_t i = b * blockDim.x + threadIdx.x;
+    if (i >= c.nl) return;
+    auto fs = [&](int x, int y, int z, double *Cg, double *Fs) { calc_fs<double, false>(c, phi, x, y, z, Cg, Fs); };
+    collide_pull<I16>(c, in, out, P, i, vox[i], fs);
+}
+
+template <bool I16>
+__global__ __launch_bounds__(BLOCK) void k_density_f32(const LbmConst c, const float *__restrict__ in,
+                                                       const PullIdx P, const int32_t *__restrict__ vox,
+                                                       double *__restrict__ phi, int xcd) {
+    int64_t b = xcd ? psx::xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+    int64_t i = b * blockDim.x + threadIdx.x;
+    if (i >= c.nl) return;
+    const int32_t v = vox[i];
+    phi[v] = colour_pull<I16>(c, in, P, i, v);
+}
+
+// One whole step over one brick of voxels per workgroup: the colour field of
+// every fluid voxel of the brick and its one-voxel halo goes into LDS (an
+// (bx+2) x (by+2) x (bz+2) box, z fastest, solid cells holding wet), then
+// the brick's own nodes collide reading the colour window from LDS.  Nodes
+// are numbered brick by brick, so a brick's nodes are one index range.
+__global__ __launch_bounds__(BLOCK) void k_step_f32(const LbmConst c, const float *__restrict__ in,
+                                                    float *__restrict__ out, const int32_t *__restrict__ src,
+                                                    const int32_t *__restrict__ vox,
+                                                    const psx_lbm_fast::Brick *__restrict__ bricks,
+                                                    const int32_t *__restrict__ hnode,
+                                                    const int16_t *__restrict__ hcell, int lx, int ly, int lz,
+                                                    int xcd) {
+    extern __shared__ double phs[];
+    const int64_t t = xcd ? psx::xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+    const psx_lbm_fast::Brick B = bricks[t];
+    const PullIdx P{src, nullptr, nullptr};
+    const int ncell = lx * ly * lz;
+    for (int k = threadIdx.x; k < ncell; k += BLOCK) phs[k] = c.wet;
+    __syncthreads();
+    for (int k = threadIdx.x; k < B.hcount; k += BLOCK) {
+        const int64_t i = hnode[B.hstart + k];
+        phs[hcell[B.hstart + k]] = colour_pull<false>(c, in, P, i, vox[i]);
+    }
+    __syncthreads();
+    auto fs = [&](int x, int y, int z, double *Cg, double *Fs) {
+        const int cell = (z - B.z0) + lz * ((y - B.y0) + ly * (x - B.x0));  // the cell of offset (-1,-1,-1)
+        auto phi3 = [&](int a, int b2, int e) -> double { return phs[cell + e + lz * (b2 + ly * a)]; };
+        calc_fs_window<false>(c, phi3, Cg, Fs);
+    };
+    for (int k = threadIdx.x; k < B.ncount; k += BLOCK) {
+        const int64_t i = B.nstart + k;
+        collide_pull<false>(c, in, out, P, i, vox[i], fs);
+    }
 }
 
 }  // namespace
@@ -337,6 +387,16 @@ hipError_t density(const LbmConst &c, const float *in, const int32_t *src, const
         hipLaunchKernelGGL(k_density_f32<true>, dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, P, vox, phi, xcd);
     else
         hipLaunchKernelGGL(k_density_f32<false>, dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, P, vox, phi, xcd);
+    return hipGetLastError();
+}
+
+hipError_t step(const LbmConst &c, const float *in, float *out, const int32_t *src, const int32_t *vox,
+                const Brick *bricks, int nbricks, const int32_t *hnode, const int16_t *hcell, const int (&box)[3],
+                int xcd, hipStream_t st) {
+    if (nbricks == 0) return hipSuccess;
+    const size_t lds = sizeof(double) * (size_t)box[0] * box[1] * box[2];
+    hipLaunchKernelGGL(k_step_f32, dim3((unsigned)nbricks), dim3(BLOCK), lds, st, c, in, out, src, vox, bricks,
+                       hnode, hcell, box[0], box[1], box[2], xcd);
     return hipGetLastError();
 }
 

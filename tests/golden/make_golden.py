@@ -13,6 +13,11 @@ Fixtures (compressed .npz, inputs and reference outputs only):
   lbm_blobs_fields.npz  the save_results files of the lbm_blobs run's last
                   iteration: velocity_field, color_field, pressure<it1> and
                   the last ind_red_blue block.
+  lbm_coupled.npz L1 with its coupled solid lattice (12x10x14, Em 0.2, nu 0.2,
+                  rs 2): 2 pre-steps + 2*6 coupled steps from initdistr; the
+                  LSM_disf2/LSM_velocity_field/LSM_force_field/LSM_fluid_force
+                  files and the last alpha_beta block (averaged tensors as
+                  printed: sums / ncub).
   lsm_xx.npz      XX_P.f, two-component 10x12x14 pack, 100 Verlet steps,
                   nthread=3: r, v, a and stress_tensor moduli.
   lsm_yz.npz      YZ_P.f, same geometry, nthread=1.
@@ -49,6 +54,16 @@ def lbm_cases():
     return (ind0, inp0), (ind1, inp1, blue)
 
 
+def coupled_case():
+    """L1 with its coupled solid: stable springs (Em 0.2, nu 0.2, rs 2)."""
+    ind = geo.sphere_pack((12, 10, 14), 0.45, 2.5, seed=7)
+    nz, ny, nx = ind.shape
+    inp = pio.LBMInput(B=0.7, sig=0.02, vir=0.1, vib=0.2, wet=0.3, idir=3, itm=2, itw=6, sat=0.5,
+                       ncx=nx, ncy=ny, ncz=nz, dt=1.0, bfr=[0, 1e-4, 2e-4], bfb=[1e-5, 0, 2e-4],
+                       Em=0.2, nu=0.2, rs=2.0, dl=1.0)
+    return ind, inp
+
+
 def lsm_geometry():
     ind = geo.sphere_pack((10, 12, 14), 0.4, 3.0, seed=5)
     return geo.two_component(ind, 0.3, seed=6, radius=2.0)
@@ -67,6 +82,11 @@ def main():
     np.savez_compressed(os.path.join(HERE, "lbm_blobs.npz"), fr=fr, fb=fb, outm=outm, steps=62)
     np.savez_compressed(os.path.join(HERE, "lbm_blobs_fields.npz"), it1=it1, iter=fl["iter"],
                         velocity=fl["velocity"], color=fl["color"], pressure=fl["pressure"], phase=fl["phase"])
+    ind, inp = coupled_case()
+    fr, fb, outm, it1, lsmf, ab = ob.run_ref_coupled(inp, ind)
+    np.savez_compressed(os.path.join(HERE, "lbm_coupled.npz"), fr=fr, fb=fb, it1=it1,
+                        **{f"lsm_{k}": v for k, v in lsmf.items()},
+                        **{f"ab_{k}": np.asarray(v) for k, v in ab.items()})
     ind = lsm_geometry()
     nz, ny, nx = ind.shape
     for case, nth in (("xx", 3), ("yz", 1)):

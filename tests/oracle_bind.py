@@ -91,6 +91,9 @@ def lib():
         L.or_lsm_stress.restype = C.c_int64
         L.or_lsm_stress.argtypes = [vp, _P(np.float64), _P(np.float64), _P(np.float64)]
         L.or_lsm_anf.argtypes = [_P(np.float64), _P(np.int32)]
+        L.or_lbm_fluid_force.argtypes = [vp, C.c_double, _P(np.float64)]
+        L.or_lsm_coupled_step.argtypes = [vp, _P(np.float64), C.c_double]
+        L.or_lsm_calc_ab.argtypes = [vp, _P(np.int32), C.c_int32, C.c_int32, C.c_double, _P(np.float64)]
         _lib = L
     return _lib
 
@@ -325,6 +328,22 @@ def run_ref_lbm(inp: pio.LBMInput, ind: np.ndarray, restart=None, timeout=600.0,
         shutil.rmtree(d, ignore_errors=True)
 
 
+def run_ref_coupled(inp: pio.LBMInput, ind: np.ndarray, timeout=600.0):
+    """Run L1 (icon as in inp); returns (fr, fb, output.m rows, it1, solid
+    files of the last save (read_lsm_files), last alpha_beta block)."""
+    d = _prep_dir(ind)
+    try:
+        pio.write_lbm_input(os.path.join(d, "INPUT_PARAMETERS"), inp)
+        nl = int((ind == inp.iipore).sum())
+        _run("l1_ref", d, timeout)
+        it1, fr, fb = pio.read_lbm_restart(os.path.join(d, "iteration2"), nl)
+        outm = pio.read_output_m(os.path.join(d, "output.m"))
+        return (fr, fb, outm, it1, read_lsm_files(d, it1, ind.shape),
+                read_alpha_beta(os.path.join(d, "alpha_beta"), int(inp.npores)))
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def read_field_files(d: str, it1: int, shape) -> dict:
     """velocity_field, color_field, pressure<it1> and the last ind_red_blue
     block as written by save_results; ``shape`` = (ncz, ncy, ncx)."""
@@ -374,3 +393,115 @@ def parse_stress_tensor(text: str) -> dict:
             if s.startswith(key):
                 out[key] = pio.parse_fortran_numbers(s.split("=", 1)[1])[0]
     return out
+
+
+def coupled_springs(Em: float, nu: float, dl: float):
+    """alf, bet of the coupled model (L1:497-498)."""
+    alf = dl * Em / (5.0 - 10.0 * nu)
+    bet = dl * dl * dl * Em * (4 * nu - 1) / (20.0 * (2 * nu * nu + nu - 1))
+    return alf, bet
+
+
+class OracleCoupled:
+    """L1's one-way coupled model: the LBM oracle, and the solid lattice of
+    the same ind (corners of solid voxels, periodic in x, y and z) moved by
+    the fluid force of every main-loop step (L1:3849-3888)."""
+
+    def __init__(self, inp: pio.LBMInput, ind: np.ndarray, S=None, cube=None):
+        self.lbm = OracleLBM(inp, ind, S)
+        self.L = self.lbm.L
+        self.inp = inp
+        nz, ny, nx = ind.shape
+        alf, bet = coupled_springs(inp.Em, inp.nu, inp.dl)
+        p = LSMParams()
+        p.nx, p.ny, p.nz, p.pad_axis, p.jump_comp, p.nthread = nx, ny, nz, -1, 0, 1
+        p.alf1 = p.alf2 = alf
+        p.bet1 = p.bet2 = bet
+        p.rs, p.dt, p.gstrain = float(inp.rs), float(inp.dt), 0.0
+        self.p = p
+        self._raw = np.ascontiguousarray(ind, np.int32)
+        self._ind = np.ascontiguousarray(np.where(ind == 0, 0, 1), np.int32)
+        cb, ca, cs, cab = cube if cube is not None else geo.cube_tables()
+        self._cube = [np.ascontiguousarray(x) for x in (cb, ca, cs, cab)]
+        self.h = self.L.or_lsm_create(C.byref(p), self._ind, *self._cube)
+        self.ns = self.L.or_lsm_count(self.h, 0)
+        assert self.ns == self.lbm.counts()["ns"]
+        self.af = np.zeros((self.ns, 3))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.or_lsm_destroy(self.h)
+            self.h = None
+
+    def step(self, n=1):
+        """n main-loop steps: LBM step, fluid force, lsm_step."""
+        rsi = 1.0 / float(self.inp.rs)
+        for _ in range(int(n)):
+            self.lbm.step(1)
+            self.L.or_lbm_fluid_force(self.lbm.h, rsi, self.af)
+            self.L.or_lsm_coupled_step(self.h, self.af, 0.25)
+
+    def state(self):
+        """r, v, a, af as (ns, 3) in the L1 solid numbering."""
+        r, v, a = (np.zeros((self.ns, 3)) for _ in range(3))
+        self.L.or_lsm_get_state(self.h, _ptr(r), _ptr(v), _ptr(a))
+        return r, v, a, self.af.copy()
+
+    def set_state(self, r, v, a):
+        self._st = [np.ascontiguousarray(x, np.float64) for x in (r, v, a)]
+        self.L.or_lsm_set_state(self.h, *(_ptr(x) for x in self._st))
+
+    def mijks(self):
+        nz, ny, nx = self._ind.shape
+        icors = np.zeros((self.ns, 3), np.int32)
+        mijks = np.zeros((nz, ny, nx), np.int32)
+        self.L.or_lsm_get_nodes(self.h, _ptr(icors), _ptr(mijks))
+        return mijks
+
+    def calc_ab(self, pres=None):
+        """calc_ab: dict of asigmap, asigmam, astrain (3,3) Fortran order
+        (row = force component), epsi, alphai, ncub, betai (npores,)."""
+        inp = self.inp
+        if pres is None:
+            pres = (1.0 - 0.99) * inp.rr0 * inp.cr2  # L1:3987
+        out = np.zeros(38 + int(inp.npores))
+        self.L.or_lsm_calc_ab(self.h, self._raw, int(inp.iipore), int(inp.npores), float(pres), out)
+        t = lambda a: a.reshape(3, 3).T  # column-major -> [row][col]
+        return {"asigmap": t(out[0:9]), "asigmam": t(out[9:18]), "astrain": t(out[18:27]), "epsi": out[27],
+                "alphai": t(out[28:37]), "ncub": int(out[37]), "betai": out[38:], "pres": pres}
+
+
+def read_lsm_files(d: str, it1: int, shape) -> dict:
+    """The solid-lattice files of save_results (L1:2176-2207) as arrays over
+    the voxel grid (nz, ny, nx, 3): disp (LSM_disf<it1>), vel, force
+    (adat/rsi), fluid_force (afdat/rsi); zero rows where no solid node."""
+    out = {}
+    for key, name in (("disp", f"LSM_disf{it1}"), ("vel", "LSM_velocity_field"), ("force", "LSM_force_field"),
+                      ("fluid_force", "LSM_fluid_force")):
+        with open(os.path.join(d, name)) as fh:
+            out[key] = pio.parse_fortran_numbers(fh.read()).reshape(tuple(shape) + (3,))
+    return out
+
+
+def read_alpha_beta(path: str, npores: int) -> dict:
+    """Last block appended to alpha_beta by calc_ab (L1:4447-4466)."""
+    with open(path) as fh:
+        lines = fh.read().splitlines()
+    start = max(i for i, l in enumerate(lines) if l.strip().startswith("Elastic medium properties"))
+    block = lines[start:]
+
+    def after(label, count):
+        i = next(k for k, l in enumerate(block) if l.strip().startswith(label))
+        rest = block[i].split("=", 1)[1]
+        vals = list(pio.parse_fortran_numbers(rest)) if rest.strip() else []
+        k = i + 1
+        while len(vals) < count:
+            vals += list(pio.parse_fortran_numbers(block[k]))
+            k += 1
+        return np.array(vals[:count])
+
+    t = lambda a: a.reshape(3, 3).T
+    return {"asigmap": t(after("asigmap", 9)), "asigmam": t(after("asigmam", 9)),
+            "astrain": t(after("astrain", 9)), "epsi": after("epsi", 1)[0], "alphai": t(after("alphai", 9)),
+            "betai": after("betai", npores), "ncub": int(after("Solid part volume", 1)[0]),
+            "pres": after("pres", 1)[0]}

@@ -180,6 +180,7 @@ def test_fp32_compressed_pull_table_is_exact(monkeypatch):
                        bfr=[1e-5, 0, 0], bfb=[1e-5, 0, 0])
     state = ob.equilibrium_state(inp, ob.fluid_order_mask(ind, geo.phase_blobs(ind, 0.5, 5, radius=2.5)))
     out = {}
+    monkeypatch.setenv("PSX_LBM_FUSED", "0")  # the int16 table serves the two-kernel step
     for flag in ("0", "1"):
         monkeypatch.setenv("PSX_LBM_IDX16", flag)
         _say(f"case fp32 pull table int16={flag}")
@@ -188,6 +189,32 @@ def test_fp32_compressed_pull_table_is_exact(monkeypatch):
             lb.step(40)
             out[flag] = lb.get_state()
     assert np.array_equal(out["0"][0], out["1"][0]) and np.array_equal(out["0"][1], out["1"][1])
+
+
+@pytest.mark.parametrize("brick", ["8,8,16", "4,4,4", "3,5,7", "16,16,16"])
+def test_fp32_fused_brick_step_is_exact(monkeypatch, brick):
+    """The fused step (colour field of each brick and its halo in LDS, then
+    collide) gives the same bits as the collide + density kernel pair, for
+    bricks that do and do not divide the grid (and one larger than it)."""
+    ind = geo.sphere_pack((14, 12, 18), 0.4, 3.0, seed=23)
+    nz, ny, nx = ind.shape
+    inp = pio.LBMInput(B=0.7, sig=0.02, vir=0.1, vib=0.2, wet=0.3, ncx=nx, ncy=ny, ncz=nz, dt=1.0,
+                       bfr=[0, 1e-5, 2e-6], bfb=[1e-6, 1e-5, 0])
+    state = ob.equilibrium_state(inp, ob.fluid_order_mask(ind, geo.phase_blobs(ind, 0.5, 6, radius=2.5)))
+    out = {}
+    monkeypatch.setenv("PSX_LBM_BRICK", brick)
+    for flag in ("0", "1"):
+        monkeypatch.setenv("PSX_LBM_FUSED", flag)
+        _say(f"case fp32 fused={flag} brick={brick}")
+        with LBM(inp, ind, precision="f32") as lb:
+            lb.set_state(*state)
+            lb.step(30)
+            out[flag] = (lb.get_state(), lb.check_state(1, 30).outm, lb.save_fields())
+    (a, da, fa), (b, db, fb_) = out["0"], out["1"]
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    np.testing.assert_allclose(da, db, rtol=1e-12, atol=1e-18, equal_nan=True)
+    for k in ("velocity", "color", "pressure", "phase"):
+        assert np.array_equal(fa[k], fb_[k]), k
 
 
 def test_fp64_multipore_selects_iipore_bitexact():

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import oracle_bind as ob
-from golden.make_golden import LSM_INP, lbm_cases, lsm_geometry
+from golden.make_golden import LSM_INP, coupled_case, lbm_cases, lsm_geometry
 from porescale_amd import geometry as geo
 from porescale_amd import io as pio
 from porescale_amd.lsm import spring_constants
@@ -190,3 +190,97 @@ def test_save_fields_files_round_trip(tmp_path):
     nz, ny, nx = ind.shape
     blocks = pio.read_ind_red_blue(str(tmp_path / "ind_red_blue"), nx, ny, nz)
     assert [b[0] for b in blocks] == [2 * inp.itw, 3 * inp.itw]
+
+
+def _coupled_compare(o, inp, lsmf, ab):
+    r, v, a, af = o.state()
+    m = o.mijks()
+    sol = m > 0
+    idx = m[sol] - 1
+    rsi = 1.0 / inp.rs
+    for key, arr in (("disp", r), ("vel", v), ("force", a / rsi), ("fluid_force", af / rsi)):
+        assert np.array_equal(lsmf[key][sol], arr[idx]), key
+        assert (lsmf[key][~sol] == 0).all(), key
+    cab = o.calc_ab()
+    for k in ("asigmap", "asigmam", "astrain"):
+        assert np.array_equal(cab[k] / cab["ncub"], ab[k]), k
+    for k in ("alphai", "betai", "epsi", "ncub", "pres"):
+        assert np.array_equal(np.asarray(cab[k]), np.asarray(ab[k])), k
+    return cab
+
+
+def test_coupled_solid_bitexact_vs_reference_files():
+    """L1's coupled solid (fluid force, lsm_step, calc_ab) after 2 pre-steps
+    and 12 coupled steps: every LSM_* file and the alpha_beta block."""
+    ind, inp = coupled_case()
+    g = _gold("lbm_coupled.npz")
+    o = ob.OracleCoupled(inp, ind)
+    o.lbm.init_random(inp.sat)
+    o.lbm.step(2)
+    o.step(inp.itm * inp.itw)
+    fr, fb = o.lbm.state()
+    assert np.array_equal(fr, g["fr"]) and np.array_equal(fb, g["fb"])
+    lsmf = {k: g[f"lsm_{k}"] for k in ("disp", "vel", "force", "fluid_force")}
+    ab = {k: g[f"ab_{k}"] for k in ("asigmap", "asigmam", "astrain", "alphai", "betai", "epsi", "ncub", "pres")}
+    cab = _coupled_compare(o, inp, lsmf, ab)
+    # sigmam is sigmap with every contribution negated; alpha = -eps I + sigmap/(V pres)
+    assert np.array_equal(cab["asigmam"], -cab["asigmap"])
+    V = float(np.prod(ind.shape))
+    np.testing.assert_allclose(cab["alphai"], -np.eye(3) * cab["epsi"] + cab["asigmap"] / V / cab["pres"],
+                               rtol=1e-15, atol=0)
+    assert cab["epsi"] == (ind == inp.iipore).sum() / V
+
+
+def test_coupled_solid_static_closed_forms():
+    """No flow: the fluid force of a rest state is the pressure term only,
+    -2 (f_eq - ppp fq) c rsi per link, whose sum over the closed solid
+    surface of a sphere-free periodic slab vanishes; the solid stays at
+    rest to rounding and calc_ab of a zero displacement is -eps I, beta 0."""
+    ind = np.ones((8, 6, 10), np.int32)
+    ind[:, :, 3:6] = 0  # a solid slab normal to x, periodic in y and z
+    nz, ny, nx = ind.shape
+    inp = pio.LBMInput(ncx=nx, ncy=ny, ncz=nz, bfr=[0, 0, 0], bfb=[0, 0, 0], Em=0.2, nu=0.2, rs=2.0, sat=0.0)
+    o = ob.OracleCoupled(inp, ind)
+    o.lbm.init_random(0.0)
+    o.step(3)
+    r, v, a, af = o.state()
+    assert np.abs(af.sum(0)).max() < 1e-14
+    # both slab faces push outward by the same amount (pressure on a wall)
+    assert np.abs(af[:, 1:]).max() < 1e-15 and np.abs(af[:, 0]).max() > 0
+    cab = o.calc_ab()
+    o.set_state(np.zeros_like(r), np.zeros_like(v), np.zeros_like(a))
+    z = o.calc_ab()
+    V = float(ind.size)
+    # an undeformed bond leaves only the rounding of (rn.xn) xn - rn
+    np.testing.assert_allclose(z["alphai"], -np.eye(3) * z["epsi"], rtol=0, atol=1e-12)
+    assert (z["betai"] == 0).all() and z["epsi"] == (ind == 1).sum() / V
+    assert cab["ncub"] == (ind == 0).sum()
+
+
+@pytest.mark.ref
+@pytest.mark.skipif(not ob.ref_available("l1_ref"), reason="oracle/_ref not built (needs /root/reference)")
+def test_coupled_two_pore_families_vs_reference_binary():
+    """Two pore families (npores 2, iipore 1): calc_ab's betai per family
+    from the reference's own displacement file.  The flow itself is not
+    compared: with a second family the reference reads cf(0) in calcFs and
+    writes afdat(1:3,0) out of bounds, where the oracle uses wet and skips
+    the write."""
+    ind = geo.sphere_pack((10, 12, 11), 0.5, 2.5, seed=19)
+    fam = geo.phase_blobs(ind, 0.4, seed=9, radius=2.0)
+    ind2 = np.where(fam, 2, ind).astype(np.int32)
+    nz, ny, nx = ind.shape
+    inp = pio.LBMInput(B=0.6, sig=0.03, vir=0.12, vib=0.12, wet=-0.2, idir=2, itm=1, itw=9, sat=0.4,
+                       npores=2, ncx=nx, ncy=ny, ncz=nz, dt=1.0, bfr=[0, 3e-4, 0], bfb=[0, 3e-4, 0],
+                       Em=0.3, nu=0.15, rs=1.5, dl=1.0)
+    fr, fb, outm, it1, lsmf, ab = ob.run_ref_coupled(inp, ind2)
+    o = ob.OracleCoupled(inp, ind2)
+    m = o.mijks()
+    r = np.zeros((o.ns, 3))
+    r[m[m > 0] - 1] = lsmf["disp"][m > 0]
+    o.set_state(r, np.zeros_like(r), np.zeros_like(r))
+    cab = o.calc_ab()
+    for k in ("alphai", "betai", "epsi", "ncub", "pres"):
+        assert np.array_equal(np.asarray(cab[k]), np.asarray(ab[k])), k
+    for k in ("asigmap", "astrain"):
+        assert np.array_equal(cab[k] / cab["ncub"], ab[k]), k
+    assert cab["betai"].shape == (2,) and (cab["betai"] != 0).all()
