@@ -189,6 +189,42 @@ int psx_lsm_batch_moduli(const psx_lsm_params *p, int32_t nvol, const int32_t *d
                          const int32_t *ind_concat, double rtol, int32_t maxit,
                          double *C_out, double *KG_out, double *ms_out);
 
+/* ============================================================ coupled ===== */
+/* The one-way coupled model of L1: the solid lattice of the LBM's own ind
+   (nodes at the corners of ind == 0 voxels, periodic in x, y and z; every
+   other code is pore for the springs, L1:501-510) moves under the momentum
+   the fluid's bounce-back links deliver each main-loop step (L1:3849-3879),
+   advanced by lsm_step (velocity Verlet, damping 0.25, L1:1721-1903).
+   Solid vectors are (3,ns) in L1's solid numbering (mijks: z outer, x
+   fastest). */
+typedef struct psx_cpl psx_cpl;
+
+/* Em, nu, rs, dl: INPUT_PARAMETERS lines 26, 27, 28, 30 (alf, bet as
+   L1:497-498); cube_*: the cube_bon/cube_ang/cube_sbond/cube_abond tables
+   that calc_ab reads (L1:4101-4113).  The solid starts at rest (L1:993-995).
+   The LBM handle must outlive the coupling. */
+int psx_cpl_create(psx_lbm *lbm, double Em, double nu, double dl, double rs, const int32_t *cube_bon,
+                   const int32_t *cube_ang, const double *cube_sbond, const double *cube_abond,
+                   psx_cpl **out);
+void psx_cpl_destroy(psx_cpl *c);
+/* counts[0] = ns solid nodes, [1] = edge links (npb2), [2] = all bounce-back
+   links, [3] = node-grid size */
+int psx_cpl_counts(psx_cpl *c, int64_t *counts);
+/* nsteps main-loop steps (L1:3634-3947): an LBM step, the fluid force of its
+   bounce-back links (afdat, L1:3849-3879), then lsm_step (L1:3884).  The two
+   pre-steps of L1 (3454-3621) are plain psx_lbm_step calls. */
+int psx_cpl_step(psx_cpl *c, int64_t nsteps);
+/* rdat, vdat, adat, afdat (L1 names; adat is the last lsm_step's
+   acceleration including damping); any pointer may be NULL. */
+int psx_cpl_get_state(psx_cpl *c, double *r, double *v, double *a, double *af);
+int psx_cpl_set_state(psx_cpl *c, const double *r, const double *v, const double *a);
+/* calc_ab (L1:4008-4468) of the current displacement, pres <= 0 meaning
+   (1 - 0.99) rr0 cr2 (L1:3987).  out[38 + npores]: [0..8] asigmap, [9..17]
+   asigmam, [18..26] astrain (3x3 column-major sums over the solid cubes;
+   alpha_beta prints them divided by ncub), [27] epsi, [28..36] alphai,
+   [37] ncub, [38..] betai of pore families 1..npores. */
+int psx_cpl_calc_ab(psx_cpl *c, int32_t npores, double pres, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,6 +50,7 @@
 
 #include "lbm_device.h"
 #include "lbm_fast.h"
+#include "lsm_cpl.h"
 
 using namespace psx_lbm_dev;
 
@@ -843,4 +844,409 @@ extern "C" int psx_lbm_check_state(psx_lbm *h, int64_t it1, int64_t itw, psx_lbm
     d->outm[10] = d->nr / nl;
     d->outm[11] = d->nb / nl;
     return PSX_OK;
+}
+
+// ============================================================ coupled ====
+// L1's one-way coupled model: the solid lattice of the LBM's own ind (nodes
+// at the corners of solid voxels, periodic in x, y and z; alf, bet from
+// Em, nu, dl as L1:497-498) moves under the momentum the bounce-back links
+// deliver each main-loop step (L1:3849-3879) by lsm_step (L1:1721-1903).
+// The solid lives on an LSM handle without imposed strain; its state is
+// [3][N] on the periodic node grid, mapped to L1's solid numbering (nodes
+// z outer / x fastest, the order of the grid) on the host.
+struct psx_cpl {
+    psx_lbm *lbm = nullptr;
+    psx_lsm *lsm = nullptr;
+    int64_t ns = 0, N = 0;
+    double alf = 0, bet = 0, rs = 1, dl = 1;
+    std::vector<int32_t> grid;   // [ns] node-grid index of L1 solid node s
+    std::vector<int32_t> mijks;  // L1 solid ids on the node grid (0 = none)
+    CplLinks L{};
+    int32_t *d_start = nullptr, *d_contrib = nullptr, *d_lnode = nullptr, *d_grid = nullptr;
+    int8_t *d_lq = nullptr;
+    double *d_af = nullptr;
+    int32_t cbon[24][3], cang[72][6];
+    double csb[24], cab[72];
+};
+
+namespace {
+
+struct FOutF64 {
+    const double *p;
+    int64_t nl;
+    __device__ __forceinline__ double operator()(int64_t n, int qb) const {
+        return p[(int64_t)qb * nl + n] + p[(int64_t)(Q + qb) * nl + n];
+    }
+};
+
+// The 48 angle types of the D3Q19 lattice (pairs of arm directions, 1-based
+// D3Q19 indices) and their plane normals and arm force directions
+// anf(9,48): n, fj = n x xj, fk = -(n x xk), x = c/|c|^2, with the
+// reference's constants 1/2, 1/sqrt(3), 1/(2 sqrt(3)) (XX:200-362).
+void angle_table(int32_t nta[48][2], double anf[48][9]) {
+    static const int r8[3][8] = {{2, 8, 4, 9, 3, 11, 5, 10}, {2, 12, 6, 13, 3, 15, 7, 14},
+                                 {4, 16, 6, 17, 5, 19, 7, 18}};
+    static const int r4[6][4] = {{12, 16, 13, 17}, {14, 18, 15, 19}, {8, 12, 10, 14},
+                                 {9, 13, 11, 15},  {8, 16, 9, 18},   {10, 17, 11, 19}};
+    int t = 0;
+    for (auto &ring : r8)
+        for (int m = 0; m < 8; m++, t++) { nta[t][0] = ring[m]; nta[t][1] = ring[(m + 1) % 8]; }
+    for (auto &ring : r4)
+        for (int m = 0; m < 4; m++, t++) { nta[t][0] = ring[m]; nta[t][1] = ring[(m + 1) % 4]; }
+    const double s3 = 1.0 / std::sqrt(3.0), s23 = 1.0 / (2.0 * std::sqrt(3.0));
+    for (t = 0; t < 48; t++) {
+        const int *cj = H_IV[nta[t][0] - 1], *ck = H_IV[nta[t][1] - 1];
+        const int sg = (t >= 8 && t < 16) ? -1 : 1;  // the reference's orientation of the xz ring
+        const int n[3] = {sg * (cj[1] * ck[2] - cj[2] * ck[1]), sg * (cj[2] * ck[0] - cj[0] * ck[2]),
+                          sg * (cj[0] * ck[1] - cj[1] * ck[0])};
+        const int n2 = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+        const int gj[3] = {n[1] * cj[2] - n[2] * cj[1], n[2] * cj[0] - n[0] * cj[2], n[0] * cj[1] - n[1] * cj[0]};
+        const int gk[3] = {n[2] * ck[1] - n[1] * ck[2], n[0] * ck[2] - n[2] * ck[0], n[1] * ck[0] - n[0] * ck[1]};
+        const int cj2 = cj[0] * cj[0] + cj[1] * cj[1] + cj[2] * cj[2];
+        const int ck2 = ck[0] * ck[0] + ck[1] * ck[1] + ck[2] * ck[2];
+        auto comp = [&](int g, int c2) -> double {  // g / (|n| |c|^2)
+            if (g == 0) return 0.0;
+            if (n2 == 1) return c2 == 1 ? (double)g : 0.5 * (double)g;
+            return (g > 0 ? 1.0 : -1.0) * (std::abs(g) == 2 ? s3 : s23);
+        };
+        for (int d = 0; d < 3; d++) {
+            anf[t][d] = n[d] == 0 ? 0.0 : (n2 == 1 ? (double)n[d] : (n[d] > 0 ? s3 : -s3));
+            anf[t][3 + d] = comp(gj[d], cj2);
+            anf[t][6 + d] = comp(gk[d], ck2);
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" int psx_cpl_create(psx_lbm *lbm, double Em, double nu, double dl, double rs, const int32_t *cube_bon,
+                              const int32_t *cube_ang, const double *cube_sbond, const double *cube_abond,
+                              psx_cpl **out) {
+    PSX_CHECK_ARG(lbm && out && cube_bon && cube_ang && cube_sbond && cube_abond,
+                  "psx_cpl_create: NULL argument (the cube tables are required)");
+    PSX_CHECK_ARG(rs > 0.0 && dl > 0.0, "psx_cpl_create: rs and dl must be > 0");
+    PSX_CHECK_ARG(5.0 - 10.0 * nu != 0.0 && 2 * nu * nu + nu - 1 != 0.0, "psx_cpl_create: nu gives no springs");
+    *out = nullptr;
+    psx_cpl *c = nullptr;
+    PSX_GUARD_BEGIN
+    c = new psx_cpl();
+    c->lbm = lbm;
+    c->rs = rs;
+    c->dl = dl;
+    c->alf = dl * Em / (5.0 - 10.0 * nu);                                      // L1:497
+    c->bet = dl * dl * dl * Em * (4 * nu - 1) / (20.0 * (2 * nu * nu + nu - 1));  // L1:498
+    std::memcpy(c->cbon, cube_bon, sizeof c->cbon);
+    std::memcpy(c->cang, cube_ang, sizeof c->cang);
+    std::memcpy(c->csb, cube_sbond, sizeof c->csb);
+    std::memcpy(c->cab, cube_abond, sizeof c->cab);
+    const int nx = lbm->nx, ny = lbm->ny, nz = lbm->nz;
+    const int64_t nv = (int64_t)nx * ny * nz;
+    // the solid lattice: voxels with ind == 0 are solid, every other code is
+    // pore for the springs (L1:501-510)
+    std::vector<int32_t> ind(nv);
+    for (int64_t a = 0; a < nv; a++) ind[a] = lbm->ind_raw[a] == 0 ? 0 : 1;
+    psx_lsm_params lp{};
+    lp.alf1 = lp.alf2 = c->alf;
+    lp.bet1 = lp.bet2 = c->bet;
+    lp.rs = rs;
+    lp.dt = lbm->p.dt;
+    lp.gstrain = 0.0;
+    lp.precision = PSX_PREC_F64;
+    int rc = psx_lsm_create(&lp, nx, ny, nz, ind.data(), nullptr, cube_bon, cube_ang, cube_sbond, cube_abond, &c->lsm);
+    if (rc) { delete c; return rc; }
+    c->N = psx_lsm_cpl::grid_nodes(c->lsm);
+    coupling_nodes(lbm, c->mijks);
+    for (int64_t g = 0; g < nv; g++)
+        if (c->mijks[g] > 0) c->grid.push_back((int32_t)g);
+    c->ns = (int64_t)c->grid.size();
+    // links: ipb2 rows, then ipb4 rows (the reference's loop order)
+    const int64_t n2 = lbm->npb2, n4 = lbm->npb4;
+    std::vector<int32_t> ipb2(4 * (size_t)n2), ipb4(6 * (size_t)n4);
+    rc = psx_lbm_tables(lbm, nullptr, nullptr, ipb2.data(), ipb4.data());
+    if (rc) { psx_cpl_destroy(c); return rc; }
+    std::vector<int32_t> inv;
+    if (!lbm->perm.empty()) {
+        inv.resize(lbm->perm.size());
+        for (size_t i = 0; i < lbm->perm.size(); i++) inv[(size_t)lbm->perm[i]] = (int32_t)i;
+    }
+    const int64_t nlink = n2 + n4;
+    std::vector<int32_t> lnode((size_t)std::max<int64_t>(nlink, 1)), cnt((size_t)c->ns + 1, 0);
+    std::vector<int8_t> lq((size_t)std::max<int64_t>(nlink, 1));
+    auto row = [&](int64_t li) -> const int32_t * { return li < n2 ? &ipb2[4 * li] : &ipb4[6 * (li - n2)]; };
+    for (int64_t li = 0; li < nlink; li++) {
+        const int32_t *e = row(li);
+        const int64_t n = e[0] - 1;
+        lnode[(size_t)li] = (int32_t)(inv.empty() ? n : inv[(size_t)n]);
+        lq[(size_t)li] = (int8_t)(e[1] - 1);
+        for (int m = 0; m < (li < n2 ? 2 : 4); m++)
+            if (e[2 + m] > 0) cnt[(size_t)e[2 + m]]++;  // id 0: a wall voxel of another pore family
+    }
+    for (int64_t s = 0; s < c->ns; s++) cnt[(size_t)s + 1] += cnt[(size_t)s];
+    std::vector<int32_t> contrib((size_t)std::max<int32_t>(cnt[(size_t)c->ns], 1)), fill(cnt.begin(), cnt.end() - 1);
+    for (int64_t li = 0; li < nlink; li++) {
+        const int32_t *e = row(li);
+        for (int m = 0; m < (li < n2 ? 2 : 4); m++)
+            if (e[2 + m] > 0) contrib[(size_t)fill[(size_t)e[2 + m] - 1]++] = (int32_t)li;
+    }
+    PSX_HIP(hipMalloc(&c->d_start, sizeof(int32_t) * cnt.size()));
+    PSX_HIP(hipMalloc(&c->d_contrib, sizeof(int32_t) * contrib.size()));
+    PSX_HIP(hipMalloc(&c->d_lnode, sizeof(int32_t) * lnode.size()));
+    PSX_HIP(hipMalloc(&c->d_lq, lq.size()));
+    PSX_HIP(hipMalloc(&c->d_grid, sizeof(int32_t) * std::max<size_t>(1, c->grid.size())));
+    PSX_HIP(hipMalloc(&c->d_af, sizeof(double) * 3 * (size_t)c->N));
+    hipStream_t st = lbm->stream;
+    PSX_HIP(hipMemcpyAsync(c->d_start, cnt.data(), sizeof(int32_t) * cnt.size(), hipMemcpyHostToDevice, st));
+    PSX_HIP(hipMemcpyAsync(c->d_contrib, contrib.data(), sizeof(int32_t) * contrib.size(), hipMemcpyHostToDevice, st));
+    PSX_HIP(hipMemcpyAsync(c->d_lnode, lnode.data(), sizeof(int32_t) * lnode.size(), hipMemcpyHostToDevice, st));
+    PSX_HIP(hipMemcpyAsync(c->d_lq, lq.data(), lq.size(), hipMemcpyHostToDevice, st));
+    if (c->ns > 0)
+        PSX_HIP(hipMemcpyAsync(c->d_grid, c->grid.data(), sizeof(int32_t) * c->grid.size(), hipMemcpyHostToDevice, st));
+    PSX_HIP(hipMemsetAsync(c->d_af, 0, sizeof(double) * 3 * (size_t)c->N, st));
+    PSX_HIP(hipStreamSynchronize(st));
+    CplLinks &L = c->L;
+    L.start = c->d_start; L.contrib = c->d_contrib; L.lnode = c->d_lnode; L.lq = c->d_lq; L.grid = c->d_grid;
+    L.ns = (int32_t)c->ns;
+    L.nedge = (int32_t)n2;
+    L.N = c->N;
+    const psx_lbm_params &p = lbm->p;
+    const double dt2 = p.dt * p.dt, dti = 1.0 / p.dt;
+    for (int q = 0; q < Q; q++) {
+        L.fq[q] = q == 0 ? p.rr0 * 0.0 : p.rr0 * (q < 7 ? p.cr2 * (1.0 / 6.0) * dt2 : p.cr2 * (1.0 / 12.0) * dt2);
+        for (int d = 0; d < 3; d++) L.cv[q][d] = (double)H_IV[q][d];
+    }
+    L.rsdti2 = (1.0 / rs) * dti * dti;
+    rc = psx_lsm_cpl::reset(c->lsm);
+    if (rc) { psx_cpl_destroy(c); return rc; }
+    *out = c;
+    return PSX_OK;
+    PSX_GUARD_END
+}
+
+extern "C" void psx_cpl_destroy(psx_cpl *c) {
+    if (!c) return;
+    if (c->lbm && c->lbm->stream) (void)hipStreamSynchronize(c->lbm->stream);
+    void *bufs[] = {c->d_start, c->d_contrib, c->d_lnode, c->d_lq, c->d_grid, c->d_af};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->lsm) psx_lsm_destroy(c->lsm);
+    delete c;
+}
+
+extern "C" int psx_cpl_counts(psx_cpl *c, int64_t *counts) {
+    PSX_CHECK_ARG(c && counts, "psx_cpl_counts: NULL argument");
+    counts[0] = c->ns;
+    counts[1] = c->L.nedge;
+    counts[2] = c->lbm->npb2 + c->lbm->npb4;
+    counts[3] = c->N;
+    return PSX_OK;
+}
+
+extern "C" int psx_cpl_step(psx_cpl *c, int64_t nsteps) {
+    PSX_CHECK_ARG(c && nsteps >= 0, "psx_cpl_step: bad argument");
+    psx_lbm *h = c->lbm;
+    for (int64_t s = 0; s < nsteps; s++) {
+        int rc = psx_lbm_step(h, 1);
+        if (rc) return rc;
+        if (c->ns > 0) {
+            if (h->f32) {
+                PSX_HIP(psx_lbm_fast::fluid_force(h->c, (const float *)h->d_pop[h->cur], c->L, c->d_af, h->stream));
+            } else {
+                FOutF64 f{(const double *)h->d_pop[h->cur], h->nl};
+                hipLaunchKernelGGL(k_fluid_force<FOutF64>, dim3((unsigned)psx::ceil_div(c->ns, 256)), dim3(256), 0,
+                                   h->stream, c->L, f, c->d_af);
+                PSX_HIP(hipGetLastError());
+            }
+        }
+        rc = psx_lsm_cpl::step(c->lsm, c->d_af, 0.25, h->stream);  // damping of L1:1888
+        if (rc) return rc;
+    }
+    return PSX_OK;
+}
+
+extern "C" int psx_cpl_get_state(psx_cpl *c, double *r, double *v, double *a, double *af) {
+    PSX_CHECK_ARG(c, "psx_cpl_get_state: NULL handle");
+    PSX_GUARD_BEGIN
+    double *dev[4];
+    psx_lsm_cpl::state(c->lsm, &dev[0], &dev[1], &dev[2]);
+    dev[3] = c->d_af;
+    double *outp[4] = {r, v, a, af};
+    std::vector<double> buf(3 * (size_t)c->N);
+    for (int k = 0; k < 4; k++) {
+        if (!outp[k]) continue;
+        PSX_HIP(hipMemcpyAsync(buf.data(), dev[k], sizeof(double) * buf.size(), hipMemcpyDeviceToHost, c->lbm->stream));
+        PSX_HIP(hipStreamSynchronize(c->lbm->stream));
+        for (int64_t s = 0; s < c->ns; s++)
+            for (int d = 0; d < 3; d++) outp[k][3 * s + d] = buf[(size_t)d * c->N + c->grid[(size_t)s]];
+    }
+    return PSX_OK;
+    PSX_GUARD_END
+}
+
+extern "C" int psx_cpl_set_state(psx_cpl *c, const double *r, const double *v, const double *a) {
+    PSX_CHECK_ARG(c && r && v && a, "psx_cpl_set_state: NULL argument");
+    PSX_GUARD_BEGIN
+    double *dev[3];
+    psx_lsm_cpl::state(c->lsm, &dev[0], &dev[1], &dev[2]);
+    const double *inp[3] = {r, v, a};
+    std::vector<double> buf(3 * (size_t)c->N);
+    for (int k = 0; k < 3; k++) {
+        std::fill(buf.begin(), buf.end(), 0.0);
+        for (int64_t s = 0; s < c->ns; s++)
+            for (int d = 0; d < 3; d++) buf[(size_t)d * c->N + c->grid[(size_t)s]] = inp[k][3 * s + d];
+        PSX_HIP(hipMemcpyAsync(dev[k], buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice, c->lbm->stream));
+        PSX_HIP(hipStreamSynchronize(c->lbm->stream));
+    }
+    return PSX_OK;
+    PSX_GUARD_END
+}
+
+extern "C" int psx_cpl_calc_ab(psx_cpl *c, int32_t npores, double pres, double *out) {
+    PSX_CHECK_ARG(c && out && npores >= 0, "psx_cpl_calc_ab: bad argument");
+    PSX_GUARD_BEGIN
+    psx_lbm *h = c->lbm;
+    const int nx = h->nx, ny = h->ny, nz = h->nz;
+    if (!(pres > 0.0)) pres = (1.0 - 0.99) * h->p.rr0 * h->p.cr2;  // L1:3987
+    std::vector<double> r(3 * (size_t)std::max<int64_t>(c->ns, 1));
+    int rc = psx_cpl_get_state(c, r.data(), nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    // lattice vectors (L1:996-1014) and the angle table
+    double vv[Q][3], vn[Q][3], vx[Q][3], vb[9][3], bn[9][3], anf[48][9];
+    int32_t nta[48][2];
+    static const int IB[9][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1},  {1, 1, 0}, {1, -1, 0},
+                                 {1, 0, 1}, {1, 0, -1}, {0, 1, 1}, {0, 1, -1}};
+    for (int q = 0; q < Q; q++) {
+        double s2 = 0.0;
+        for (int d = 0; d < 3; d++) { vv[q][d] = H_IV[q][d]; s2 += vv[q][d] * vv[q][d]; }
+        const double nr = std::sqrt(s2);
+        for (int d = 0; d < 3; d++) { vn[q][d] = q ? vv[q][d] / nr : 0.0; vx[q][d] = q ? vv[q][d] / s2 : 0.0; }
+    }
+    for (int b = 0; b < 9; b++) {
+        double s2 = 0.0;
+        for (int d = 0; d < 3; d++) { vb[b][d] = IB[b][d]; s2 += vb[b][d] * vb[b][d]; }
+        const double nr = std::sqrt(s2);
+        for (int d = 0; d < 3; d++) bn[b][d] = vb[b][d] / nr;
+    }
+    angle_table(nta, anf);
+    double sb[24], ab[72];
+    for (int l = 0; l < 24; l++) sb[l] = c->csb[l] * c->alf;
+    for (int l = 0; l < 72; l++) ab[l] = c->cab[l] * c->bet;
+    auto raw = [&](int i, int j, int k) { return h->ind_raw[(size_t)(i - 1) + (size_t)nx * ((j - 1) + (size_t)ny * (k - 1))]; };
+    auto node = [&](const int *p3) -> const double * {
+        const int32_t id = c->mijks[(size_t)(p3[0] - 1) + (size_t)nx * ((p3[1] - 1) + (size_t)ny * (p3[2] - 1))];
+        return &r[3 * (size_t)(id - 1)];
+    };
+    const double vcell = (double)((int32_t)(nx * ny * nz));
+    double epsi = 0.0;
+    for (int i = 1; i <= nx; i++)
+        for (int j = 1; j <= ny; j++)
+            for (int k = 1; k <= nz; k++)
+                if (raw(i, j, k) == h->p.iipore) epsi = epsi + 1.0;
+    epsi = epsi / vcell;
+    static const int icv[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+    double asp[3][3] = {{0}}, asm_[3][3] = {{0}}, ast[3][3] = {{0}};  // [column][row]
+    int64_t ncub = 0;
+    for (int i = 1; i <= nx; i++)  // calc_ab's cube loop (L1:4170-4340)
+        for (int j = 1; j <= ny; j++)
+            for (int k = 1; k <= nz; k++) {
+                if (raw(i, j, k) != 0) continue;
+                int icp[8][3];
+                for (int l = 0; l < 8; l++) {
+                    icp[l][0] = wrap1(i + icv[l][0], nx);
+                    icp[l][1] = wrap1(j + icv[l][1], ny);
+                    icp[l][2] = wrap1(k + icv[l][2], nz);
+                }
+                double sp[3][3] = {{0}}, sm[3][3] = {{0}}, stn[3][3] = {{0}};
+                for (int l = 0; l < 8; l++) {
+                    const double *up = node(icp[l]);
+                    const double sv[3] = {icv[l][0] ? 0.5 : -0.5, icv[l][1] ? 0.5 : -0.5, icv[l][2] ? 0.5 : -0.5};
+                    for (int m = 0; m < 3; m++)
+                        for (int n = 0; n < 3; n++) stn[n][m] = stn[n][m] + sv[m] * up[n] + sv[n] * up[m];
+                }
+                auto add = [&](int cdir, const double *f, double sgn) {
+                    for (int d = 0; d < 3; d++) {
+                        sp[cdir][d] = sp[cdir][d] + sgn * f[d];
+                        sm[cdir][d] = sm[cdir][d] - sgn * f[d];
+                    }
+                };
+                for (int l = 0; l < 24; l++) {
+                    const int32_t *e = c->cbon[l];
+                    const double *r1 = node(icp[e[0] - 1]), *r2 = node(icp[e[1] - 1]);
+                    const double *xn = bn[e[2] - 1], *rn = vb[e[2] - 1];
+                    double rt[3], fl[3];
+                    for (int d = 0; d < 3; d++) rt[d] = r2[d] - r1[d] + rn[d];
+                    const double dot = rt[0] * xn[0] + rt[1] * xn[1] + rt[2] * xn[2];
+                    for (int d = 0; d < 3; d++) fl[d] = sb[l] * (dot * xn[d] - rn[d]);
+                    for (int cd = 0; cd < 3; cd++) {
+                        if (xn[cd] > 0) add(cd, fl, 1.0);
+                        if (xn[cd] < 0) add(cd, fl, -1.0);
+                    }
+                }
+                for (int l = 0; l < 72; l++) {
+                    const int32_t *e = c->cang[l];
+                    const double *ri = node(icp[e[0] - 1]), *rj = node(icp[e[1] - 1]), *rk = node(icp[e[2] - 1]);
+                    const double *xnj = vn[e[3] - 1], *xrj = vx[e[3] - 1], *rnj = vv[e[3] - 1];
+                    const double *xnk = vn[e[4] - 1], *xrk = vx[e[4] - 1], *rnk = vv[e[4] - 1];
+                    const double *an = anf[e[5] - 1];
+                    double rtj[3], rtk[3];
+                    for (int d = 0; d < 3; d++) { rtj[d] = rj[d] - ri[d] + rnj[d]; rtk[d] = rk[d] - ri[d] + rnk[d]; }
+                    const double dfi = ((xrk[1] * rtk[2] - xrk[2] * rtk[1] - xrj[1] * rtj[2] + xrj[2] * rtj[1]) * an[0] -
+                                        (xrk[0] * rtk[2] - xrk[2] * rtk[0] - xrj[0] * rtj[2] + xrj[2] * rtj[0]) * an[1] +
+                                        (xrk[0] * rtk[1] - xrk[1] * rtk[0] - xrj[0] * rtj[1] + xrj[1] * rtj[0]) * an[2]) *
+                                       ab[l];
+                    const double fj[3] = {dfi * an[3], dfi * an[4], dfi * an[5]};
+                    const double fk[3] = {dfi * an[6], dfi * an[7], dfi * an[8]};
+                    for (int cd = 0; cd < 3; cd++) {
+                        if (xnj[cd] > 0) add(cd, fj, -1.0);
+                        if (xnj[cd] < 0) add(cd, fj, 1.0);
+                    }
+                    for (int cd = 0; cd < 3; cd++) {
+                        if (xnk[cd] > 0) add(cd, fk, -1.0);
+                        if (xnk[cd] < 0) add(cd, fk, 1.0);
+                    }
+                }
+                for (int a = 0; a < 3; a++)
+                    for (int d = 0; d < 3; d++) {
+                        asp[a][d] = asp[a][d] + sp[a][d];
+                        asm_[a][d] = asm_[a][d] + sm[a][d];
+                        ast[a][d] = ast[a][d] + stn[a][d] * 0.25;
+                    }
+                ncub++;
+            }
+    for (int a = 0; a < 3; a++)
+        for (int d = 0; d < 3; d++) {
+            out[d + 3 * a] = asp[a][d];
+            out[9 + d + 3 * a] = asm_[a][d];
+            out[18 + d + 3 * a] = ast[a][d];
+            out[28 + d + 3 * a] = -(a == d ? 1.0 : 0.0) * epsi + asp[a][d] / vcell / pres;  // L1:4378
+        }
+    out[27] = epsi;
+    out[37] = (double)ncub;
+    for (int iip = 1; iip <= npores; iip++) {  // betai (L1:4381-4445)
+        double b = 0.0;
+        for (int k = 1; k <= nz; k++)
+            for (int j = 1; j <= ny; j++)
+                for (int i = 1; i <= nx; i++) {
+                    if (raw(i, j, k) != iip) continue;
+                    for (int l = 1; l <= 6; l++) {
+                        const int *cl = H_IV[l];
+                        if (raw(wrap1(i + cl[0], nx), wrap1(j + cl[1], ny), wrap1(k + cl[2], nz)) != 0) continue;
+                        const double *pr[4];
+                        for (int m = 0; m < 4; m++) {
+                            const int o1 = m & 1, o2 = m >> 1;
+                            int off[3];
+                            if (cl[0] != 0) { off[0] = (cl[0] + 1) / 2; off[1] = o1; off[2] = o2; }
+                            else if (cl[1] != 0) { off[0] = o1; off[1] = (cl[1] + 1) / 2; off[2] = o2; }
+                            else { off[0] = o1; off[1] = o2; off[2] = (cl[2] + 1) / 2; }
+                            const int p3[3] = {wrap1(i + off[0], nx), wrap1(j + off[1], ny), wrap1(k + off[2], nz)};
+                            pr[m] = node(p3);
+                        }
+                        double up[3];
+                        for (int d = 0; d < 3; d++) up[d] = (pr[0][d] + pr[1][d] + pr[2][d] + pr[3][d]) * 0.25;
+                        b = b - up[0] * vv[l][0] - up[1] * vv[l][1] - up[2] * vv[l][2];
+                    }
+                }
+        out[38 + iip - 1] = b / vcell / pres;
+    }
+    return PSX_OK;
+    PSX_GUARD_END
 }

@@ -473,6 +473,49 @@ __global__ __launch_bounds__(256) void k_fields(const LbmConst c, const St in, c
     phase[i] = cf > 0.9 ? 1 : cf < -0.9 ? 2 : (cf >= 0.0 && cf <= 0.9) ? 6 : 7;
 }
 
+// LSM-LBM coupling force (L1:3849-3879) on the coupled model's solid nodes.
+// Every bounce-back link gives -2 (f_out - 0.99 fq) c_in rsi/dt^2 to the 2
+// (edge link) or 4 (face link) solid nodes of its wall, half or a quarter
+// each; a node sums its contributions in the reference's loop order (edge
+// links, then face links, each in table order), so the FP64 path reproduces
+// the reference's sums bit for bit.
+struct CplLinks {
+    const int32_t *start;    // [ns+1] contribution range of solid node s
+    const int32_t *contrib;  // link ids; ids < nedge are edge links
+    const int32_t *lnode;    // [nlink] fluid node (storage index)
+    const int8_t *lq;        // [nlink] direction of the returning population (0-based)
+    const int32_t *grid;     // [ns] node-grid index of solid node s
+    int32_t ns, nedge;
+    int64_t N;               // node-grid size; af is [3][N]
+    double fq[Q];            // rr0 (cr2 dt^2/6 | cr2 dt^2/12), L1:3417
+    double cv[Q][3];         // lattice velocities
+    double rsdti2;           // rsi/dt^2
+};
+
+// FOut(n, qb): the post-collision population that returns to node n in
+// direction qb, fr + fb.
+template <class FOut>
+__global__ __launch_bounds__(256) void k_fluid_force(const CplLinks L, const FOut fout, double *__restrict__ af) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= L.ns) return;
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int32_t k = L.start[s]; k < L.start[s + 1]; k++) {
+        const int32_t li = L.contrib[k];
+        const int64_t n = L.lnode[li];
+        const int qb = L.lq[li];
+        const double x = fout(n, qb) - 0.99 * L.fq[qb];
+        const double share = li < L.nedge ? 0.5 : 0.25;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double floc = -2.0 * x * L.cv[qb][d] * L.rsdti2;
+            a[d] = a[d] + floc * share;
+        }
+    }
+    const int64_t g = L.grid[s];
+#pragma unroll
+    for (int d = 0; d < 3; d++) af[d * L.N + g] = a[d];
+}
+
 // check_state partial sums; see NDIAG layout in diag_finish().
 constexpr int NDIAG = 36;
 constexpr int BLOCK = 256;

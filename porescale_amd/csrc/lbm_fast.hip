@@ -400,6 +400,27 @@ hipError_t step(const LbmConst &c, const float *in, float *out, const int32_t *s
     return hipGetLastError();
 }
 
+namespace {
+// post-collision population returning in direction qb: stored, unstreamed,
+// in slot (opp(qb), n)
+struct FOutF32 {
+    const float *p;
+    double w0[Q];
+    __device__ __forceinline__ double operator()(int64_t n, int qb) const {
+        const int q = opp(qb);
+        return w0[q] + (double)p[aos(q, n)];
+    }
+};
+}  // namespace
+
+hipError_t fluid_force(const LbmConst &c, const float *pop, const CplLinks &L, double *af, hipStream_t st) {
+    if (L.ns == 0) return hipSuccess;
+    FOutF32 f{pop, {}};
+    for (int q = 0; q < Q; q++) f.w0[q] = c.w0[q];
+    hipLaunchKernelGGL(k_fluid_force<FOutF32>, dim3((unsigned)psx::ceil_div(L.ns, 256)), dim3(256), 0, st, L, f, af);
+    return hipGetLastError();
+}
+
 hipError_t diag(const LbmConst &c, const float *in, const int32_t *src, const int32_t *vox, const double *phi,
                 double *partial, hipStream_t st) {
     int64_t nb = psx::ceil_div(c.nl, BLOCK);

@@ -43,6 +43,7 @@
 
 #include "psx_common.h"
 #include "lsm_common.h"
+#include "lsm_cpl.h"
 #include "lsm_element.h"
 #include "lsm_mg.h"
 
@@ -507,6 +508,29 @@ __global__ __launch_bounds__(TILE) void k_verlet_a(int64_t n3, double *__restric
     double v = V[i] + A[i] * dt * 0.5;  // XX:1388-1392
     V[i] = v;
     U[i] = U[i] + v * dt;               // XX:1398-1400
+}
+
+// lsm_step of the coupled model (L1:1758-1903): the fluid force AF enters
+// both half kicks; damping `damp` (0.25 there)
+__global__ __launch_bounds__(TILE) void k_cverlet_a(int64_t n3, double *__restrict__ U, double *__restrict__ V,
+                                                    const double *__restrict__ A, const double *__restrict__ AF,
+                                                    double dt) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    double v = V[i] + A[i] * dt * 0.5 + AF[i] * dt * 0.5;  // L1:1762-1766
+    V[i] = v;
+    U[i] = U[i] + v * dt;                                  // L1:1775-1777
+}
+
+__global__ __launch_bounds__(TILE) void k_cverlet_b(int64_t n3, const double *__restrict__ F, double *__restrict__ V,
+                                                    double *__restrict__ A, const double *__restrict__ AF,
+                                                    double rsi, double damp, double dt) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    double a = F[i] * rsi;       // L1:1880
+    a = a - damp * V[i];         // L1:1888
+    A[i] = a;
+    V[i] = V[i] + a * dt * 0.5 + AF[i] * dt * 0.5;  // L1:1897-1898
 }
 
 __global__ __launch_bounds__(TILE) void k_verlet_b(int64_t n3, const double *__restrict__ F, double *__restrict__ V,
@@ -1201,6 +1225,48 @@ int check_case(int lc) {
     return PSX_OK;
 }
 }  // namespace
+
+namespace psx_lsm_cpl {
+
+int64_t grid_nodes(const psx_lsm *h) { return h->B.Ntot; }
+
+int reset(psx_lsm *h) {
+    PSX_GUARD_BEGIN
+    RC(ensure_solver(h, 1));
+    const int64_t n3 = 3 * h->B.Ntot;
+    if (!h->d_vu) {
+        RC(dalloc(&h->d_vu, n3)); RC(dalloc(&h->d_vv, n3)); RC(dalloc(&h->d_va, n3)); RC(dalloc(&h->d_vf, n3));
+    }
+    int gb = (int)psx::ceil_div(n3, 256);
+    hipLaunchKernelGGL(k_zero<double>, dim3(gb), dim3(256), 0, h->stream, h->d_vu, n3);
+    hipLaunchKernelGGL(k_zero<double>, dim3(gb), dim3(256), 0, h->stream, h->d_vv, n3);
+    hipLaunchKernelGGL(k_zero<double>, dim3(gb), dim3(256), 0, h->stream, h->d_va, n3);
+    PSX_HIP(hipGetLastError());
+    PSX_HIP(hipStreamSynchronize(h->stream));
+    h->verlet_case = -2;  // the Verlet buffers hold the coupled state
+    return PSX_OK;
+    PSX_GUARD_END
+}
+
+int step(psx_lsm *h, const double *d_af, double damp, hipStream_t st) {
+    const int64_t n3 = 3 * h->B.Ntot;
+    const double dt = h->p.dt, rsi = 1.0 / h->p.rs;
+    const int gb = (int)psx::ceil_div(n3, 256);
+    hipLaunchKernelGGL(k_cverlet_a, dim3(gb), dim3(256), 0, st, n3, h->d_vu, h->d_vv, h->d_va, d_af, dt);
+    launch_apply<double, false, true, false>(st, h->B, 1, h->B.d_ka, h->B.d_kb, h->d_vu, h->d_vf, nullptr, nullptr,
+                                             nullptr);
+    hipLaunchKernelGGL(k_cverlet_b, dim3(gb), dim3(256), 0, st, n3, h->d_vf, h->d_vv, h->d_va, d_af, rsi, damp, dt);
+    PSX_HIP(hipGetLastError());
+    return PSX_OK;
+}
+
+void state(psx_lsm *h, double **r, double **v, double **a) {
+    *r = h->d_vu;
+    *v = h->d_vv;
+    *a = h->d_va;
+}
+
+}  // namespace psx_lsm_cpl
 
 extern "C" int psx_lsm_set_u(psx_lsm *h, int32_t lc, const double *u) {
     PSX_CHECK_ARG(h && u, "psx_lsm_set_u: NULL argument");

@@ -177,3 +177,64 @@ class LBM:
                 write_save_fields(save_dir, it1, itw, self.save_fields())
             rows.append(list(self.check_state(it1, itw).outm))
         return np.array(rows)
+
+
+class Coupled:
+    """The one-way coupled model of L1 on top of an :class:`LBM`: the solid
+    lattice of the same ``ind`` moves under the fluid's momentum exchange
+    each main-loop step (L1:3849-3888) and ``calc_ab`` gives Biot's alpha
+    tensor and beta per pore family (L1:4008-4468).  Em, nu, rs, dl come
+    from the LBM's INPUT_PARAMETERS (lines 26-30)."""
+
+    def __init__(self, lbm: LBM, cube=None):
+        from .geometry import cube_tables
+        L = lbm._L
+        inp = lbm.inp
+        bon, ang, sb, ab = cube if cube is not None else cube_tables()
+        self._cube = [np.ascontiguousarray(bon, np.int32), np.ascontiguousarray(ang, np.int32),
+                      np.ascontiguousarray(sb, np.float64), np.ascontiguousarray(ab, np.float64)]
+        h = C.c_void_p()
+        _lib.check(L.psx_cpl_create(lbm._h, float(inp.Em), float(inp.nu), float(inp.dl), float(inp.rs),
+                                    *(_lib.ptr(x) for x in self._cube), C.byref(h)))
+        self._h, self._L, self.lbm = h, L, lbm
+        c = np.zeros(4, np.int64)
+        _lib.check(L.psx_cpl_counts(h, _lib.ptr(c)))
+        self.ns = int(c[0])
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.psx_cpl_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def step(self, nsteps: int = 1):
+        """nsteps main-loop steps: LBM step, fluid force, lsm_step."""
+        _lib.check(self._L.psx_cpl_step(self._h, int(nsteps)))
+
+    def get_state(self):
+        """rdat, vdat, adat, afdat as (ns, 3) in L1's solid numbering."""
+        r, v, a, af = (np.zeros((self.ns, 3)) for _ in range(4))
+        _lib.check(self._L.psx_cpl_get_state(self._h, *(_lib.ptr(x) for x in (r, v, a, af))))
+        return r, v, a, af
+
+    def set_state(self, r, v, a):
+        arrs = [np.ascontiguousarray(x, np.float64).reshape(self.ns, 3) for x in (r, v, a)]
+        _lib.check(self._L.psx_cpl_set_state(self._h, *(_lib.ptr(x) for x in arrs)))
+
+    def calc_ab(self, npores=None, pres=0.0):
+        """asigmap, asigmam, astrain (3,3 sums over the solid cubes, row =
+        force component), epsi, alphai (3,3), ncub, betai (npores,)."""
+        npores = int(self.lbm.inp.npores if npores is None else npores)
+        out = np.zeros(38 + npores)
+        _lib.check(self._L.psx_cpl_calc_ab(self._h, npores, float(pres), _lib.ptr(out)))
+        t = lambda a: a.reshape(3, 3).T
+        return {"asigmap": t(out[0:9]), "asigmam": t(out[9:18]), "astrain": t(out[18:27]), "epsi": out[27],
+                "alphai": t(out[28:37]), "ncub": int(out[37]), "betai": out[38:]}
