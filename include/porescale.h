@@ -210,6 +210,10 @@ void psx_cpl_destroy(psx_cpl *c);
 /* counts[0] = ns solid nodes, [1] = edge links (npb2), [2] = all bounce-back
    links, [3] = node-grid size */
 int psx_cpl_counts(psx_cpl *c, int64_t *counts);
+/* icors(3,ns): 1-based coordinates of the solid nodes; mijks(ncx,ncy,ncz):
+   their L1 ids on the node grid, 0 where none (L1:625-645).  Either may be
+   NULL. */
+int psx_cpl_nodes(psx_cpl *c, int32_t *icors, int32_t *mijks);
 /* nsteps main-loop steps (L1:3634-3947): an LBM step, the fluid force of its
    bounce-back links (afdat, L1:3849-3879), then lsm_step (L1:3884).  The two
    pre-steps of L1 (3454-3621) are plain psx_lbm_step calls. */

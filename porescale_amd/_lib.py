@@ -70,7 +70,7 @@ EXPORTS = (
     "psx_lsm_create", "psx_lsm_destroy", "psx_lsm_counts", "psx_lsm_set_u", "psx_lsm_get_u",
     "psx_lsm_forces", "psx_lsm_verlet", "psx_lsm_solve", "psx_lsm_stress", "psx_lsm_moduli",
     "psx_lsm_adjoint", "psx_lsm_profile", "psx_lsm_batch_moduli",
-    "psx_cpl_create", "psx_cpl_destroy", "psx_cpl_counts", "psx_cpl_step", "psx_cpl_get_state",
+    "psx_cpl_create", "psx_cpl_destroy", "psx_cpl_counts", "psx_cpl_nodes", "psx_cpl_step", "psx_cpl_get_state",
     "psx_cpl_set_state", "psx_cpl_calc_ab",
 )
 
@@ -117,6 +117,7 @@ def _declare(L):
     L.psx_cpl_destroy.argtypes = [_vp]
     L.psx_cpl_destroy.restype = None
     L.psx_cpl_counts.argtypes = [_vp, _vp]
+    L.psx_cpl_nodes.argtypes = [_vp, _vp, _vp]
     L.psx_cpl_step.argtypes = [_vp, _i64]
     L.psx_cpl_get_state.argtypes = [_vp, _vp, _vp, _vp, _vp]
     L.psx_cpl_set_state.argtypes = [_vp, _vp, _vp, _vp]

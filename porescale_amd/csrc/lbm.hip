@@ -1041,6 +1041,21 @@ extern "C" int psx_cpl_counts(psx_cpl *c, int64_t *counts) {
     return PSX_OK;
 }
 
+extern "C" int psx_cpl_nodes(psx_cpl *c, int32_t *icors, int32_t *mijks) {
+    PSX_CHECK_ARG(c, "psx_cpl_nodes: NULL handle");
+    const int nx = c->lbm->nx, ny = c->lbm->ny;
+    for (int64_t s = 0; s < c->ns; s++) {
+        const int64_t g = c->grid[(size_t)s];
+        if (icors) {
+            icors[3 * s] = (int32_t)(g % nx) + 1;
+            icors[3 * s + 1] = (int32_t)((g / nx) % ny) + 1;
+            icors[3 * s + 2] = (int32_t)(g / ((int64_t)nx * ny)) + 1;
+        }
+    }
+    if (mijks) std::memcpy(mijks, c->mijks.data(), sizeof(int32_t) * c->mijks.size());
+    return PSX_OK;
+}
+
 extern "C" int psx_cpl_step(psx_cpl *c, int64_t nsteps) {
     PSX_CHECK_ARG(c && nsteps >= 0, "psx_cpl_step: bad argument");
     psx_lbm *h = c->lbm;

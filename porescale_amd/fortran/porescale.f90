@@ -236,6 +236,60 @@ module porescale
        real(c_double) :: dab(4)
        type(c_ptr), value :: dstiff
      end function psx_lsm_adjoint
+
+     ! coupled LBM -> LSM model (lsm_lbm_static_alpha_beta_multipore.f)
+     function psx_cpl_create(lbm, Em, nu, dl, rs, cube_bon, cube_ang, cube_sbond, cube_abond, h) &
+          bind(C, name='psx_cpl_create')
+       import :: c_int, c_int32_t, c_double, c_ptr
+       integer(c_int) :: psx_cpl_create
+       type(c_ptr), value :: lbm
+       real(c_double), value :: Em, nu, dl, rs
+       integer(c_int32_t) :: cube_bon(*), cube_ang(*)
+       real(c_double) :: cube_sbond(*), cube_abond(*)
+       type(c_ptr), intent(out) :: h
+     end function psx_cpl_create
+
+     subroutine psx_cpl_destroy(h) bind(C, name='psx_cpl_destroy')
+       import :: c_ptr
+       type(c_ptr), value :: h
+     end subroutine psx_cpl_destroy
+
+     function psx_cpl_counts(h, counts) bind(C, name='psx_cpl_counts')
+       import :: c_int, c_int64_t, c_ptr
+       integer(c_int) :: psx_cpl_counts
+       type(c_ptr), value :: h
+       integer(c_int64_t) :: counts(4)
+     end function psx_cpl_counts
+
+     function psx_cpl_nodes(h, icors, mijks) bind(C, name='psx_cpl_nodes')
+       import :: c_int, c_int32_t, c_ptr
+       integer(c_int) :: psx_cpl_nodes
+       type(c_ptr), value :: h
+       integer(c_int32_t) :: icors(*), mijks(*)
+     end function psx_cpl_nodes
+
+     function psx_cpl_step(h, nsteps) bind(C, name='psx_cpl_step')
+       import :: c_int, c_int64_t, c_ptr
+       integer(c_int) :: psx_cpl_step
+       type(c_ptr), value :: h
+       integer(c_int64_t), value :: nsteps
+     end function psx_cpl_step
+
+     function psx_cpl_get_state(h, r, v, a, af) bind(C, name='psx_cpl_get_state')
+       import :: c_int, c_double, c_ptr
+       integer(c_int) :: psx_cpl_get_state
+       type(c_ptr), value :: h
+       real(c_double) :: r(*), v(*), a(*), af(*)
+     end function psx_cpl_get_state
+
+     function psx_cpl_calc_ab(h, npores, pres, out) bind(C, name='psx_cpl_calc_ab')
+       import :: c_int, c_int32_t, c_double, c_ptr
+       integer(c_int) :: psx_cpl_calc_ab
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: npores
+       real(c_double), value :: pres
+       real(c_double) :: out(*)
+     end function psx_cpl_calc_ab
   end interface
 
 contains

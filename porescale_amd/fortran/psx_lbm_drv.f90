@@ -12,6 +12,13 @@
 ! a block appended to ind_red_blue.  settings_19 receives the node and table
 ! counts.
 !
+! When the cube_bon/cube_ang/cube_sbond/cube_abond files are present (the
+! reference's run directory has them), the driver runs the reference's
+! coupled model as it does: after the pre-steps every step also moves the
+! solid lattice by the fluid's momentum exchange and lsm_step (:3849-3888),
+! save_results adds the LSM_* files (:2176-2207) and every check appends
+! calc_ab's block to alpha_beta (:3987-3991, 4447-4466).
+!
 ! Optional lines after line 30 (ignored by the reference, which stops reading
 ! at line 30) select the GPU options:
 !   31  precision  0 = FP64 populations (bit-identical to the reference),
@@ -34,6 +41,10 @@ program psx_lbm_drv
   real(8), allocatable :: fr(:, :), fb(:, :)
   integer(c_int64_t) :: counts(5), nl
   real(8) :: Sr(19), Sb(19)
+  type(c_ptr) :: cpl
+  logical :: coupled
+  integer(c_int32_t), allocatable :: cbon(:), cang(:)
+  real(c_double), allocatable :: csb(:), cab(:)
 
   open(newunit=u, file='INPUT_PARAMETERS', status='old', iostat=ios)
   if (ios /= 0) then
@@ -170,12 +181,19 @@ program psx_lbm_drv
   write(u, *) 'precision = ', merge('FP32', 'FP64', prec == 1)
   close(u)
 
+  coupled = read_cube_tables()
+  if (coupled) call psx_check(psx_cpl_create(h, Em, nu, dl, rs, cbon, cang, csb, cab, cpl), 'psx_cpl_create')
+
   ! two pre-steps (lsm_lbm_static_alpha_beta_multipore.f:3454-3621)
   call psx_check(psx_lbm_step(h, 2_c_int64_t), 'psx_lbm_step')
   is = 0
   do it1 = itms, itm
      is = is + 1
-     call psx_check(psx_lbm_step(h, int(itw, c_int64_t)), 'psx_lbm_step')
+     if (coupled) then
+        call psx_check(psx_cpl_step(cpl, int(itw, c_int64_t)), 'psx_cpl_step')
+     else
+        call psx_check(psx_lbm_step(h, int(itw, c_int64_t)), 'psx_lbm_step')
+     end if
      call psx_check(psx_lbm_get_state(h, fr, fb), 'psx_lbm_get_state')
      open(newunit=u, file='iteration2', status='replace')
      write(u, *) it1
@@ -185,6 +203,7 @@ program psx_lbm_drv
      if (is == its) then
         is = 0
         call save_fields(it1)
+        if (coupled) call save_solid(it1)
      end if
      call psx_check(psx_lbm_check_state(h, int(it1, c_int64_t), int(itw, c_int64_t), d), &
           'psx_lbm_check_state')
@@ -192,14 +211,132 @@ program psx_lbm_drv
      open(newunit=u, file='output.m', position='append')
      write(u, 1984) it1 * itw, d%outm(2:12)
      close(u)
+     if (coupled) call alpha_beta()
   end do
 1984 format(1X, I9, 11(1X, E15.8), 1X, ';')
   open(newunit=u, file='output.m', position='append')
   write(u, *) '];'
   close(u)
+  if (coupled) call psx_cpl_destroy(cpl)
   call psx_lbm_destroy(h)
 
 contains
+
+  logical function read_cube_tables()
+    integer :: v, st
+    read_cube_tables = .false.
+    allocate(cbon(3 * 24), cang(6 * 72), csb(24), cab(72))
+    open(newunit=v, file='cube_bon', status='old', iostat=st)
+    if (st /= 0) return
+    read(v, *, iostat=st) cbon
+    close(v)
+    if (st /= 0) return
+    open(newunit=v, file='cube_ang', status='old', iostat=st)
+    if (st /= 0) return
+    read(v, *, iostat=st) cang
+    close(v)
+    if (st /= 0) return
+    open(newunit=v, file='cube_sbond', status='old', iostat=st)
+    if (st /= 0) return
+    read(v, *, iostat=st) csb
+    close(v)
+    if (st /= 0) return
+    open(newunit=v, file='cube_abond', status='old', iostat=st)
+    if (st /= 0) return
+    read(v, *, iostat=st) cab
+    close(v)
+    read_cube_tables = st == 0
+  end function read_cube_tables
+
+  ! calc_ab's block of alpha_beta (lsm_lbm_static_alpha_beta_multipore.f:
+  ! 4447-4466); the tensors are printed divided by the cube count.
+  subroutine alpha_beta()
+    real(c_double), allocatable :: o(:)
+    real(8) :: Cxxxx, Cyyxx, Cxyxy, vcell, rncub
+    integer :: uo
+    allocate(o(38 + npores))
+    call psx_check(psx_cpl_calc_ab(cpl, int(npores, c_int32_t), 0.d0, o), 'psx_cpl_calc_ab')
+    Cxxxx = Em * (nu - 1.d0) / (2.d0 * nu * nu + nu - 1.d0)
+    Cyyxx = Em * nu / ((1.d0 + nu) * (1.d0 - 2.d0 * nu))
+    Cxyxy = Em / (2.d0 * (1.d0 + nu))
+    vcell = ncx * ncy * ncz
+    rncub = o(38)
+    open(newunit=uo, file='alpha_beta', position='append')
+    write(uo, *) 'Elastic medium properties'
+    write(uo, *) 'Ypung modulus Em = ', Em
+    write(uo, *) 'Poisson ratio nu = ', nu
+    write(uo, *) 'Stiffness coefficients'
+    write(uo, *) 'C_xxxx = ', Cxxxx
+    write(uo, *) 'C_yyxx = ', Cyyxx
+    write(uo, *) 'C_xyxy = ', Cxyxy
+    write(uo, *) 'Simulation parameters'
+    write(uo, *) 'Cell volume vcell = ', vcell
+    write(uo, *) 'Solid part volume(elementary cubes) rncub = ', rncub
+    write(uo, *) 'Simulation results'
+    write(uo, *) 'averaged stress tensor'
+    write(uo, *) 'asigmap = '
+    write(uo, *) o(1:9) / rncub
+    write(uo, *) 'asigmam = '
+    write(uo, *) o(10:18) / rncub
+    write(uo, *) 'averaged strain'
+    write(uo, *) 'astrain = '
+    write(uo, *) o(19:27) / rncub
+    write(uo, *) 'epsi = ', o(28)
+    write(uo, *) 'alphai = '
+    write(uo, *) o(29:37)
+    write(uo, *) 'betai  = '
+    write(uo, *) o(39:38 + npores)
+    write(uo, *) 'pres = ', (1.d0 - 0.99d0) * rr0 * cr2
+    close(uo)
+  end subroutine alpha_beta
+
+  ! The solid-lattice files of save_results (:2176-2207).
+  subroutine save_solid(it)
+    integer, intent(in) :: it
+    real(c_double), allocatable :: r(:, :), v(:, :), a(:, :), af(:, :)
+    integer(c_int32_t), allocatable :: icors(:, :), mijks(:, :, :)
+    integer(c_int64_t) :: cc(4)
+    integer :: uu(6), ii, jj, kk, l
+    character(len=32) :: dname
+    real(8) :: rsi, z3(3)
+    call psx_check(psx_cpl_counts(cpl, cc), 'psx_cpl_counts')
+    allocate(r(3, max(cc(1), 1_c_int64_t)), v(3, max(cc(1), 1_c_int64_t)), a(3, max(cc(1), 1_c_int64_t)), &
+         af(3, max(cc(1), 1_c_int64_t)), icors(3, max(cc(1), 1_c_int64_t)), mijks(ncx, ncy, ncz))
+    call psx_check(psx_cpl_get_state(cpl, r, v, a, af), 'psx_cpl_get_state')
+    call psx_check(psx_cpl_nodes(cpl, icors, mijks), 'psx_cpl_nodes')
+    rsi = 1.d0 / rs
+    z3 = 0.d0
+    write(dname, '(A, I0)') 'LSM_disf', it
+    open(newunit=uu(1), file='LSM_coord_field', status='replace')
+    open(newunit=uu(2), file=trim(dname), status='replace')
+    open(newunit=uu(3), file='LSM_velocity_field', status='replace')
+    open(newunit=uu(4), file='LSM_force_field', status='replace')
+    open(newunit=uu(5), file='LSM_icors', status='replace')
+    open(newunit=uu(6), file='LSM_fluid_force', status='replace')
+    do kk = 1, ncz
+       do jj = 1, ncy
+          do ii = 1, ncx
+             l = mijks(ii, jj, kk)
+             if (l /= 0) then
+                write(uu(1), *) r(:, l)
+                write(uu(2), *) r(:, l)
+                write(uu(3), *) v(:, l)
+                write(uu(4), *) a(:, l) / rsi
+                write(uu(5), *) icors(:, l)
+                write(uu(6), *) af(:, l) / rsi
+             else
+                write(uu(2), *) z3
+                write(uu(3), *) z3
+                write(uu(4), *) z3
+                write(uu(6), *) z3
+             end if
+          end do
+       end do
+    end do
+    do l = 1, 6
+       close(uu(l))
+    end do
+  end subroutine save_solid
 
   ! The fluid files of save_results (lsm_lbm_static_alpha_beta_multipore.f:
   ! 2067-2155) in their list-directed layout; the pressure file is named

@@ -215,6 +215,14 @@ class Coupled:
     def __exit__(self, *exc):
         self.close()
 
+    def nodes(self):
+        """icors (ns, 3) 1-based and mijks (ncz, ncy, ncx) L1 solid ids."""
+        nz, ny, nx = self.lbm.shape
+        icors = np.zeros((self.ns, 3), np.int32)
+        mijks = np.zeros((nz, ny, nx), np.int32)
+        _lib.check(self._L.psx_cpl_nodes(self._h, _lib.ptr(icors), _lib.ptr(mijks)))
+        return icors, mijks
+
     def step(self, nsteps: int = 1):
         """nsteps main-loop steps: LBM step, fluid force, lsm_step."""
         _lib.check(self._L.psx_cpl_step(self._h, int(nsteps)))

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle_bind as ob
-from golden.make_golden import LSM_INP, lbm_cases, lsm_geometry
+from golden.make_golden import LSM_INP, coupled_case, lbm_cases, lsm_geometry
 from porescale_amd import geometry as geo
 from porescale_amd import io as pio
 
@@ -74,6 +74,37 @@ def test_lbm_driver_restart_matches_reference_golden():
     assert rows.shape == g["outm"].shape
     np.testing.assert_allclose(rows, g["outm"], rtol=1e-7, atol=1e-15, equal_nan=True)
     assert f"{fr0.shape[0]}" in settings
+
+
+def test_lbm_driver_coupled_matches_reference_files():
+    """With the cube_* files present the driver runs L1's coupled model: the
+    LSM_* files and the alpha_beta block of the reference program (fluid
+    force to the bit, the solid state and Biot coefficients to rounding)."""
+    ind, inp = coupled_case()
+    g = np.load(os.path.join(GOLD, "lbm_coupled.npz"))
+    d = _prep(ind)
+    try:
+        pio.write_lbm_input(os.path.join(d, "INPUT_PARAMETERS"), inp)
+        _run([_drv("psx_lbm_drv")], d)
+        it1, fr, fb = pio.read_lbm_restart(os.path.join(d, "iteration2"), g["fr"].shape[0])
+        lsmf = ob.read_lsm_files(d, it1, ind.shape)
+        ab = ob.read_alpha_beta(os.path.join(d, "alpha_beta"), inp.npores)
+        n_blocks = open(os.path.join(d, "alpha_beta")).read().count("Elastic medium properties")
+        icors = np.loadtxt(os.path.join(d, "LSM_icors"), dtype=np.int64)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    assert it1 == inp.itm and n_blocks == inp.itm
+    assert np.array_equal(fr, g["fr"]) and np.array_equal(fb, g["fb"])
+    assert np.array_equal(lsmf["fluid_force"], g["lsm_fluid_force"])
+    for k in ("disp", "vel", "force"):
+        want = g[f"lsm_{k}"]
+        np.testing.assert_allclose(lsmf[k], want, rtol=0, atol=1e-12 * np.abs(want).max())
+    for k in ("alphai", "betai", "asigmap", "astrain"):
+        want = g[f"ab_{k}"]
+        np.testing.assert_allclose(ab[k], want, rtol=0, atol=1e-10 * np.abs(want).max())
+    assert ab["ncub"] == int(g["ab_ncub"]) and ab["epsi"] == g["ab_epsi"] and ab["pres"] == g["ab_pres"]
+    mijks = ob.OracleCoupled(inp, ind).mijks()
+    assert icors.shape == (int(mijks.max()), 3)
 
 
 def test_lbm_driver_fp32_option_runs():
