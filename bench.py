@@ -15,10 +15,12 @@ Also reported (objects in the same line):
                with the moduli error; 64 uneven two-component sub-volumes in
                one batch; adjoint gradients of (K, G) at 128^3 with a
                lognormal stiffness field, and Gauss-Newton calibration steps.
+  coupled      L1's coupled model on the same 256^3 workload: ms per main-loop
+               step (LBM + momentum exchange + lsm_step) and calc_ab.
   cpu_baseline the oracle (C restatement of the reference, 1 thread) timed on
                this host over a bounded sample of the same LBM workload.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-lsm] [--no-cpu] [--full]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-lsm] [--no-cpu] [--no-coupled] [--full]
     python bench.py --lsm-only         # the LSM rows alone (profiling)
     python bench.py --calib-copy MIB   # plain float32 elementwise kernel (torch.neg)
                                        # over MIB MiB in and out, ten launches:
@@ -84,6 +86,42 @@ def run_lbm(steps, warmup):
         out.update(nl=lb.nl, tot_ms=tot, coll_ms=tc, dens_ms=td, wall_s=wall, permr=d.permr, permb=d.permb,
                    sat_b=d.outm[11])
     return out, ind, inp, blue
+
+
+def run_coupled(ind, inp, blue, steps=20):
+    """L1's coupled model on the same 256^3 workload: each main-loop step is
+    the FP32 LBM step, the bounce-back momentum exchange and lsm_step of the
+    solid lattice (FP64, periodic node grid); stable springs Em 0.2, nu 0.2,
+    rs 2.  Also times calc_ab (host, the reference's loops) once."""
+    import dataclasses
+    import numpy as np
+    from porescale_amd.lbm import LBM, Coupled, mrt_rates
+    inp2 = dataclasses.replace(inp, Em=0.2, nu=0.2, rs=2.0)
+    coll = ("mrt", mrt_rates(inp.vir), mrt_rates(inp.vib))
+    with LBM(inp2, ind, collision=coll, precision="f32") as lb:
+        lb.set_state(*initial_state(inp2, ind, blue))
+        lb.step(2)
+        with Coupled(lb) as cp:
+            cp.step(2)
+            lb.sync()
+            t0 = time.perf_counter()
+            cp.step(steps)
+            lb.sync()
+            t_c = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            lb.step(steps)
+            lb.sync()
+            t_l = time.perf_counter() - t0
+            r, v, a, af = cp.get_state()
+            t0 = time.perf_counter()
+            ab = cp.calc_ab()
+            t_ab = time.perf_counter() - t0
+            ns = cp.ns
+    return {"workload": "coupled lbm-d3q19 256^3 + solid lattice (Em 0.2, nu 0.2, rs 2)", "solid_nodes": ns,
+            "steps": steps, "ms_per_step": t_c / steps * 1e3, "lbm_ms_per_step": t_l / steps * 1e3,
+            "solid_ms_per_step": (t_c - t_l) / steps * 1e3, "calc_ab_s": t_ab,
+            "max_disp": float(np.abs(r).max()), "alphai_diag": np.diag(ab["alphai"]).tolist(),
+            "betai": ab["betai"].tolist()}
 
 
 def cpu_baseline(ind, inp, blue, budget_s=12.0):
@@ -225,6 +263,7 @@ def main():
     ap.add_argument("--lsm", dest="lsm", action="store_true", default=True)
     ap.add_argument("--no-lsm", dest="lsm", action="store_false")
     ap.add_argument("--no-cpu", dest="cpu", action="store_false", default=True)
+    ap.add_argument("--no-coupled", dest="coupled", action="store_false", default=True)
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB")
     ap.add_argument("--full", action="store_true", help="50-step calibration in the LSM rows")
     ap.add_argument("--lsm-only", action="store_true", help="only the LSM rows (for profiling them)")
@@ -278,6 +317,9 @@ def main():
                      "mean_launch_ms": lbm["coll_ms"]},
         "lbm_diag": {"permr": lbm["permr"], "permb": lbm["permb"], "sat_b": lbm["sat_b"]},
     }
+    if args.coupled:
+        out["coupled"] = run_coupled(ind, inp, blue)
+        log(f"coupled {json.dumps(out['coupled'])}")
     if args.lsm:
         out["lsm"] = run_lsm(args.full)
         log(f"LSM {json.dumps(out['lsm'])[:400]}")

@@ -867,6 +867,9 @@ struct psx_cpl {
     double *d_af = nullptr;
     int32_t cbon[24][3], cang[72][6];
     double csb[24], cab[72];
+    void *d_tab = nullptr;               // calc_ab tables (CubeTables, device)
+    int8_t *d_raw = nullptr;             // ind codes as read (device)
+    double *d_part = nullptr;            // calc_ab block partials
 };
 
 namespace {
@@ -914,6 +917,163 @@ void angle_table(int32_t nta[48][2], double anf[48][9]) {
             anf[t][3 + d] = comp(gj[d], cj2);
             anf[t][6 + d] = comp(gk[d], ck2);
         }
+    }
+}
+
+// Tables of calc_ab's per-cube loops (L1:4101-4113, 996-1014).
+struct CubeTables {
+    int32_t bon[24][3], ang[72][6];
+    double sb[24], ab[72];  // cube_sbond * alf, cube_abond * bet
+    double anf[48][9];
+    double bn[9][3], vb[9][3];
+    double vn[Q][3], vx[Q][3], vv[Q][3];
+};
+
+constexpr int NAB = 20;  // per-cube sums: sigmap (9), strain / 4 (9), cubes, pore voxels
+
+// calc_ab's cube loop (L1:4170-4340) with one thread per voxel: every solid
+// voxel evaluates its cube's stress (sigmap; sigmam is its exact negative)
+// and strain in the reference's operation order; the block sums go to
+// partial[block][NAB].  r is the displacement on the periodic node grid,
+// [3][N], node (x,y,z) at x + nx (y + ny z).
+__global__ __launch_bounds__(256) void k_calc_ab_cubes(const CubeTables *__restrict__ T,
+                                                       const int8_t *__restrict__ raw, int nx, int ny, int nz,
+                                                       int iipore, const double *__restrict__ r, int64_t N,
+                                                       double *__restrict__ partial) {
+    const int64_t nv = (int64_t)nx * ny * nz;
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[NAB];
+#pragma unroll
+    for (int k = 0; k < NAB; k++) acc[k] = 0.0;
+    if (v < nv) {
+        const int i = (int)(v % nx), j = (int)((v / nx) % ny), k = (int)(v / ((int64_t)nx * ny));
+        const int code = raw[v];
+        if (code == iipore) acc[19] = 1.0;
+        if (code == 0) {
+            acc[18] = 1.0;
+            int64_t g[8];
+            for (int l = 0; l < 8; l++) {
+                const int x = (i + (l & 1)) % nx, y = (j + ((l >> 1) & 1)) % ny, z = (k + (l >> 2)) % nz;
+                g[l] = x + (int64_t)nx * (y + (int64_t)ny * z);
+            }
+            auto R = [&](int l, int d) { return r[(int64_t)d * N + g[l]]; };
+            double sp[3][3] = {{0.0}}, st[3][3] = {{0.0}};  // [column][row]
+            for (int l = 0; l < 8; l++) {
+                const double sv[3] = {(l & 1) ? 0.5 : -0.5, ((l >> 1) & 1) ? 0.5 : -0.5, (l >> 2) ? 0.5 : -0.5};
+                const double up[3] = {R(l, 0), R(l, 1), R(l, 2)};
+                for (int m = 0; m < 3; m++)
+                    for (int n = 0; n < 3; n++) st[n][m] = st[n][m] + sv[m] * up[n] + sv[n] * up[m];
+            }
+            for (int l = 0; l < 24; l++) {
+                const int p1 = T->bon[l][0] - 1, p2 = T->bon[l][1] - 1, b = T->bon[l][2] - 1;
+                double rt[3], fl[3];
+                for (int d = 0; d < 3; d++) rt[d] = R(p2, d) - R(p1, d) + T->vb[b][d];
+                const double dot = rt[0] * T->bn[b][0] + rt[1] * T->bn[b][1] + rt[2] * T->bn[b][2];
+                for (int d = 0; d < 3; d++) fl[d] = T->sb[l] * (dot * T->bn[b][d] - T->vb[b][d]);
+                for (int cd = 0; cd < 3; cd++) {
+                    if (T->bn[b][cd] > 0)
+                        for (int d = 0; d < 3; d++) sp[cd][d] = sp[cd][d] + fl[d];
+                    if (T->bn[b][cd] < 0)
+                        for (int d = 0; d < 3; d++) sp[cd][d] = sp[cd][d] - fl[d];
+                }
+            }
+            for (int l = 0; l < 72; l++) {
+                const int32_t *e = T->ang[l];
+                const int pi = e[0] - 1, pj = e[1] - 1, pk = e[2] - 1, qj = e[3] - 1, qk = e[4] - 1;
+                const double *an = T->anf[e[5] - 1];
+                const double *xrj = T->vx[qj], *xrk = T->vx[qk];
+                double rtj[3], rtk[3];
+                for (int d = 0; d < 3; d++) {
+                    rtj[d] = R(pj, d) - R(pi, d) + T->vv[qj][d];
+                    rtk[d] = R(pk, d) - R(pi, d) + T->vv[qk][d];
+                }
+                const double dfi = ((xrk[1] * rtk[2] - xrk[2] * rtk[1] - xrj[1] * rtj[2] + xrj[2] * rtj[1]) * an[0] -
+                                    (xrk[0] * rtk[2] - xrk[2] * rtk[0] - xrj[0] * rtj[2] + xrj[2] * rtj[0]) * an[1] +
+                                    (xrk[0] * rtk[1] - xrk[1] * rtk[0] - xrj[0] * rtj[1] + xrj[1] * rtj[0]) * an[2]) *
+                                   T->ab[l];
+                for (int cd = 0; cd < 3; cd++) {
+                    if (T->vn[qj][cd] > 0)
+                        for (int d = 0; d < 3; d++) sp[cd][d] = sp[cd][d] - dfi * an[3 + d];
+                    if (T->vn[qj][cd] < 0)
+                        for (int d = 0; d < 3; d++) sp[cd][d] = sp[cd][d] + dfi * an[3 + d];
+                }
+                for (int cd = 0; cd < 3; cd++) {
+                    if (T->vn[qk][cd] > 0)
+                        for (int d = 0; d < 3; d++) sp[cd][d] = sp[cd][d] - dfi * an[6 + d];
+                    if (T->vn[qk][cd] < 0)
+                        for (int d = 0; d < 3; d++) sp[cd][d] = sp[cd][d] + dfi * an[6 + d];
+                }
+            }
+            for (int c = 0; c < 3; c++)
+                for (int d = 0; d < 3; d++) {
+                    acc[d + 3 * c] = sp[c][d];
+                    acc[9 + d + 3 * c] = st[c][d] * 0.25;
+                }
+        }
+    }
+    __shared__ double sh[4][NAB];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int k = 0; k < NAB; k++) {
+        const double t = psx::wave_sum(acc[k]);
+        if (lane == 0) sh[w][k] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < NAB) {
+        const int k = threadIdx.x;
+        partial[(int64_t)blockIdx.x * NAB + k] = ((sh[0][k] + sh[1][k]) + sh[2][k]) + sh[3][k];
+    }
+}
+
+constexpr int MAXPORES = 9;  // pore family codes are single digits
+
+// calc_ab's beta loop (L1:4381-4445): every face between a voxel of pore
+// family f and a solid voxel adds -(u_face . c) to family f, u_face the mean
+// displacement of the face's 4 nodes; block sums per family.
+__global__ __launch_bounds__(256) void k_calc_ab_faces(const int8_t *__restrict__ raw, int nx, int ny, int nz,
+                                                       int npores, const double *__restrict__ r, int64_t N,
+                                                       double *__restrict__ partial) {
+    const int64_t nv = (int64_t)nx * ny * nz;
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[MAXPORES];
+#pragma unroll
+    for (int k = 0; k < MAXPORES; k++) acc[k] = 0.0;
+    if (v < nv) {
+        const int i = (int)(v % nx), j = (int)((v / nx) % ny), k = (int)(v / ((int64_t)nx * ny));
+        const int code = raw[v];
+        if (code >= 1 && code <= npores) {
+            double b = 0.0;
+            for (int l = 1; l <= 6; l++) {
+                const int c0 = H_IV[l][0], c1 = H_IV[l][1], c2 = H_IV[l][2];
+                const int xn = (i + c0 + nx) % nx, yn = (j + c1 + ny) % ny, zn = (k + c2 + nz) % nz;
+                if (raw[xn + (int64_t)nx * (yn + (int64_t)ny * zn)] != 0) continue;
+                double up[3] = {0.0, 0.0, 0.0};
+                int64_t g[4];
+                for (int m = 0; m < 4; m++) {
+                    const int o1 = m & 1, o2 = m >> 1;
+                    int off[3];
+                    if (c0 != 0) { off[0] = (c0 + 1) / 2; off[1] = o1; off[2] = o2; }
+                    else if (c1 != 0) { off[0] = o1; off[1] = (c1 + 1) / 2; off[2] = o2; }
+                    else { off[0] = o1; off[1] = o2; off[2] = (c2 + 1) / 2; }
+                    g[m] = (i + off[0]) % nx + (int64_t)nx * ((j + off[1]) % ny + (int64_t)ny * ((k + off[2]) % nz));
+                }
+                for (int d = 0; d < 3; d++)
+                    up[d] = (r[d * N + g[0]] + r[d * N + g[1]] + r[d * N + g[2]] + r[d * N + g[3]]) * 0.25;
+                b = b - up[0] * (double)c0 - up[1] * (double)c1 - up[2] * (double)c2;
+            }
+            for (int f = 0; f < MAXPORES; f++)
+                if (f == code - 1) acc[f] = b;
+        }
+    }
+    __shared__ double sh[4][MAXPORES];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int f = 0; f < MAXPORES; f++) {
+        const double t = psx::wave_sum(acc[f]);
+        if (lane == 0) sh[w][f] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < MAXPORES) {
+        const int f = threadIdx.x;
+        partial[(int64_t)blockIdx.x * MAXPORES + f] = ((sh[0][f] + sh[1][f]) + sh[2][f]) + sh[3][f];
     }
 }
 
@@ -1015,6 +1175,38 @@ extern "C" int psx_cpl_create(psx_lbm *lbm, double Em, double nu, double dl, dou
         for (int d = 0; d < 3; d++) L.cv[q][d] = (double)H_IV[q][d];
     }
     L.rsdti2 = (1.0 / rs) * dti * dti;
+    {  // calc_ab tables and the raw voxel codes
+        CubeTables T{};
+        std::memcpy(T.bon, c->cbon, sizeof T.bon);
+        std::memcpy(T.ang, c->cang, sizeof T.ang);
+        for (int l = 0; l < 24; l++) T.sb[l] = c->csb[l] * c->alf;
+        for (int l = 0; l < 72; l++) T.ab[l] = c->cab[l] * c->bet;
+        int32_t nta[48][2];
+        angle_table(nta, T.anf);
+        static const int IB[9][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1},  {1, 1, 0}, {1, -1, 0},
+                                     {1, 0, 1}, {1, 0, -1}, {0, 1, 1}, {0, 1, -1}};
+        for (int q = 0; q < Q; q++) {
+            double s2 = 0.0;
+            for (int d = 0; d < 3; d++) { T.vv[q][d] = H_IV[q][d]; s2 += T.vv[q][d] * T.vv[q][d]; }
+            const double nr = std::sqrt(s2);
+            for (int d = 0; d < 3; d++) { T.vn[q][d] = q ? T.vv[q][d] / nr : 0.0; T.vx[q][d] = q ? T.vv[q][d] / s2 : 0.0; }
+        }
+        for (int b = 0; b < 9; b++) {
+            double s2 = 0.0;
+            for (int d = 0; d < 3; d++) { T.vb[b][d] = IB[b][d]; s2 += T.vb[b][d] * T.vb[b][d]; }
+            const double nr = std::sqrt(s2);
+            for (int d = 0; d < 3; d++) T.bn[b][d] = T.vb[b][d] / nr;
+        }
+        std::vector<int8_t> raw8((size_t)nv);
+        for (int64_t a = 0; a < nv; a++) raw8[(size_t)a] = (int8_t)lbm->ind_raw[(size_t)a];
+        const int64_t nb = psx::ceil_div(nv, 256);
+        PSX_HIP(hipMalloc(&c->d_tab, sizeof(CubeTables)));  // holds T
+        PSX_HIP(hipMalloc(&c->d_raw, (size_t)nv));
+        PSX_HIP(hipMalloc(&c->d_part, sizeof(double) * (size_t)nb * NAB + sizeof(double) * NAB));
+        PSX_HIP(hipMemcpyAsync(c->d_tab, &T, sizeof T, hipMemcpyHostToDevice, st));
+        PSX_HIP(hipMemcpyAsync(c->d_raw, raw8.data(), (size_t)nv, hipMemcpyHostToDevice, st));
+        PSX_HIP(hipStreamSynchronize(st));
+    }
     rc = psx_lsm_cpl::reset(c->lsm);
     if (rc) { psx_cpl_destroy(c); return rc; }
     *out = c;
@@ -1025,7 +1217,7 @@ extern "C" int psx_cpl_create(psx_lbm *lbm, double Em, double nu, double dl, dou
 extern "C" void psx_cpl_destroy(psx_cpl *c) {
     if (!c) return;
     if (c->lbm && c->lbm->stream) (void)hipStreamSynchronize(c->lbm->stream);
-    void *bufs[] = {c->d_start, c->d_contrib, c->d_lnode, c->d_lq, c->d_grid, c->d_af};
+    void *bufs[] = {c->d_start, c->d_contrib, c->d_lnode, c->d_lq, c->d_grid, c->d_af, c->d_tab, c->d_raw, c->d_part};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->lsm) psx_lsm_destroy(c->lsm);
@@ -1116,152 +1308,43 @@ extern "C" int psx_cpl_set_state(psx_cpl *c, const double *r, const double *v, c
 }
 
 extern "C" int psx_cpl_calc_ab(psx_cpl *c, int32_t npores, double pres, double *out) {
-    PSX_CHECK_ARG(c && out && npores >= 0, "psx_cpl_calc_ab: bad argument");
+    PSX_CHECK_ARG(c && out && npores >= 0 && npores <= MAXPORES, "psx_cpl_calc_ab: bad argument (npores 0..9)");
     PSX_GUARD_BEGIN
     psx_lbm *h = c->lbm;
     const int nx = h->nx, ny = h->ny, nz = h->nz;
+    const int64_t nv = (int64_t)nx * ny * nz, nb = psx::ceil_div(nv, 256);
     if (!(pres > 0.0)) pres = (1.0 - 0.99) * h->p.rr0 * h->p.cr2;  // L1:3987
-    std::vector<double> r(3 * (size_t)std::max<int64_t>(c->ns, 1));
-    int rc = psx_cpl_get_state(c, r.data(), nullptr, nullptr, nullptr);
-    if (rc) return rc;
-    // lattice vectors (L1:996-1014) and the angle table
-    double vv[Q][3], vn[Q][3], vx[Q][3], vb[9][3], bn[9][3], anf[48][9];
-    int32_t nta[48][2];
-    static const int IB[9][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1},  {1, 1, 0}, {1, -1, 0},
-                                 {1, 0, 1}, {1, 0, -1}, {0, 1, 1}, {0, 1, -1}};
-    for (int q = 0; q < Q; q++) {
-        double s2 = 0.0;
-        for (int d = 0; d < 3; d++) { vv[q][d] = H_IV[q][d]; s2 += vv[q][d] * vv[q][d]; }
-        const double nr = std::sqrt(s2);
-        for (int d = 0; d < 3; d++) { vn[q][d] = q ? vv[q][d] / nr : 0.0; vx[q][d] = q ? vv[q][d] / s2 : 0.0; }
+    double *r, *v, *a;
+    psx_lsm_cpl::state(c->lsm, &r, &v, &a);
+    double *red = c->d_part + nb * NAB;
+    double sums[NAB] = {0.0}, fam[MAXPORES] = {0.0};
+    hipLaunchKernelGGL(k_calc_ab_cubes, dim3((unsigned)nb), dim3(256), 0, h->stream,
+                       (const CubeTables *)c->d_tab, c->d_raw, nx, ny, nz,
+                       (int)h->p.iipore, r, c->N, c->d_part);
+    hipLaunchKernelGGL(k_reduce_cols, dim3(NAB), dim3(256), 0, h->stream, c->d_part, nb, NAB, red);
+    PSX_HIP(hipGetLastError());
+    PSX_HIP(hipMemcpyAsync(sums, red, sizeof sums, hipMemcpyDeviceToHost, h->stream));
+    PSX_HIP(hipStreamSynchronize(h->stream));
+    if (npores > 0) {
+        hipLaunchKernelGGL(k_calc_ab_faces, dim3((unsigned)nb), dim3(256), 0, h->stream, c->d_raw, nx, ny, nz, npores,
+                           r, c->N, c->d_part);
+        hipLaunchKernelGGL(k_reduce_cols, dim3(MAXPORES), dim3(256), 0, h->stream, c->d_part, nb, MAXPORES, red);
+        PSX_HIP(hipGetLastError());
+        PSX_HIP(hipMemcpyAsync(fam, red, sizeof fam, hipMemcpyDeviceToHost, h->stream));
+        PSX_HIP(hipStreamSynchronize(h->stream));
     }
-    for (int b = 0; b < 9; b++) {
-        double s2 = 0.0;
-        for (int d = 0; d < 3; d++) { vb[b][d] = IB[b][d]; s2 += vb[b][d] * vb[b][d]; }
-        const double nr = std::sqrt(s2);
-        for (int d = 0; d < 3; d++) bn[b][d] = vb[b][d] / nr;
-    }
-    angle_table(nta, anf);
-    double sb[24], ab[72];
-    for (int l = 0; l < 24; l++) sb[l] = c->csb[l] * c->alf;
-    for (int l = 0; l < 72; l++) ab[l] = c->cab[l] * c->bet;
-    auto raw = [&](int i, int j, int k) { return h->ind_raw[(size_t)(i - 1) + (size_t)nx * ((j - 1) + (size_t)ny * (k - 1))]; };
-    auto node = [&](const int *p3) -> const double * {
-        const int32_t id = c->mijks[(size_t)(p3[0] - 1) + (size_t)nx * ((p3[1] - 1) + (size_t)ny * (p3[2] - 1))];
-        return &r[3 * (size_t)(id - 1)];
-    };
     const double vcell = (double)((int32_t)(nx * ny * nz));
-    double epsi = 0.0;
-    for (int i = 1; i <= nx; i++)
-        for (int j = 1; j <= ny; j++)
-            for (int k = 1; k <= nz; k++)
-                if (raw(i, j, k) == h->p.iipore) epsi = epsi + 1.0;
-    epsi = epsi / vcell;
-    static const int icv[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
-    double asp[3][3] = {{0}}, asm_[3][3] = {{0}}, ast[3][3] = {{0}};  // [column][row]
-    int64_t ncub = 0;
-    for (int i = 1; i <= nx; i++)  // calc_ab's cube loop (L1:4170-4340)
-        for (int j = 1; j <= ny; j++)
-            for (int k = 1; k <= nz; k++) {
-                if (raw(i, j, k) != 0) continue;
-                int icp[8][3];
-                for (int l = 0; l < 8; l++) {
-                    icp[l][0] = wrap1(i + icv[l][0], nx);
-                    icp[l][1] = wrap1(j + icv[l][1], ny);
-                    icp[l][2] = wrap1(k + icv[l][2], nz);
-                }
-                double sp[3][3] = {{0}}, sm[3][3] = {{0}}, stn[3][3] = {{0}};
-                for (int l = 0; l < 8; l++) {
-                    const double *up = node(icp[l]);
-                    const double sv[3] = {icv[l][0] ? 0.5 : -0.5, icv[l][1] ? 0.5 : -0.5, icv[l][2] ? 0.5 : -0.5};
-                    for (int m = 0; m < 3; m++)
-                        for (int n = 0; n < 3; n++) stn[n][m] = stn[n][m] + sv[m] * up[n] + sv[n] * up[m];
-                }
-                auto add = [&](int cdir, const double *f, double sgn) {
-                    for (int d = 0; d < 3; d++) {
-                        sp[cdir][d] = sp[cdir][d] + sgn * f[d];
-                        sm[cdir][d] = sm[cdir][d] - sgn * f[d];
-                    }
-                };
-                for (int l = 0; l < 24; l++) {
-                    const int32_t *e = c->cbon[l];
-                    const double *r1 = node(icp[e[0] - 1]), *r2 = node(icp[e[1] - 1]);
-                    const double *xn = bn[e[2] - 1], *rn = vb[e[2] - 1];
-                    double rt[3], fl[3];
-                    for (int d = 0; d < 3; d++) rt[d] = r2[d] - r1[d] + rn[d];
-                    const double dot = rt[0] * xn[0] + rt[1] * xn[1] + rt[2] * xn[2];
-                    for (int d = 0; d < 3; d++) fl[d] = sb[l] * (dot * xn[d] - rn[d]);
-                    for (int cd = 0; cd < 3; cd++) {
-                        if (xn[cd] > 0) add(cd, fl, 1.0);
-                        if (xn[cd] < 0) add(cd, fl, -1.0);
-                    }
-                }
-                for (int l = 0; l < 72; l++) {
-                    const int32_t *e = c->cang[l];
-                    const double *ri = node(icp[e[0] - 1]), *rj = node(icp[e[1] - 1]), *rk = node(icp[e[2] - 1]);
-                    const double *xnj = vn[e[3] - 1], *xrj = vx[e[3] - 1], *rnj = vv[e[3] - 1];
-                    const double *xnk = vn[e[4] - 1], *xrk = vx[e[4] - 1], *rnk = vv[e[4] - 1];
-                    const double *an = anf[e[5] - 1];
-                    double rtj[3], rtk[3];
-                    for (int d = 0; d < 3; d++) { rtj[d] = rj[d] - ri[d] + rnj[d]; rtk[d] = rk[d] - ri[d] + rnk[d]; }
-                    const double dfi = ((xrk[1] * rtk[2] - xrk[2] * rtk[1] - xrj[1] * rtj[2] + xrj[2] * rtj[1]) * an[0] -
-                                        (xrk[0] * rtk[2] - xrk[2] * rtk[0] - xrj[0] * rtj[2] + xrj[2] * rtj[0]) * an[1] +
-                                        (xrk[0] * rtk[1] - xrk[1] * rtk[0] - xrj[0] * rtj[1] + xrj[1] * rtj[0]) * an[2]) *
-                                       ab[l];
-                    const double fj[3] = {dfi * an[3], dfi * an[4], dfi * an[5]};
-                    const double fk[3] = {dfi * an[6], dfi * an[7], dfi * an[8]};
-                    for (int cd = 0; cd < 3; cd++) {
-                        if (xnj[cd] > 0) add(cd, fj, -1.0);
-                        if (xnj[cd] < 0) add(cd, fj, 1.0);
-                    }
-                    for (int cd = 0; cd < 3; cd++) {
-                        if (xnk[cd] > 0) add(cd, fk, -1.0);
-                        if (xnk[cd] < 0) add(cd, fk, 1.0);
-                    }
-                }
-                for (int a = 0; a < 3; a++)
-                    for (int d = 0; d < 3; d++) {
-                        asp[a][d] = asp[a][d] + sp[a][d];
-                        asm_[a][d] = asm_[a][d] + sm[a][d];
-                        ast[a][d] = ast[a][d] + stn[a][d] * 0.25;
-                    }
-                ncub++;
-            }
-    for (int a = 0; a < 3; a++)
-        for (int d = 0; d < 3; d++) {
-            out[d + 3 * a] = asp[a][d];
-            out[9 + d + 3 * a] = asm_[a][d];
-            out[18 + d + 3 * a] = ast[a][d];
-            out[28 + d + 3 * a] = -(a == d ? 1.0 : 0.0) * epsi + asp[a][d] / vcell / pres;  // L1:4378
-        }
-    out[27] = epsi;
-    out[37] = (double)ncub;
-    for (int iip = 1; iip <= npores; iip++) {  // betai (L1:4381-4445)
-        double b = 0.0;
-        for (int k = 1; k <= nz; k++)
-            for (int j = 1; j <= ny; j++)
-                for (int i = 1; i <= nx; i++) {
-                    if (raw(i, j, k) != iip) continue;
-                    for (int l = 1; l <= 6; l++) {
-                        const int *cl = H_IV[l];
-                        if (raw(wrap1(i + cl[0], nx), wrap1(j + cl[1], ny), wrap1(k + cl[2], nz)) != 0) continue;
-                        const double *pr[4];
-                        for (int m = 0; m < 4; m++) {
-                            const int o1 = m & 1, o2 = m >> 1;
-                            int off[3];
-                            if (cl[0] != 0) { off[0] = (cl[0] + 1) / 2; off[1] = o1; off[2] = o2; }
-                            else if (cl[1] != 0) { off[0] = o1; off[1] = (cl[1] + 1) / 2; off[2] = o2; }
-                            else { off[0] = o1; off[1] = o2; off[2] = (cl[2] + 1) / 2; }
-                            const int p3[3] = {wrap1(i + off[0], nx), wrap1(j + off[1], ny), wrap1(k + off[2], nz)};
-                            pr[m] = node(p3);
-                        }
-                        double up[3];
-                        for (int d = 0; d < 3; d++) up[d] = (pr[0][d] + pr[1][d] + pr[2][d] + pr[3][d]) * 0.25;
-                        b = b - up[0] * vv[l][0] - up[1] * vv[l][1] - up[2] * vv[l][2];
-                    }
-                }
-        out[38 + iip - 1] = b / vcell / pres;
+    const double epsi = sums[19] / vcell;
+    for (int k = 0; k < 9; k++) {
+        const int row = k % 3, col = k / 3;
+        out[k] = sums[k];
+        out[9 + k] = -sums[k];  // sigmam mirrors sigmap term by term
+        out[18 + k] = sums[9 + k];
+        out[28 + k] = -(row == col ? 1.0 : 0.0) * epsi + sums[k] / vcell / pres;  // L1:4378
     }
+    out[27] = epsi;
+    out[37] = sums[18];
+    for (int f = 0; f < npores; f++) out[38 + f] = fam[f] / vcell / pres;
     return PSX_OK;
     PSX_GUARD_END
 }

@@ -63,8 +63,10 @@ def test_coupled_fp64_vs_reference_files():
 
 
 def test_coupled_calc_ab_equals_oracle_on_same_state():
-    """calc_ab of the library and of the oracle on the GPU's displacement:
-    identical loops, identical bits."""
+    """calc_ab on the GPU and in the oracle from the same displacement: the
+    per-cube terms are evaluated in the reference's order, the sums over
+    cubes and faces in a tree, so the totals agree to summation rounding;
+    counts exactly."""
     _say("case coupled calc_ab vs oracle")
     ind, inp, lb, cp = _run()
     r, v, a, af = cp.get_state()
@@ -72,8 +74,11 @@ def test_coupled_calc_ab_equals_oracle_on_same_state():
     o.set_state(r, v, a)
     want = o.calc_ab()
     got = cp.calc_ab()
-    for k in ("asigmap", "asigmam", "astrain", "alphai", "betai", "epsi", "ncub"):
-        assert np.array_equal(np.asarray(got[k]), np.asarray(want[k])), k
+    for k in ("asigmap", "asigmam", "astrain", "alphai", "betai"):
+        w = np.asarray(want[k])
+        np.testing.assert_allclose(np.asarray(got[k]), w, rtol=0, atol=1e-13 * np.abs(w).max())
+    assert got["epsi"] == want["epsi"] and got["ncub"] == want["ncub"]
+    assert np.array_equal(got["asigmam"], -got["asigmap"])
     cp.close()
     lb.close()
 
