@@ -402,19 +402,37 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
         }
         build_bricks(h, bd, vox, bricks, hnode, hcell);
     }
-    // Compressed pull table (FP32 path): the source of direction q lies in
-    // plane X - c_qx; its index is n + pbase[c_qx + 1][X] + d with d the
-    // difference of the two nodes' ranks within their planes, which fits
-    // int16 on porous media (adjacent planes are alike).  Kept only if every
-    // residual fits; PSX_LBM_IDX16=0 disables it.
+    // Compressed pull table (FP32 path), 36 B per node instead of 72:
+    //  - direct: src = n + d with d an int16 offset; fits when an x-plane
+    //    holds fewer than ~32k fluid nodes (256^3 at porosity 0.3: 19.7k);
+    //  - plane-based otherwise: the source of direction q lies in plane
+    //    X - c_qx; its index is n + pbase[c_qx + 1][X] + d with d the
+    //    difference of the two nodes' ranks within their planes, which fits
+    //    int16 on porous media (adjacent planes are alike).
+    // Each form is kept only if every offset fits; PSX_LBM_IDX16=0 keeps the
+    // int32 table only, PSX_LBM_IDX16=2 skips the direct form.
     std::vector<int16_t> s16;
     std::vector<int32_t> pbase;
-    bool use16 = false;
+    bool use16 = false, direct = false;
+    int idx16 = 1;
     if (h->f32 && !h->fused && nl > 0) {
         const char *e16 = std::getenv("PSX_LBM_IDX16");
-        use16 = !e16 || std::atoi(e16) != 0;
+        idx16 = e16 ? std::atoi(e16) : 1;
+        use16 = idx16 != 0;
     }
-    if (use16) {
+    if (use16 && idx16 == 1) {
+        direct = true;
+        s16.assign(18 * (size_t)nl, 0);
+        for (int q = 1; q < Q && direct; q++)
+            for (int64_t n = 0; n < nl; n++) {
+                const int32_t sidx = h->dst_h[(size_t)(q - 1) * nl + n];
+                if (sidx < 0) { s16[(size_t)(q - 1) * nl + n] = (int16_t)-32768; continue; }
+                const int64_t d = (int64_t)sidx - n;
+                if (d < -32767 || d > 32767) { direct = false; break; }
+                s16[(size_t)(q - 1) * nl + n] = (int16_t)d;
+            }
+    }
+    if (use16 && !direct) {
         std::vector<int64_t> pstart(nx + 1, 0);
         for (int64_t n = 0; n < nl; n++) pstart[h->icor[3 * n]]++;  // counts of plane x-1 at index x
         for (int X = 0; X < nx; X++) pstart[X + 1] += pstart[X];
@@ -446,9 +464,11 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
     PSX_HIP(hipMalloc(&h->d_vox, sizeof(int32_t) * (size_t)(nl > 0 ? nl : 1)));
     if (use16) {
         PSX_HIP(hipMalloc(&h->d_src16, sizeof(int16_t) * 18 * (size_t)nl));
-        PSX_HIP(hipMalloc(&h->d_pbase, sizeof(int32_t) * 3 * (size_t)nx));
         PSX_HIP(hipMemcpyAsync(h->d_src16, s16.data(), sizeof(int16_t) * 18 * nl, hipMemcpyHostToDevice, h->stream));
-        PSX_HIP(hipMemcpyAsync(h->d_pbase, pbase.data(), sizeof(int32_t) * 3 * nx, hipMemcpyHostToDevice, h->stream));
+        if (!direct) {  // the plane-based form needs the bases
+            PSX_HIP(hipMalloc(&h->d_pbase, sizeof(int32_t) * 3 * (size_t)nx));
+            PSX_HIP(hipMemcpyAsync(h->d_pbase, pbase.data(), sizeof(int32_t) * 3 * nx, hipMemcpyHostToDevice, h->stream));
+        }
     }
     size_t phib = (size_t)nv * sizeof(double);
     PSX_HIP(hipMalloc(&h->d_phi, phib));
@@ -862,8 +882,10 @@ struct psx_cpl {
     std::vector<int32_t> grid;   // [ns] node-grid index of L1 solid node s
     std::vector<int32_t> mijks;  // L1 solid ids on the node grid (0 = none)
     CplLinks L{};
-    int32_t *d_start = nullptr, *d_contrib = nullptr, *d_lnode = nullptr, *d_grid = nullptr;
+    int32_t *d_start = nullptr, *d_lnode = nullptr, *d_grid = nullptr;
+    uint32_t *d_contrib = nullptr;
     int8_t *d_lq = nullptr;
+    double *d_y = nullptr;
     double *d_af = nullptr;
     int32_t cbon[24][3], cang[72][6];
     double csb[24], cab[72];
@@ -1130,6 +1152,11 @@ extern "C" int psx_cpl_create(psx_lbm *lbm, double Em, double nu, double dl, dou
         for (size_t i = 0; i < lbm->perm.size(); i++) inv[(size_t)lbm->perm[i]] = (int32_t)i;
     }
     const int64_t nlink = n2 + n4;
+    if (nlink >= ((int64_t)1 << CPL_QSHIFT)) {
+        psx::set_error("psx_cpl_create: %lld bounce-back links exceed the coupling table", (long long)nlink);
+        psx_cpl_destroy(c);
+        return PSX_EINVAL;
+    }
     std::vector<int32_t> lnode((size_t)std::max<int64_t>(nlink, 1)), cnt((size_t)c->ns + 1, 0);
     std::vector<int8_t> lq((size_t)std::max<int64_t>(nlink, 1));
     auto row = [&](int64_t li) -> const int32_t * { return li < n2 ? &ipb2[4 * li] : &ipb4[6 * (li - n2)]; };
@@ -1139,33 +1166,48 @@ extern "C" int psx_cpl_create(psx_lbm *lbm, double Em, double nu, double dl, dou
         lnode[(size_t)li] = (int32_t)(inv.empty() ? n : inv[(size_t)n]);
         lq[(size_t)li] = (int8_t)(e[1] - 1);
         for (int m = 0; m < (li < n2 ? 2 : 4); m++)
-            if (e[2 + m] > 0) cnt[(size_t)e[2 + m]]++;  // id 0: a wall voxel of another pore family
+            if (e[2 + m] > 0) cnt[(size_t)e[2 + m] - 1]++;  // id 0: a wall voxel of another pore family
     }
-    for (int64_t s = 0; s < c->ns; s++) cnt[(size_t)s + 1] += cnt[(size_t)s];
-    std::vector<int32_t> contrib((size_t)std::max<int32_t>(cnt[(size_t)c->ns], 1)), fill(cnt.begin(), cnt.end() - 1);
+    // surface solid nodes (those with links), their contribution ranges
+    std::vector<int32_t> sidx((size_t)c->ns, -1), sgrid, start(1, 0);
+    for (int64_t s = 0; s < c->ns; s++)
+        if (cnt[(size_t)s] > 0) {
+            sidx[(size_t)s] = (int32_t)sgrid.size();
+            sgrid.push_back(c->grid[(size_t)s]);
+            start.push_back(start.back() + cnt[(size_t)s]);
+        }
+    const int64_t nsurf = (int64_t)sgrid.size();
+    std::vector<uint32_t> contrib((size_t)std::max<int32_t>(start.back(), 1));
+    std::vector<int32_t> fill(start.begin(), start.end() - 1);
     for (int64_t li = 0; li < nlink; li++) {
         const int32_t *e = row(li);
         for (int m = 0; m < (li < n2 ? 2 : 4); m++)
-            if (e[2 + m] > 0) contrib[(size_t)fill[(size_t)e[2 + m] - 1]++] = (int32_t)li;
+            if (e[2 + m] > 0)
+                contrib[(size_t)fill[(size_t)sidx[(size_t)e[2 + m] - 1]]++] =
+                    (uint32_t)li | ((uint32_t)(e[1] - 1) << CPL_QSHIFT);
     }
-    PSX_HIP(hipMalloc(&c->d_start, sizeof(int32_t) * cnt.size()));
-    PSX_HIP(hipMalloc(&c->d_contrib, sizeof(int32_t) * contrib.size()));
+    PSX_HIP(hipMalloc(&c->d_start, sizeof(int32_t) * start.size()));
+    PSX_HIP(hipMalloc(&c->d_contrib, sizeof(uint32_t) * contrib.size()));
     PSX_HIP(hipMalloc(&c->d_lnode, sizeof(int32_t) * lnode.size()));
     PSX_HIP(hipMalloc(&c->d_lq, lq.size()));
-    PSX_HIP(hipMalloc(&c->d_grid, sizeof(int32_t) * std::max<size_t>(1, c->grid.size())));
+    PSX_HIP(hipMalloc(&c->d_y, sizeof(double) * lnode.size()));
+    PSX_HIP(hipMalloc(&c->d_grid, sizeof(int32_t) * std::max<size_t>(1, sgrid.size())));
     PSX_HIP(hipMalloc(&c->d_af, sizeof(double) * 3 * (size_t)c->N));
     hipStream_t st = lbm->stream;
-    PSX_HIP(hipMemcpyAsync(c->d_start, cnt.data(), sizeof(int32_t) * cnt.size(), hipMemcpyHostToDevice, st));
-    PSX_HIP(hipMemcpyAsync(c->d_contrib, contrib.data(), sizeof(int32_t) * contrib.size(), hipMemcpyHostToDevice, st));
+    PSX_HIP(hipMemcpyAsync(c->d_start, start.data(), sizeof(int32_t) * start.size(), hipMemcpyHostToDevice, st));
+    PSX_HIP(hipMemcpyAsync(c->d_contrib, contrib.data(), sizeof(uint32_t) * contrib.size(), hipMemcpyHostToDevice, st));
     PSX_HIP(hipMemcpyAsync(c->d_lnode, lnode.data(), sizeof(int32_t) * lnode.size(), hipMemcpyHostToDevice, st));
     PSX_HIP(hipMemcpyAsync(c->d_lq, lq.data(), lq.size(), hipMemcpyHostToDevice, st));
-    if (c->ns > 0)
-        PSX_HIP(hipMemcpyAsync(c->d_grid, c->grid.data(), sizeof(int32_t) * c->grid.size(), hipMemcpyHostToDevice, st));
+    if (nsurf > 0)
+        PSX_HIP(hipMemcpyAsync(c->d_grid, sgrid.data(), sizeof(int32_t) * sgrid.size(), hipMemcpyHostToDevice, st));
+    // af stays zero at solid nodes without links
     PSX_HIP(hipMemsetAsync(c->d_af, 0, sizeof(double) * 3 * (size_t)c->N, st));
     PSX_HIP(hipStreamSynchronize(st));
     CplLinks &L = c->L;
-    L.start = c->d_start; L.contrib = c->d_contrib; L.lnode = c->d_lnode; L.lq = c->d_lq; L.grid = c->d_grid;
-    L.ns = (int32_t)c->ns;
+    L.lnode = c->d_lnode; L.lq = c->d_lq; L.y = c->d_y;
+    L.start = c->d_start; L.contrib = c->d_contrib; L.sgrid = c->d_grid;
+    L.nlink = (int32_t)nlink;
+    L.nsurf = (int32_t)nsurf;
     L.nedge = (int32_t)n2;
     L.N = c->N;
     const psx_lbm_params &p = lbm->p;
@@ -1217,7 +1259,8 @@ extern "C" int psx_cpl_create(psx_lbm *lbm, double Em, double nu, double dl, dou
 extern "C" void psx_cpl_destroy(psx_cpl *c) {
     if (!c) return;
     if (c->lbm && c->lbm->stream) (void)hipStreamSynchronize(c->lbm->stream);
-    void *bufs[] = {c->d_start, c->d_contrib, c->d_lnode, c->d_lq, c->d_grid, c->d_af, c->d_tab, c->d_raw, c->d_part};
+    void *bufs[] = {c->d_start, c->d_contrib, c->d_lnode, c->d_lq, c->d_y, c->d_grid, c->d_af, c->d_tab, c->d_raw,
+                    c->d_part};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->lsm) psx_lsm_destroy(c->lsm);
@@ -1254,15 +1297,17 @@ extern "C" int psx_cpl_step(psx_cpl *c, int64_t nsteps) {
     for (int64_t s = 0; s < nsteps; s++) {
         int rc = psx_lbm_step(h, 1);
         if (rc) return rc;
-        if (c->ns > 0) {
+        if (c->L.nsurf > 0) {
             if (h->f32) {
-                PSX_HIP(psx_lbm_fast::fluid_force(h->c, (const float *)h->d_pop[h->cur], c->L, c->d_af, h->stream));
+                PSX_HIP(psx_lbm_fast::link_force(h->c, (const float *)h->d_pop[h->cur], c->L, h->stream));
             } else {
                 FOutF64 f{(const double *)h->d_pop[h->cur], h->nl};
-                hipLaunchKernelGGL(k_fluid_force<FOutF64>, dim3((unsigned)psx::ceil_div(c->ns, 256)), dim3(256), 0,
-                                   h->stream, c->L, f, c->d_af);
-                PSX_HIP(hipGetLastError());
+                hipLaunchKernelGGL(k_link_force<FOutF64>, dim3((unsigned)psx::ceil_div(c->L.nlink, 256)), dim3(256), 0,
+                                   h->stream, c->L, f);
             }
+            hipLaunchKernelGGL(k_node_force, dim3((unsigned)psx::ceil_div(c->L.nsurf, 256)), dim3(256), 0, h->stream,
+                               c->L, c->d_af);
+            PSX_HIP(hipGetLastError());
         }
         rc = psx_lsm_cpl::step(c->lsm, c->d_af, 0.25, h->stream);  // damping of L1:1888
         if (rc) return rc;

@@ -476,42 +476,54 @@ __global__ __launch_bounds__(256) void k_fields(const LbmConst c, const St in, c
 // LSM-LBM coupling force (L1:3849-3879) on the coupled model's solid nodes.
 // Every bounce-back link gives -2 (f_out - 0.99 fq) c_in rsi/dt^2 to the 2
 // (edge link) or 4 (face link) solid nodes of its wall, half or a quarter
-// each; a node sums its contributions in the reference's loop order (edge
-// links, then face links, each in table order), so the FP64 path reproduces
-// the reference's sums bit for bit.
+// each.  Two passes: k_link_force evaluates y = -2 (f_out - 0.99 fq) once per
+// link (links in node order, so the population reads stay local), then
+// k_node_force sums y c rsi/dt^2 x share over the links of every solid node
+// that has any, in the reference's loop order (edge links, then face links,
+// each in table order): the FP64 path reproduces the reference's sums bit
+// for bit.
 struct CplLinks {
-    const int32_t *start;    // [ns+1] contribution range of solid node s
-    const int32_t *contrib;  // link ids; ids < nedge are edge links
     const int32_t *lnode;    // [nlink] fluid node (storage index)
     const int8_t *lq;        // [nlink] direction of the returning population (0-based)
-    const int32_t *grid;     // [ns] node-grid index of solid node s
-    int32_t ns, nedge;
+    double *y;               // [nlink] per-link term
+    const int32_t *start;    // [nsurf+1] contribution range of surface solid node s
+    const uint32_t *contrib;  // link id | direction << 27; ids < nedge are edge links
+    const int32_t *sgrid;    // [nsurf] node-grid index of surface solid node s
+    int32_t nlink, nsurf, nedge, pad_;
     int64_t N;               // node-grid size; af is [3][N]
     double fq[Q];            // rr0 (cr2 dt^2/6 | cr2 dt^2/12), L1:3417
     double cv[Q][3];         // lattice velocities
     double rsdti2;           // rsi/dt^2
 };
+constexpr int CPL_QSHIFT = 27;
 
 // FOut(n, qb): the post-collision population that returns to node n in
 // direction qb, fr + fb.
 template <class FOut>
-__global__ __launch_bounds__(256) void k_fluid_force(const CplLinks L, const FOut fout, double *__restrict__ af) {
+__global__ __launch_bounds__(256) void k_link_force(const CplLinks L, const FOut fout) {
+    const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= L.nlink) return;
+    const int qb = L.lq[li];
+    L.y[li] = -2.0 * (fout(L.lnode[li], qb) - 0.99 * L.fq[qb]);
+}
+
+static __global__ __launch_bounds__(256) void k_node_force(const CplLinks L, double *__restrict__ af) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= L.ns) return;
+    if (s >= L.nsurf) return;
     double a[3] = {0.0, 0.0, 0.0};
     for (int32_t k = L.start[s]; k < L.start[s + 1]; k++) {
-        const int32_t li = L.contrib[k];
-        const int64_t n = L.lnode[li];
-        const int qb = L.lq[li];
-        const double x = fout(n, qb) - 0.99 * L.fq[qb];
+        const uint32_t e = L.contrib[k];
+        const int32_t li = (int32_t)(e & ((1u << CPL_QSHIFT) - 1u));
+        const int qb = (int)(e >> CPL_QSHIFT);
+        const double y = L.y[li];
         const double share = li < L.nedge ? 0.5 : 0.25;
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            const double floc = -2.0 * x * L.cv[qb][d] * L.rsdti2;
+            const double floc = y * L.cv[qb][d] * L.rsdti2;
             a[d] = a[d] + floc * share;
         }
     }
-    const int64_t g = L.grid[s];
+    const int64_t g = L.sgrid[s];
 #pragma unroll
     for (int d = 0; d < 3; d++) af[d * L.N + g] = a[d];
 }

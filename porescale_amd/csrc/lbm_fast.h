@@ -28,9 +28,8 @@ hipError_t density(const psx_lbm_dev::LbmConst &c, const float *in, const int32_
 hipError_t step(const psx_lbm_dev::LbmConst &c, const float *in, float *out, const int32_t *src,
                 const int32_t *vox, const Brick *bricks, int nbricks, const int32_t *hnode, const int16_t *hcell,
                 const int (&box)[3], int xcd, hipStream_t st);
-// Coupling force of the FP32 state (k_fluid_force); af [3][N].
-hipError_t fluid_force(const psx_lbm_dev::LbmConst &c, const float *pop, const psx_lbm_dev::CplLinks &L, double *af,
-                       hipStream_t st);
+// Per-link coupling term of the FP32 state (k_link_force).
+hipError_t link_force(const psx_lbm_dev::LbmConst &c, const float *pop, const psx_lbm_dev::CplLinks &L, hipStream_t st);
 // check_state partial sums (k_diag) of the pulled state.
 hipError_t diag(const psx_lbm_dev::LbmConst &c, const float *in, const int32_t *src, const int32_t *vox,
                 const double *phi, double *partial, hipStream_t st);

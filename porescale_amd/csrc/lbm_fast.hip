@@ -22,35 +22,43 @@ __host__ __device__ constexpr int cxq(int q) {
     return q == 1 || q == 7 || q == 9 || q == 11 || q == 13 ? 1 : q == 2 || q == 8 || q == 10 || q == 12 || q == 14 ? -1 : 0;
 }
 
-// Pull table in one of two forms.  I16: src = i + pb[cx_q + 1] + d with d
-// the int16 residual (rank difference of source and node within their
-// x-planes) and pb the per-plane bases of the node's plane; d = -32768 marks
-// a bounce-back link.  Otherwise the int32 source index (-1: bounce-back).
+// Pull table in one of three forms (template parameter M):
+//   M = 2: src = i + d, d the int16 offset (fits when an x-plane holds fewer
+//          than ~32k fluid nodes);
+//   M = 1: src = i + pb[cx_q + 1] + d, d the int16 rank difference of source
+//          and node within their x-planes and pb the per-plane bases of the
+//          node's plane;
+//   in both d = -32768 marks a bounce-back link;
+//   M = 0: the int32 source index (-1: bounce-back).
 struct PullIdx {
     const int32_t *__restrict__ src;
     const int16_t *__restrict__ src16;
     const int32_t *__restrict__ pbase;  // [3][nx]
 };
 
-template <bool I16>
+template <int M>
 __device__ __forceinline__ void load_bases(const LbmConst &c, const PullIdx &P, int32_t v, int32_t (&pb)[3]) {
-    if (I16) {
+    if (M == 1) {
         const int X = v / (c.ny * c.nz);
         pb[0] = P.pbase[X]; pb[1] = P.pbase[c.nx + X]; pb[2] = P.pbase[2 * c.nx + X];
     }
 }
 
 // slot (direction k, node j) that direction q of node i pulls from
-template <bool I16>
+template <int M>
 __device__ __forceinline__ void pull_from(const PullIdx &P, const int32_t (&pb)[3], int64_t nl, int64_t i, int q,
                                           int64_t &j, int &k) {
     j = i;
     k = q;
     if (q == 0) return;
-    if (I16) {
+    if (M == 1) {
         int d = P.src16[(int64_t)(q - 1) * nl + i];
         if (d == -32768) k = opp(q);
         else j = i + pb[cxq(q) + 1] + d;
+    } else if (M == 2) {
+        int d = P.src16[(int64_t)(q - 1) * nl + i];
+        if (d == -32768) k = opp(q);
+        else j = i + d;
     } else {
         int32_t s = P.src[(int64_t)(q - 1) * nl + i];
         if (s >= 0) j = s;
@@ -184,13 +192,13 @@ __device__ __forceinline__ void collide_fast(const LbmConst &c, double t1, doubl
 
 // Pull + collide + recolour of node i (voxel v) into its own slots of out.
 // fs(x, y, z, Cg, Fs) evaluates calcFs at the node's voxel.
-template <bool I16, class CalcFs>
+template <int M, class CalcFs>
 __device__ __forceinline__ void collide_pull(const LbmConst &c, const float *__restrict__ in,
                                              float *__restrict__ out, const PullIdx &P, int64_t i, int32_t v,
                                              const CalcFs &fs) {
     const int64_t nl = c.nl;
     int32_t pb[3];
-    load_bases<I16>(c, P, v, pb);
+    load_bases<M>(c, P, v, pb);
     double f[Q];
     int32_t at[Q];  // where direction q is pulled from (AoSoA index; < 2^31 for nl < 5.6e7)
     double rr = 0.0, rb = 0.0;
@@ -199,7 +207,7 @@ __device__ __forceinline__ void collide_pull(const LbmConst &c, const float *__r
     for (int q = 0; q < Q; q++) {
         int64_t j;
         int k;
-        pull_from<I16>(P, pb, nl, i, q, j, k);
+        pull_from<M>(P, pb, nl, i, q, j, k);
         at[q] = (int32_t)aos(k, j);
         double t = c.w0[q] + (double)in[at[q]];
         float r = in[at[q] + Q * AOS];
@@ -283,17 +291,17 @@ __device__ __forceinline__ void collide_pull(const LbmConst &c, const float *__r
 }
 
 // Colour field of node i from its pulled populations.
-template <bool I16>
+template <int M>
 __device__ __forceinline__ double colour_pull(const LbmConst &c, const float *__restrict__ in, const PullIdx &P,
                                               int64_t i, int32_t v) {
     int32_t pb[3];
-    load_bases<I16>(c, P, v, pb);
+    load_bases<M>(c, P, v, pb);
     double rr = 0.0, rb = 0.0;
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         int64_t j;
         int k;
-        pull_from<I16>(P, pb, c.nl, i, q, j, k);
+        pull_from<M>(P, pb, c.nl, i, q, j, k);
         int64_t a = aos(k, j);
         double t = c.w0[q] + (double)in[a];
         double rt = (double)in[a + Q * AOS] * t;
@@ -303,7 +311,7 @@ __device__ __forceinline__ double colour_pull(const LbmConst &c, const float *__
     return (rr - rb) / (rr + rb);
 }
 
-template <bool I16>
+template <int M>
 __global__ __launch_bounds__(BLOCK) void k_collide_f32(const LbmConst c, const float *__restrict__ in,
                                                        float *__restrict__ out, const PullIdx P,
                                                        const int32_t *__restrict__ vox,
@@ -312,10 +320,10 @@ __global__ __launch_bounds__(BLOCK) void k_collide_f32(const LbmConst c, const f
     int64_t i = b * blockDim.x + threadIdx.x;
     if (i >= c.nl) return;
     auto fs = [&](int x, int y, int z, double *Cg, double *Fs) { calc_fs<double, false>(c, phi, x, y, z, Cg, Fs); };
-    collide_pull<I16>(c, in, out, P, i, vox[i], fs);
+    collide_pull<M>(c, in, out, P, i, vox[i], fs);
 }
 
-template <bool I16>
+template <int M>
 __global__ __launch_bounds__(BLOCK) void k_density_f32(const LbmConst c, const float *__restrict__ in,
                                                        const PullIdx P, const int32_t *__restrict__ vox,
                                                        double *__restrict__ phi, int xcd) {
@@ -323,7 +331,7 @@ __global__ __launch_bounds__(BLOCK) void k_density_f32(const LbmConst c, const f
     int64_t i = b * blockDim.x + threadIdx.x;
     if (i >= c.nl) return;
     const int32_t v = vox[i];
-    phi[v] = colour_pull<I16>(c, in, P, i, v);
+    phi[v] = colour_pull<M>(c, in, P, i, v);
 }
 
 // One whole step over one brick of voxels per workgroup: the colour field of
@@ -347,7 +355,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_f32(const LbmConst c, const floa
     __syncthreads();
     for (int k = threadIdx.x; k < B.hcount; k += BLOCK) {
         const int64_t i = hnode[B.hstart + k];
-        phs[hcell[B.hstart + k]] = colour_pull<false>(c, in, P, i, vox[i]);
+        phs[hcell[B.hstart + k]] = colour_pull<0>(c, in, P, i, vox[i]);
     }
     __syncthreads();
     auto fs = [&](int x, int y, int z, double *Cg, double *Fs) {
@@ -357,7 +365,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_f32(const LbmConst c, const floa
     };
     for (int k = threadIdx.x; k < B.ncount; k += BLOCK) {
         const int64_t i = B.nstart + k;
-        collide_pull<false>(c, in, out, P, i, vox[i], fs);
+        collide_pull<0>(c, in, out, P, i, vox[i], fs);
     }
 }
 
@@ -369,12 +377,16 @@ hipError_t collide(const LbmConst &c, const float *in, float *out, const int32_t
                    const int32_t *pbase, const int32_t *vox, const double *phi, int xcd, hipStream_t st) {
     int64_t nb = psx::ceil_div(c.nl, BLOCK);
     if (nb == 0) return hipSuccess;
-    // the collide kernel is latency-bound (occupancy 3): it takes the int32
-    // table, whose source indices need no dependent load of the plane base
-    (void)src16;
-    (void)pbase;
-    const PullIdx P{src, nullptr, nullptr};
-    hipLaunchKernelGGL(k_collide_f32<false>, dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, out, P, vox, phi, xcd);
+    // the collide kernel is latency-bound (occupancy 3): it takes the direct
+    // int16 offsets when they exist, else the int32 table, whose source
+    // indices need no dependent load of a plane base
+    if (src16 && !pbase) {
+        const PullIdx P{src, src16, nullptr};
+        hipLaunchKernelGGL(k_collide_f32<2>, dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, out, P, vox, phi, xcd);
+    } else {
+        const PullIdx P{src, nullptr, nullptr};
+        hipLaunchKernelGGL(k_collide_f32<0>, dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, out, P, vox, phi, xcd);
+    }
     return hipGetLastError();
 }
 
@@ -383,10 +395,12 @@ hipError_t density(const LbmConst &c, const float *in, const int32_t *src, const
     int64_t nb = psx::ceil_div(c.nl, BLOCK);
     if (nb == 0) return hipSuccess;
     const PullIdx P{src, src16, pbase};
-    if (src16)
-        hipLaunchKernelGGL(k_density_f32<true>, dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, P, vox, phi, xcd);
+    if (src16 && pbase)
+        hipLaunchKernelGGL(k_density_f32<1>, dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, P, vox, phi, xcd);
+    else if (src16)
+        hipLaunchKernelGGL(k_density_f32<2>, dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, P, vox, phi, xcd);
     else
-        hipLaunchKernelGGL(k_density_f32<false>, dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, P, vox, phi, xcd);
+        hipLaunchKernelGGL(k_density_f32<0>, dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, P, vox, phi, xcd);
     return hipGetLastError();
 }
 
@@ -413,11 +427,11 @@ struct FOutF32 {
 };
 }  // namespace
 
-hipError_t fluid_force(const LbmConst &c, const float *pop, const CplLinks &L, double *af, hipStream_t st) {
-    if (L.ns == 0) return hipSuccess;
+hipError_t link_force(const LbmConst &c, const float *pop, const CplLinks &L, hipStream_t st) {
+    if (L.nlink == 0) return hipSuccess;
     FOutF32 f{pop, {}};
     for (int q = 0; q < Q; q++) f.w0[q] = c.w0[q];
-    hipLaunchKernelGGL(k_fluid_force<FOutF32>, dim3((unsigned)psx::ceil_div(L.ns, 256)), dim3(256), 0, st, L, f, af);
+    hipLaunchKernelGGL(k_link_force<FOutF32>, dim3((unsigned)psx::ceil_div(L.nlink, 256)), dim3(256), 0, st, L, f);
     return hipGetLastError();
 }
 

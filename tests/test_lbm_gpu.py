@@ -172,8 +172,9 @@ def test_fp32_state_round_trip_and_first_step():
 
 
 def test_fp32_compressed_pull_table_is_exact(monkeypatch):
-    """The int16 pull table (rank differences within x-planes) addresses
-    exactly the populations the int32 table does: states bit-identical."""
+    """The int16 pull tables (direct offsets, flag 1; rank differences within
+    x-planes, flag 2) address exactly the populations the int32 table does
+    (flag 0): states bit-identical."""
     ind = geo.sphere_pack((20, 14, 16), 0.4, 3.0, seed=17)
     nz, ny, nx = ind.shape
     inp = pio.LBMInput(B=0.7, sig=0.02, vir=0.1, vib=0.2, wet=0.3, ncx=nx, ncy=ny, ncz=nz, dt=1.0,
@@ -181,14 +182,15 @@ def test_fp32_compressed_pull_table_is_exact(monkeypatch):
     state = ob.equilibrium_state(inp, ob.fluid_order_mask(ind, geo.phase_blobs(ind, 0.5, 5, radius=2.5)))
     out = {}
     monkeypatch.setenv("PSX_LBM_FUSED", "0")  # the int16 table serves the two-kernel step
-    for flag in ("0", "1"):
+    for flag in ("0", "1", "2"):
         monkeypatch.setenv("PSX_LBM_IDX16", flag)
         _say(f"case fp32 pull table int16={flag}")
         with LBM(inp, ind, precision="f32") as lb:
             lb.set_state(*state)
             lb.step(40)
             out[flag] = lb.get_state()
-    assert np.array_equal(out["0"][0], out["1"][0]) and np.array_equal(out["0"][1], out["1"][1])
+    for flag in ("1", "2"):
+        assert np.array_equal(out["0"][0], out[flag][0]) and np.array_equal(out["0"][1], out[flag][1])
 
 
 @pytest.mark.parametrize("brick", ["8,8,16", "4,4,4", "3,5,7", "16,16,16"])
