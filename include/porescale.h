@@ -162,6 +162,18 @@ int psx_lsm_forces(psx_lsm *h, int32_t loadcase, const double *u, double *f);
 /* Reference solver: nsteps of the damped velocity-Verlet loop (XX:1377-1545)
    from the reference's initial state (XX:1366-1375) or the current one. */
 int psx_lsm_verlet(psx_lsm *h, int32_t loadcase, int64_t nsteps, int32_t restart);
+/* The node list of XX_P.f / YZ_P.f for a load case (XX:474-578: the grid
+   padded by 2 along the load axis, z outer / x fastest): count = ns; for
+   each reference node its periodic-grid node index and whether it is the far
+   twin (displacement = grid value + jump).  grid/twin may be NULL. */
+int psx_lsm_reference_nodes(psx_lsm *h, int32_t loadcase, int64_t *count, int64_t *grid, int8_t *twin);
+/* The Verlet state in the reference's node order: rdat, vdat, adat (3,ns),
+   the content of XX's iteration2 (written XX:1604-1609).  get needs a state
+   of that load case (after psx_lsm_verlet or set); set restarts from one as
+   icon = 1 does (XX:1288-1294), with the glued pairs reset to -+gstrain
+   (XX:1366-1375); psx_lsm_verlet(..., restart = 0) continues from it. */
+int psx_lsm_get_verlet(psx_lsm *h, int32_t loadcase, double *r, double *v, double *a);
+int psx_lsm_set_verlet(psx_lsm *h, int32_t loadcase, const double *r, const double *v, const double *a);
 /* Equilibrium solve K u = f_jump to ||r||/||b|| <= rtol (preconditioned CG;
    FP32 inner solves with FP64 refinement when precision = PSX_PREC_F32). */
 int psx_lsm_solve(psx_lsm *h, int32_t loadcase, double rtol, int32_t maxit,

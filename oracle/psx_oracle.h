@@ -109,6 +109,8 @@ void or_lsm_get_tables(const or_lsm *h, int32_t *bon, double *sbond1, double *sb
                        int32_t *ang, double *abond1, double *abond2);
 void or_lsm_get_state(const or_lsm *h, double *r, double *v, double *a);
 void or_lsm_set_state(or_lsm *h, const double *r, const double *v, const double *a);
+/* icon = 1: set_state, then the glued pairs back to -+gstrain (XX:1288-1294, 1366-1375) */
+void or_lsm_restart(or_lsm *h, const double *r, const double *v, const double *a);
 void or_lsm_forces(const or_lsm *h, const double *r, double *f); /* bonds+angles, XX:1404-1500 */
 void or_lsm_verlet(or_lsm *h, int64_t nsteps);
 /* calc_stress: asigmap(3,3) and astrain(3,3) column-major (sums over cubes),

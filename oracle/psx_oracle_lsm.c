@@ -394,6 +394,20 @@ void or_lsm_set_state(or_lsm *h, const double *r, const double *v, const double 
     if (a) memcpy(h->a, a, 3 * h->ns * sizeof(double));
 }
 
+/* icon = 1 (XX:1288-1294): state read from iteration2, then the glued pairs
+   set to -+gstrain along the jump component as XX:1366-1375 does after the
+   read. */
+void or_lsm_restart(or_lsm *h, const double *r, const double *v, const double *a) {
+    or_lsm_set_state(h, r, v, a);
+    for (int64_t q = 0; q < h->npair; q++) {
+        int64_t l1 = h->pairs[2 * q] - 1, l2 = h->pairs[2 * q + 1] - 1;
+        for (int d = 0; d < 3; d++) {
+            h->r[3 * l1 + d] = d == h->D ? -h->p.gstrain : 0.0;
+            h->r[3 * l2 + d] = d == h->D ? h->p.gstrain : 0.0;
+        }
+    }
+}
+
 /* bond + angle force loops (XX:1404-1500), two-colour layer order */
 void or_lsm_forces(const or_lsm *h, const double *r, double *f) {
     memset(f, 0, 3 * h->ns * sizeof(double));

@@ -1372,6 +1372,131 @@ extern "C" int psx_lsm_verlet(psx_lsm *h, int32_t lc, int64_t nsteps, int32_t re
     PSX_GUARD_END
 }
 
+namespace {
+// The node list of XX_P.f / YZ_P.f for load case lc (XX:474-578): the grid
+// padded by two layers along the load axis p, a node wherever one of its
+// (up to 8) voxels is solid, numbered z outer / x fastest.  Each reference
+// node maps to its periodic-grid node; the far layer (index n_p) is the twin
+// of layer 0, whose displacement is the grid value plus the jump.
+void reference_nodes(const psx_lsm *h, int lc, std::vector<int64_t> &grid, std::vector<int8_t> &twin) {
+    const int pa = CASE_PD[lc][0];
+    const int n[3] = {h->nx, h->ny, h->nz};
+    int G[3] = {n[0], n[1], n[2]};
+    G[pa] += 2;
+    grid.clear();
+    twin.clear();
+    for (int k = 0; k < G[2]; k++)
+        for (int j = 0; j < G[1]; j++)
+            for (int i = 0; i < G[0]; i++) {
+                const int c[3] = {i, j, k};
+                bool solid = false;
+                for (int corner = 0; corner < 8 && !solid; corner++) {
+                    int v[3];
+                    bool ok = true;
+                    for (int a = 0; a < 3; a++) {
+                        v[a] = c[a] - ((corner >> a) & 1);
+                        if (a == pa) ok = ok && v[a] >= 0 && v[a] < n[a];
+                        else v[a] = (v[a] % n[a] + n[a]) % n[a];
+                    }
+                    if (ok) solid = h->B.comp_h[v[0] + (int64_t)n[0] * (v[1] + (int64_t)n[1] * v[2])] != 0;
+                }
+                if (!solid) continue;
+                grid.push_back(c[0] % n[0] + (int64_t)n[0] * (c[1] % n[1] + (int64_t)n[1] * (c[2] % n[2])));
+                twin.push_back(c[pa] == n[pa] ? 1 : 0);
+            }
+}
+
+int verlet_buffers(psx_lsm *h) {
+    RC(ensure_solver(h, 1));
+    const int64_t n3 = 3 * h->B.Ntot;
+    if (!h->d_vu) {
+        RC(dalloc(&h->d_vu, n3)); RC(dalloc(&h->d_vv, n3)); RC(dalloc(&h->d_va, n3)); RC(dalloc(&h->d_vf, n3));
+    }
+    return PSX_OK;
+}
+}  // namespace
+
+extern "C" int psx_lsm_reference_nodes(psx_lsm *h, int32_t lc, int64_t *count, int64_t *grid, int8_t *twin) {
+    PSX_CHECK_ARG(h && count, "psx_lsm_reference_nodes: NULL argument");
+    RC(check_case(lc));
+    PSX_GUARD_BEGIN
+    std::vector<int64_t> g;
+    std::vector<int8_t> t;
+    reference_nodes(h, lc, g, t);
+    *count = (int64_t)g.size();
+    if (grid) std::memcpy(grid, g.data(), sizeof(int64_t) * g.size());
+    if (twin) std::memcpy(twin, t.data(), t.size());
+    return PSX_OK;
+    PSX_GUARD_END
+}
+
+extern "C" int psx_lsm_get_verlet(psx_lsm *h, int32_t lc, double *r, double *v, double *a) {
+    PSX_CHECK_ARG(h, "psx_lsm_get_verlet: NULL handle");
+    RC(check_case(lc));
+    if (h->verlet_case != lc) {
+        psx::set_error("psx_lsm_get_verlet: no Verlet state of load case %d (run psx_lsm_verlet or set it)", lc);
+        return PSX_ESTATE;
+    }
+    PSX_GUARD_BEGIN
+    std::vector<int64_t> g;
+    std::vector<int8_t> t;
+    reference_nodes(h, lc, g, t);
+    const int64_t N = h->B.Ntot;
+    const int dc = CASE_PD[lc][1];
+    const double J = 2.0 * h->p.gstrain;
+    std::vector<double> buf(3 * (size_t)N);
+    const double *dev[3] = {h->d_vu, h->d_vv, h->d_va};
+    double *outp[3] = {r, v, a};
+    for (int k = 0; k < 3; k++) {
+        if (!outp[k]) continue;
+        PSX_HIP(hipMemcpyAsync(buf.data(), dev[k], sizeof(double) * buf.size(), hipMemcpyDeviceToHost, h->stream));
+        PSX_HIP(hipStreamSynchronize(h->stream));
+        for (size_t s = 0; s < g.size(); s++)
+            for (int d = 0; d < 3; d++)
+                outp[k][3 * s + d] = buf[(size_t)d * N + g[s]] + (k == 0 && t[s] && d == dc ? J : 0.0);
+    }
+    return PSX_OK;
+    PSX_GUARD_END
+}
+
+extern "C" int psx_lsm_set_verlet(psx_lsm *h, int32_t lc, const double *r, const double *v, const double *a) {
+    PSX_CHECK_ARG(h && r && v && a, "psx_lsm_set_verlet: NULL argument");
+    RC(check_case(lc));
+    PSX_GUARD_BEGIN
+    RC(verlet_buffers(h));
+    std::vector<int64_t> g;
+    std::vector<int8_t> t;
+    reference_nodes(h, lc, g, t);
+    const int64_t N = h->B.Ntot;
+    const int dc = CASE_PD[lc][1];
+    const double J = 2.0 * h->p.gstrain, gs = h->p.gstrain;
+    std::vector<double> U(3 * (size_t)N, 0.0), V(3 * (size_t)N, 0.0), A(3 * (size_t)N, 0.0);
+    std::vector<int8_t> near((size_t)N, 0), far((size_t)N, 0);
+    for (int pass = 0; pass < 2; pass++)  // far twins first, so a near node's values win
+        for (size_t s = 0; s < g.size(); s++) {
+            if ((t[s] != 0) != (pass == 0)) continue;
+            (t[s] ? far : near)[(size_t)g[s]] = 1;
+            for (int d = 0; d < 3; d++) {
+                U[(size_t)d * N + g[s]] = r[3 * s + d] - (t[s] && d == dc ? J : 0.0);
+                V[(size_t)d * N + g[s]] = v[3 * s + d];
+                A[(size_t)d * N + g[s]] = a[3 * s + d];
+            }
+        }
+    // glued pairs restart at -+gstrain along the jump component (XX:1366-1375
+    // runs after the iteration2 read of XX:1288-1294)
+    for (int64_t n = 0; n < N; n++)
+        if (near[(size_t)n] && far[(size_t)n])
+            for (int d = 0; d < 3; d++) U[(size_t)d * N + n] = d == dc ? -gs : 0.0;
+    const double *src[3] = {U.data(), V.data(), A.data()};
+    double *dev[3] = {h->d_vu, h->d_vv, h->d_va};
+    for (int k = 0; k < 3; k++)
+        PSX_HIP(hipMemcpyAsync(dev[k], src[k], sizeof(double) * U.size(), hipMemcpyHostToDevice, h->stream));
+    PSX_HIP(hipStreamSynchronize(h->stream));
+    h->verlet_case = lc;
+    return PSX_OK;
+    PSX_GUARD_END
+}
+
 extern "C" int psx_lsm_solve(psx_lsm *h, int32_t lc, double rtol, int32_t maxit, int32_t *iters, double *relres) {
     PSX_CHECK_ARG(h && rtol > 0.0 && maxit > 0, "psx_lsm_solve: bad argument");
     RC(check_case(lc));

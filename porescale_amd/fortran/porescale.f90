@@ -199,6 +199,32 @@ module porescale
        integer(c_int32_t), value :: restart
      end function psx_lsm_verlet
 
+     function psx_lsm_reference_nodes(h, loadcase, count, grid, twin) bind(C, name='psx_lsm_reference_nodes')
+       import :: c_int, c_int32_t, c_int64_t, c_ptr
+       integer(c_int) :: psx_lsm_reference_nodes
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: loadcase
+       integer(c_int64_t) :: count
+       type(c_ptr), value :: grid, twin
+     end function psx_lsm_reference_nodes
+
+     ! iteration2 state (rdat, vdat, adat as (3,ns)) of the Verlet loop
+     function psx_lsm_get_verlet(h, loadcase, r, v, a) bind(C, name='psx_lsm_get_verlet')
+       import :: c_int, c_int32_t, c_double, c_ptr
+       integer(c_int) :: psx_lsm_get_verlet
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: loadcase
+       real(c_double) :: r(*), v(*), a(*)
+     end function psx_lsm_get_verlet
+
+     function psx_lsm_set_verlet(h, loadcase, r, v, a) bind(C, name='psx_lsm_set_verlet')
+       import :: c_int, c_int32_t, c_double, c_ptr
+       integer(c_int) :: psx_lsm_set_verlet
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: loadcase
+       real(c_double) :: r(*), v(*), a(*)
+     end function psx_lsm_set_verlet
+
      function psx_lsm_solve(h, loadcase, rtol, maxit, iters, relres) bind(C, name='psx_lsm_solve')
        import :: c_int, c_int32_t, c_double, c_ptr
        integer(c_int) :: psx_lsm_solve

@@ -12,7 +12,9 @@
 !     mode  solve  (default) preconditioned CG to the relative residual of
 !                  line 17 (default 1e-10), then one stress_tensor block
 !           verlet the reference's damped Verlet relaxation: itm blocks of
-!                  its steps, a stress_tensor block after each (XX_P.f:1377-1545)
+!                  its steps, a stress_tensor block and the iteration2 state
+!                  after each (XX_P.f:1377-1545, 1604-1609); icon = 1 restarts
+!                  from iteration2 (XX_P.f:1288-1294)
 !   case all solves the six cases and writes the 6x6 stiffness and the Voigt
 !   bulk and shear moduli to the file 'moduli'.
 !
@@ -29,7 +31,9 @@ program psx_lsm_drv
   real(8) :: asig(3, 3), moduli(3, 3), C(6, 6)
   integer(c_int32_t) :: iters
   real(8) :: relres
-  integer(c_int64_t) :: counts(4)
+  integer(c_int64_t) :: counts(4), nref
+  real(c_double), allocatable :: rr(:, :), vv(:, :), aa(:, :)
+  integer :: j
   integer(c_int32_t), allocatable :: ind(:, :, :)
   integer(c_int32_t), allocatable, target :: cbon(:), cang(:)
   real(c_double), allocatable, target :: csb(:), cab(:)
@@ -127,11 +131,32 @@ program psx_lsm_drv
      call psx_check(psx_lsm_stress(h, int(lc, c_int32_t), asig, moduli), 'psx_lsm_stress')
      call write_block(0, 0)
   case ('verlet')
+     call psx_check(psx_lsm_reference_nodes(h, int(lc, c_int32_t), nref, c_null_ptr, c_null_ptr), &
+          'psx_lsm_reference_nodes')
+     allocate(rr(3, max(nref, 1_c_int64_t)), vv(3, max(nref, 1_c_int64_t)), aa(3, max(nref, 1_c_int64_t)))
+     if (icon == 1) then
+        open(newunit=u, file='iteration2', status='old', iostat=ios)
+        if (ios /= 0) then
+           write(*, *) 'ERROR! icon = 1 needs iteration2'
+           stop 1
+        end if
+        read(u, *) ((rr(j, i), j = 1, 3), i = 1, nref)
+        read(u, *) ((vv(j, i), j = 1, 3), i = 1, nref)
+        read(u, *) ((aa(j, i), j = 1, 3), i = 1, nref)
+        close(u)
+        call psx_check(psx_lsm_set_verlet(h, int(lc, c_int32_t), rr, vv, aa), 'psx_lsm_set_verlet')
+     end if
      do iti = 1, itm
         call psx_check(psx_lsm_verlet(h, int(lc, c_int32_t), int(its, c_int64_t), &
-             merge(1_c_int32_t, 0_c_int32_t, iti == 1)), 'psx_lsm_verlet')
+             merge(1_c_int32_t, 0_c_int32_t, iti == 1 .and. icon /= 1)), 'psx_lsm_verlet')
         call psx_check(psx_lsm_stress(h, int(lc, c_int32_t), asig, moduli), 'psx_lsm_stress')
         call write_block(iti, its)
+        call psx_check(psx_lsm_get_verlet(h, int(lc, c_int32_t), rr, vv, aa), 'psx_lsm_get_verlet')
+        open(newunit=u, file='iteration2', status='replace')
+        write(u, *) ((rr(j, i), j = 1, 3), i = 1, nref)
+        write(u, *) ((vv(j, i), j = 1, 3), i = 1, nref)
+        write(u, *) ((aa(j, i), j = 1, 3), i = 1, nref)
+        close(u)
      end do
   case default
      write(*, *) 'ERROR! mode must be solve or verlet'

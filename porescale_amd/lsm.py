@@ -166,6 +166,30 @@ class LSM:
         """nsteps of the reference's damped velocity-Verlet loop."""
         _lib.check(self._L.psx_lsm_verlet(self._h, _case(case), int(nsteps), 1 if restart else 0))
 
+    def reference_nodes(self, case):
+        """(grid index, far-twin flag) of every node of the reference's node
+        list for ``case`` (XX:474-578)."""
+        n = C.c_int64(0)
+        _lib.check(self._L.psx_lsm_reference_nodes(self._h, _case(case), C.byref(n), None, None))
+        grid = np.zeros(n.value, np.int64)
+        twin = np.zeros(n.value, np.int8)
+        _lib.check(self._L.psx_lsm_reference_nodes(self._h, _case(case), C.byref(n), _lib.ptr(grid), _lib.ptr(twin)))
+        return grid, twin.astype(bool)
+
+    def get_verlet(self, case):
+        """rdat, vdat, adat (ns, 3) of the Verlet state, reference node order
+        (what XX writes to iteration2)."""
+        ns = len(self.reference_nodes(case)[0])
+        r, v, a = (np.zeros((ns, 3)) for _ in range(3))
+        _lib.check(self._L.psx_lsm_get_verlet(self._h, _case(case), *(_lib.ptr(x) for x in (r, v, a))))
+        return r, v, a
+
+    def set_verlet(self, case, r, v, a):
+        """Restart the Verlet loop from an iteration2 state (icon = 1); the
+        glued pairs restart at -+gstrain as in XX:1366-1375."""
+        arrs = [np.ascontiguousarray(x, np.float64) for x in (r, v, a)]
+        _lib.check(self._L.psx_lsm_set_verlet(self._h, _case(case), *(_lib.ptr(x) for x in arrs)))
+
     def solve(self, case, rtol: float = 1e-10, maxit: int = 20000):
         it, rel = C.c_int32(0), C.c_double(0)
         _lib.check(self._L.psx_lsm_solve(self._h, _case(case), float(rtol), int(maxit), C.byref(it), C.byref(rel)))

@@ -21,6 +21,8 @@ Fixtures (compressed .npz, inputs and reference outputs only):
   lsm_xx.npz      XX_P.f, two-component 10x12x14 pack, 100 Verlet steps,
                   nthread=3: r, v, a and stress_tensor moduli.
   lsm_yz.npz      YZ_P.f, same geometry, nthread=1.
+  lsm_xx_restart.npz, lsm_yz_restart.npz  icon=1 from the state above
+                  (iteration2), 50 more steps: r, v, a and the moduli.
   ref_tables.npz  the reference's velocities19/velocities9 and cube_* files
                   as read from /root/reference (data tables of the API).
 """
@@ -95,6 +97,11 @@ def main():
         mods = ob.parse_stress_tensor(st)
         np.savez_compressed(os.path.join(HERE, f"lsm_{case}.npz"), r=r, v=v, a=a, steps=100, nthread=nth,
                             **{k: np.float64(v_) for k, v_ in mods.items()})
+        # icon = 1 from that state: 1 block of 50 more steps
+        inp1 = pio.LSMInput(ncx=nx, ncy=ny, ncz=nz, nthread=nth, **dict(LSM_INP, itm=1))
+        r1, v1, a1, st1 = ob.run_ref_lsm(case, ind, inp1, restart=(r, v, a))
+        np.savez_compressed(os.path.join(HERE, f"lsm_{case}_restart.npz"), r=r1, v=v1, a=a1, steps=50,
+                            **{k: np.float64(v_) for k, v_ in ob.parse_stress_tensor(st1).items()})
     ref = "/root/reference/LSM2S/Cxxxx"
     b, an, sb, ab = pio.read_cube_tables(ref)
     np.savez_compressed(os.path.join(HERE, "ref_tables.npz"), velocities19=pio.read_velocities(

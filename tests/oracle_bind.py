@@ -93,6 +93,7 @@ def lib():
         L.or_lsm_anf.argtypes = [_P(np.float64), _P(np.int32)]
         L.or_lbm_fluid_force.argtypes = [vp, C.c_double, _P(np.float64)]
         L.or_lsm_coupled_step.argtypes = [vp, _P(np.float64), C.c_double]
+        L.or_lsm_restart.argtypes = [vp, vp, vp, vp]
         L.or_lsm_calc_ab.argtypes = [vp, _P(np.int32), C.c_int32, C.c_int32, C.c_double, _P(np.float64)]
         _lib = L
     return _lib
@@ -263,6 +264,11 @@ class OracleLSM:
     def set_state(self, r=None, v=None, a=None):
         conv = [None if x is None else np.ascontiguousarray(x, np.float64) for x in (r, v, a)]
         self.L.or_lsm_set_state(self.h, *(_ptr(x) for x in conv))
+
+    def restart(self, r, v, a):
+        """icon = 1: state from iteration2, glued pairs reset (XX:1366-1375)."""
+        self._rs = [np.ascontiguousarray(x, np.float64) for x in (r, v, a)]
+        self.L.or_lsm_restart(self.h, *(_ptr(x) for x in self._rs))
 
     def forces(self, r):
         f = np.zeros((self.ns, 3))

@@ -133,11 +133,40 @@ def test_lsm_driver_verlet_matches_reference_golden():
                             pio.LSMInput(ncx=nx, ncy=ny, ncz=nz, nthread=3, **LSM_INP))
         _run([_drv("psx_lsm_drv"), "xx", "verlet"], d)
         st = open(os.path.join(d, "stress_tensor")).read()
+        r, v, a = pio.read_lsm_restart(os.path.join(d, "iteration2"), g["r"].shape[0])
     finally:
         shutil.rmtree(d, ignore_errors=True)
     m = ob.parse_stress_tensor(st)
     assert m["Exxxx"] == pytest.approx(float(g["Exxxx"]), rel=1e-11)
     assert m["Eyyxx"] == pytest.approx(float(g["Eyyxx"]), rel=1e-10)
+    for x, key in ((r, "r"), (v, "v"), (a, "a")):
+        np.testing.assert_allclose(x, g[key], rtol=0, atol=1e-13 * np.abs(g[key]).max())
+
+
+@pytest.mark.parametrize("case", ["xx", "yz"])
+def test_lsm_driver_restart_matches_reference(case):
+    """icon = 1: the driver reads the reference's iteration2 and continues
+    as XX_P.f / YZ_P.f do (glued pairs reset); its iteration2 and moduli after
+    50 steps match the reference's restart run."""
+    ind = lsm_geometry()
+    nz, ny, nx = ind.shape
+    g = np.load(os.path.join(GOLD, f"lsm_{case}.npz"))
+    g1 = np.load(os.path.join(GOLD, f"lsm_{case}_restart.npz"))
+    d = _prep(ind)
+    try:
+        inp = pio.LSMInput(ncx=nx, ncy=ny, ncz=nz, nthread=int(g["nthread"]), **dict(LSM_INP, itm=1, icon=1))
+        pio.write_lsm_input(os.path.join(d, "INPUT_PARAMETERS"), inp)
+        pio.write_lsm_restart(os.path.join(d, "iteration2"), g["r"], g["v"], g["a"])
+        _run([_drv("psx_lsm_drv"), case, "verlet"], d)
+        st = open(os.path.join(d, "stress_tensor")).read()
+        r, v, a = pio.read_lsm_restart(os.path.join(d, "iteration2"), g1["r"].shape[0])
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    for x, key in ((r, "r"), (v, "v"), (a, "a")):
+        np.testing.assert_allclose(x, g1[key], rtol=0, atol=1e-12 * np.abs(g1[key]).max())
+    m = ob.parse_stress_tensor(st)
+    key = "Exxxx" if case == "xx" else "Eyzyz"
+    assert m[key] == pytest.approx(float(g1[key]), rel=1e-10)
 
 
 def test_lsm_driver_all_cases_closed_form():

@@ -64,6 +64,31 @@ def test_verlet_trajectory_matches_golden_reference(case):
         assert m[1, 2] == pytest.approx(float(g["Eyzyz"]), rel=1e-11)
 
 
+@pytest.mark.parametrize("case", ["xx", "yz"])
+def test_verlet_restart_in_reference_node_order(case):
+    """The state in the reference's node order (iteration2): after 100 steps
+    it is the reference's own iteration2; restarting from it (icon = 1, glued
+    pairs reset) and running 50 more steps gives the reference's restart run."""
+    ind = lsm_geometry()
+    g = np.load(os.path.join(GOLD, f"lsm_{case}.npz"))
+    g1 = np.load(os.path.join(GOLD, f"lsm_{case}_restart.npz"))
+    a1, b1 = spring_constants(LSM_INP["Em1"], LSM_INP["nu1"])
+    a2, b2 = spring_constants(LSM_INP["Em2"], LSM_INP["nu2"])
+    _say(f"case verlet restart {case}")
+    with LSM(ind, a1, b1, a2, b2, rs=1.0, dt=0.01, gstrain=0.01) as s:
+        grid, twin = s.reference_nodes(case)
+        assert len(grid) == g["r"].shape[0]
+        s.verlet(case, int(g["steps"]))
+        r, v, a = s.get_verlet(case)
+        for x, key in ((r, "r"), (v, "v"), (a, "a")):
+            np.testing.assert_allclose(x, g[key], rtol=0, atol=1e-13 * np.abs(g[key]).max())
+        s.set_verlet(case, g["r"], g["v"], g["a"])
+        s.verlet(case, int(g1["steps"]), restart=False)
+        r, v, a = s.get_verlet(case)
+    for x, key in ((r, "r"), (v, "v"), (a, "a")):
+        np.testing.assert_allclose(x, g1[key], rtol=0, atol=1e-12 * np.abs(g1[key]).max())
+
+
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_all_solid_closed_form(prec):
     a, b = 1.0, 0.1

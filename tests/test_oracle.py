@@ -71,6 +71,22 @@ def test_lsm_verlet_bitexact_vs_reference(case):
         assert m[1, 2] == pytest.approx(float(g["Eyzyz"]), rel=1e-14)
 
 
+@pytest.mark.parametrize("case", ["xx", "yz"])
+def test_lsm_restart_bitexact_vs_reference(case):
+    """icon = 1 from iteration2 (XX:1288-1294): the glued pairs are reset to
+    -+gstrain after the read (XX:1366-1375); 50 more steps, bit-exact."""
+    ind = lsm_geometry()
+    g, g1 = _gold(f"lsm_{case}.npz"), _gold(f"lsm_{case}_restart.npz")
+    a1, b1 = spring_constants(LSM_INP["Em1"], LSM_INP["nu1"])
+    a2, b2 = spring_constants(LSM_INP["Em2"], LSM_INP["nu2"])
+    o = ob.OracleLSM(ind, case, alf=a1, bet=b1, alf2=a2, bet2=b2, rs=1.0, dt=0.01, gstrain=0.01,
+                     nthread=int(g["nthread"]))
+    o.restart(g["r"], g["v"], g["a"])
+    o.verlet(int(g1["steps"]))
+    r, v, a = o.state()
+    assert np.array_equal(r, g1["r"]) and np.array_equal(v, g1["v"]) and np.array_equal(a, g1["a"])
+
+
 def test_lsm_tables_counts_consistent():
     ind = lsm_geometry()
     o = ob.OracleLSM(ind, "xx", alf=1.0, bet=0.1)
