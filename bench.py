@@ -142,10 +142,10 @@ def cpu_baseline(ind, inp, blue, budget_s=12.0):
             "sample": f"{n} steps of the same 256^3 two-phase MRT workload, oracle in C, 1 thread"}
 
 
-def _plain_rate(ind, cases, rtol, pars=(1.0, 0.1), maxit=200000):
+def _plain_rate(ind, cases, rtol, pars=(1.0, 0.1), maxit=200000, stiff=None):
     """Plain FP64 GPU path: block-Jacobi PCG, one load case per solve."""
     from porescale_amd.lsm import LSM
-    with LSM(ind, *pars, precision="f64") as s:
+    with LSM(ind, *pars, precision="f64", stiff=stiff) as s:
         s.solve(cases[0], rtol, maxit)  # warm-up (allocation, graph build)
         t0 = time.perf_counter()
         its = [s.solve(c, rtol, maxit)[0] for c in cases]
@@ -218,6 +218,12 @@ def run_lsm(full=False):
             t_a = time.perf_counter() - t0
         adj[prec] = {"forward_s": t_f, "adjoint_s": t_a, "grads_per_s": 1.0 / (t_f + t_a), "K": K5, "G": G5}
     adj["speedup"] = adj["f32"]["grads_per_s"] / adj["f64"]["grads_per_s"]
+    # plain FP64 (one load case per block-Jacobi PCG solve) on the forward
+    # moduli; its adjoint has the same six systems, so a plain FP64 gradient
+    # costs at least the batched FP64 one above
+    t_plain, its5 = _plain_rate(ind5, range(6), 1e-10, pars, stiff=stiff)
+    adj["plain_fp64_forward"] = {"seconds": t_plain, "solves_per_s": 6 / t_plain, "iters": its5,
+                                 "speedup_f32_forward": t_plain / adj["f32"]["forward_s"]}
     steps = 50 if full else 5
     tgt = np.array([adj["f32"]["K"], adj["f32"]["G"]]) * np.array([1.05, 0.97])
     t0 = time.perf_counter()

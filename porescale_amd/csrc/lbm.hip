@@ -403,8 +403,9 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
         build_bricks(h, bd, vox, bricks, hnode, hcell);
     }
     // Compressed pull table (FP32 path), 36 B per node instead of 72:
-    //  - direct: src = n + d with d an int16 offset; fits when an x-plane
-    //    holds fewer than ~32k fluid nodes (256^3 at porosity 0.3: 19.7k);
+    //  - direct: src = n + d with d an int16 offset; fits when every link
+    //    spans fewer than 32k fluid nodes (a plane and a row for the x
+    //    links, so small and thin grids);
     //  - plane-based otherwise: the source of direction q lies in plane
     //    X - c_qx; its index is n + pbase[c_qx + 1][X] + d with d the
     //    difference of the two nodes' ranks within their planes, which fits
