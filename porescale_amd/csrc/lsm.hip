@@ -177,6 +177,7 @@ __global__ __launch_bounds__(256) void k_sys_reduce(const VolDesc *__restrict__ 
         st.rr = sb;
         st.bb = sb;
         st.iters = 0;
+        st.tol2 = 0.0;
         st.conv = sb <= 0.0 || sb <= NOISE2 * st.bref;
         st.active = !st.conv;
         return;
@@ -191,7 +192,8 @@ __global__ __launch_bounds__(256) void k_sys_reduce(const VolDesc *__restrict__ 
         st.rz = sa;
         st.rr = sb;
         st.iters += 1;
-        if (sb <= tol2 * st.bb || sb <= NOISE2 * st.bref) { st.conv = 1; st.active = 0; }
+        const double t2 = st.tol2 > 0.0 ? st.tol2 : tol2;
+        if (sb <= t2 * st.bb || sb <= NOISE2 * st.bref) { st.conv = 1; st.active = 0; }
         else if (st.iters >= maxit || !(sb == sb) || sb > 1e12 * st.bb) st.active = 0;
     }
 }
@@ -429,11 +431,25 @@ __global__ __launch_bounds__(256) void k_set_bref(const VolDesc *__restrict__ vo
 }
 
 // Inner FP32 CG systems follow the outer activity.
-__global__ void k_inner_from_outer(int S, const SysState *__restrict__ outer, SysState *__restrict__ inner) {
+// Inner system state from the outer one.  Each system's inner tolerance
+// splits the reduction its outer residual still needs to reach tol evenly
+// over the fewest passes of a nominal reduction `nominal` each, at least
+// floor (what the FP32 solve reaches).  A last pass that needs less than
+// one nominal reduction is asked for just that, so a small remainder costs
+// a few iterations rather than a full pass.
+__global__ void k_inner_from_outer(int S, const SysState *__restrict__ outer, SysState *__restrict__ inner,
+                                   double tol, double floor_, double nominal) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < S) {
-        inner[s].active = outer[s].active;
+        const SysState &o = outer[s];
+        inner[s].active = o.active;
         inner[s].bref = 0.0;  // the inner rhs is the scaled residual
+        const double need = (o.rr > 0.0 && o.bb > 0.0) ? fmin(1.0, tol / sqrt(o.rr / o.bb)) : 1.0;
+        // passes needed, with a tenth of a pass of slack: 0.95e-5 still
+        // counts as one nominal pass
+        const double k = fmax(1.0, ceil(log(need) / log(nominal) - 0.1));
+        const double pt = fmin(0.5, fmax(floor_, pow(need, 1.0 / k)));
+        inner[s].tol2 = pt * pt;
     }
 }
 
@@ -1041,8 +1057,13 @@ int solve_batch(hipStream_t st, const Batch &B, Solver &S, bool f32, double tol,
     int gb = (int)psx::ceil_div(n, 256);
     int g = grid_for(B.ntiles);
     hipLaunchKernelGGL(k_zero<double>, dim3(gb), dim3(256), 0, st, S.X, n);
-    double inner_tol = 1e-5;
+    // Inner tolerance of each refinement pass, per system (k_inner_from_outer):
+    // passes of a nominal 1e-5 reduction, the last one asked for what is
+    // left.  A pass of the FP32 solve asked for 1e-5 reduces the true
+    // residual by 0.85e-5 to 1.2e-5 (64^3 and 128^3 packs).
+    double inner_tol = 1e-5, inner_floor = 1e-6;
     if (const char *e = std::getenv("PSX_LSM_INNER_TOL")) inner_tol = std::atof(e);
+    if (const char *e = std::getenv("PSX_LSM_INNER_FLOOR")) inner_floor = std::min(std::atof(e), inner_tol);
     // multigrid-preconditioned inner solves (PSX_MG=0 selects block Jacobi)
     const char *mge = std::getenv("PSX_MG");
     const bool use_mg = !mge || std::atoi(mge) != 0;
@@ -1069,7 +1090,7 @@ int solve_batch(hipStream_t st, const Batch &B, Solver &S, bool f32, double tol,
                            S.outer, B.d_ka, B.d_kb, S.R32, S.X32, S.Rr, B.Ntot);
         cg_start<float>(st, B, S, S.sys, S.X32, S.R32, S.Z32, S.P32, S.R32, B.d_minv32, use_mg);
         hipLaunchKernelGGL(k_inner_from_outer, dim3((unsigned)psx::ceil_div(S.S, 64)), dim3(64), 0, st, S.S, S.outer,
-                           S.sys);
+                           S.sys, tol, inner_floor, inner_tol);
         PSX_HIP(hipGetLastError());
         RC(cg_run<float>(st, B, S, S.sys, S.X32, S.R32, S.Z32, S.P32, S.Q32, B.d_ka32, B.d_kb32, B.d_minv32,
                          inner_tol, maxit, chunk, use_graph, use_mg, maxit - total_iters));
@@ -1080,8 +1101,9 @@ int solve_batch(hipStream_t st, const Batch &B, Solver &S, bool f32, double tol,
         total_iters += mx;
         if (std::getenv("PSX_MG_DEBUG"))
             for (int k = 0; k < S.S; k++)
-                std::fprintf(stderr, "outer %d sys %d: outer rr %.3e (bb %.3e bref %.3e) active %d | inner iters %d rr %.3e conv %d\n",
-                             outer, k, ho[k].rr, ho[k].bb, ho[k].bref, ho[k].active, hs[k].iters, hs[k].rr, hs[k].conv);
+                std::fprintf(stderr, "outer %d sys %d: outer rr %.3e (bb %.3e bref %.3e) active %d | inner tol %.1e iters %d rr %.3e conv %d\n",
+                             outer, k, ho[k].rr, ho[k].bb, ho[k].bref, ho[k].active, std::sqrt(hs[k].tol2),
+                             hs[k].iters, hs[k].rr, hs[k].conv);
         hipLaunchKernelGGL(k_unscale_add, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles, S.R,
                            S.outer, S.X, S.X32, B.Ntot);
         PSX_HIP(hipGetLastError());

@@ -30,6 +30,7 @@ struct SysState {
     double bref;  // squared norm of the rhs summed without cancellation (0 = unused):
                   // a residual below 1e-14 of it is rounding noise and counts as converged
     double mean[3];  // outer refinement: mean residual over the nodes with springs
+    double tol2;     // inner solves: squared relative tolerance of this system (0 = the launch's)
     int32_t active, iters, conv, pad2_;
 };
 constexpr double NOISE2 = 1e-28;
