@@ -1169,14 +1169,25 @@ extern "C" int psx_cpl_create(psx_lbm *lbm, double Em, double nu, double dl, dou
         for (int m = 0; m < (li < n2 ? 2 : 4); m++)
             if (e[2 + m] > 0) cnt[(size_t)e[2 + m] - 1]++;  // id 0: a wall voxel of another pore family
     }
-    // surface solid nodes (those with links), their contribution ranges
-    std::vector<int32_t> sidx((size_t)c->ns, -1), sgrid, start(1, 0);
+    // surface solid nodes (those with links), their contribution ranges; a
+    // node's links sit next to it in the fluid numbering (x outer, z
+    // fastest), so the surface list follows that order too and neighbouring
+    // threads read neighbouring per-link terms
+    std::vector<int32_t> order;
     for (int64_t s = 0; s < c->ns; s++)
-        if (cnt[(size_t)s] > 0) {
-            sidx[(size_t)s] = (int32_t)sgrid.size();
-            sgrid.push_back(c->grid[(size_t)s]);
-            start.push_back(start.back() + cnt[(size_t)s]);
-        }
+        if (cnt[(size_t)s] > 0) order.push_back((int32_t)s);
+    auto fluid_key = [&](int32_t s) {
+        const int64_t g = c->grid[(size_t)s];
+        const int64_t x = g % nx, y = (g / nx) % ny, z = g / ((int64_t)nx * ny);
+        return (x * ny + y) * (int64_t)nz + z;
+    };
+    std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return fluid_key(a) < fluid_key(b); });
+    std::vector<int32_t> sidx((size_t)c->ns, -1), sgrid, start(1, 0);
+    for (int32_t s : order) {
+        sidx[(size_t)s] = (int32_t)sgrid.size();
+        sgrid.push_back(c->grid[(size_t)s]);
+        start.push_back(start.back() + cnt[(size_t)s]);
+    }
     const int64_t nsurf = (int64_t)sgrid.size();
     std::vector<uint32_t> contrib((size_t)std::max<int32_t>(start.back(), 1));
     std::vector<int32_t> fill(start.begin(), start.end() - 1);

@@ -104,6 +104,41 @@ __global__ __launch_bounds__(TILE) void k_apply(const VolDesc *__restrict__ vols
     }
 }
 
+// Spring forces F = -K u (one system, no jump) with the second half of
+// lsm_step fused in (L1:1880-1898): a = F rsi - damp v; v += a dt/2 + af dt/2.
+__global__ __launch_bounds__(TILE) void k_apply_cverlet(const VolDesc *__restrict__ vols,
+                                                        const int32_t *__restrict__ tvol,
+                                                        const int32_t *__restrict__ tstart, int ntiles,
+                                                        const double *__restrict__ ka, const double *__restrict__ kb,
+                                                        const double *__restrict__ u, double *__restrict__ V,
+                                                        double *__restrict__ A, const double *__restrict__ AF,
+                                                        int64_t Ntot, double rsi, double damp, double dt) {
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const VolDesc vd = vols[tvol[t]];
+        int x, yy, z;
+        bool ok;
+        tile_node(vd, tstart[t], x, yy, z, ok);
+        if (!ok) continue;
+        const int64_t n = vd.off + x + (int64_t)vd.nx * (yy + (int64_t)vd.ny * z);
+        double a[8], b[8];
+        double f[3] = {0.0, 0.0, 0.0};
+        if (gather_k(vd, ka, kb, x, yy, z, a, b)) {
+            double uu[27][3];
+            const double J[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+            gather27<double, false>(vd, u, Ntot, x, yy, z, J, uu);
+            lsm_el::node_force<double>(uu, a, b, f);
+        }
+#pragma unroll
+        for (int e = 0; e < 3; e++) {
+            const int64_t i = (int64_t)e * Ntot + n;
+            double acc = f[e] * rsi;
+            acc = acc - damp * V[i];
+            A[i] = acc;
+            V[i] = V[i] + acc * dt * 0.5 + AF[i] * dt * 0.5;
+        }
+    }
+}
+
 // Block-Jacobi preconditioner: inverse of each node's 3x3 diagonal block
 // (symmetric, stored as 6 values xx,yy,zz,xy,xz,yz); zero for nodes without
 // springs.
@@ -526,8 +561,8 @@ __global__ __launch_bounds__(TILE) void k_verlet_a(int64_t n3, double *__restric
     U[i] = U[i] + v * dt;               // XX:1398-1400
 }
 
-// lsm_step of the coupled model (L1:1758-1903): the fluid force AF enters
-// both half kicks; damping `damp` (0.25 there)
+// lsm_step of the coupled model (L1:1758-1903), first half: the fluid force
+// AF enters both half kicks (the second is fused into k_apply_cverlet)
 __global__ __launch_bounds__(TILE) void k_cverlet_a(int64_t n3, double *__restrict__ U, double *__restrict__ V,
                                                     const double *__restrict__ A, const double *__restrict__ AF,
                                                     double dt) {
@@ -536,17 +571,6 @@ __global__ __launch_bounds__(TILE) void k_cverlet_a(int64_t n3, double *__restri
     double v = V[i] + A[i] * dt * 0.5 + AF[i] * dt * 0.5;  // L1:1762-1766
     V[i] = v;
     U[i] = U[i] + v * dt;                                  // L1:1775-1777
-}
-
-__global__ __launch_bounds__(TILE) void k_cverlet_b(int64_t n3, const double *__restrict__ F, double *__restrict__ V,
-                                                    double *__restrict__ A, const double *__restrict__ AF,
-                                                    double rsi, double damp, double dt) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n3) return;
-    double a = F[i] * rsi;       // L1:1880
-    a = a - damp * V[i];         // L1:1888
-    A[i] = a;
-    V[i] = V[i] + a * dt * 0.5 + AF[i] * dt * 0.5;  // L1:1897-1898
 }
 
 __global__ __launch_bounds__(TILE) void k_verlet_b(int64_t n3, const double *__restrict__ F, double *__restrict__ V,
@@ -1275,9 +1299,9 @@ int step(psx_lsm *h, const double *d_af, double damp, hipStream_t st) {
     const double dt = h->p.dt, rsi = 1.0 / h->p.rs;
     const int gb = (int)psx::ceil_div(n3, 256);
     hipLaunchKernelGGL(k_cverlet_a, dim3(gb), dim3(256), 0, st, n3, h->d_vu, h->d_vv, h->d_va, d_af, dt);
-    launch_apply<double, false, true, false>(st, h->B, 1, h->B.d_ka, h->B.d_kb, h->d_vu, h->d_vf, nullptr, nullptr,
-                                             nullptr);
-    hipLaunchKernelGGL(k_cverlet_b, dim3(gb), dim3(256), 0, st, n3, h->d_vf, h->d_vv, h->d_va, d_af, rsi, damp, dt);
+    const Batch &B = h->B;
+    hipLaunchKernelGGL(k_apply_cverlet, dim3(grid_for(B.ntiles)), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart,
+                       B.ntiles, B.d_ka, B.d_kb, h->d_vu, h->d_vv, h->d_va, d_af, B.Ntot, rsi, damp, dt);
     PSX_HIP(hipGetLastError());
     return PSX_OK;
 }
