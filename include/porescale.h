@@ -164,9 +164,11 @@ int psx_lsm_forces(psx_lsm *h, int32_t loadcase, const double *u, double *f);
 int psx_lsm_verlet(psx_lsm *h, int32_t loadcase, int64_t nsteps, int32_t restart);
 /* The node list of XX_P.f / YZ_P.f for a load case (XX:474-578: the grid
    padded by 2 along the load axis, z outer / x fastest): count = ns; for
-   each reference node its periodic-grid node index and whether it is the far
-   twin (displacement = grid value + jump).  grid/twin may be NULL. */
-int psx_lsm_reference_nodes(psx_lsm *h, int32_t loadcase, int64_t *count, int64_t *grid, int8_t *twin);
+   each reference node its periodic-grid node index, whether it is the far
+   twin (displacement = grid value + jump) and its 1-based coordinates on the
+   padded grid, icors(3,ns).  grid/twin/icors may be NULL. */
+int psx_lsm_reference_nodes(psx_lsm *h, int32_t loadcase, int64_t *count, int64_t *grid, int8_t *twin,
+                            int32_t *icors);
 /* The Verlet state in the reference's node order: rdat, vdat, adat (3,ns),
    the content of XX's iteration2 (written XX:1604-1609).  get needs a state
    of that load case (after psx_lsm_verlet or set); set restarts from one as

@@ -114,7 +114,7 @@ def _declare(L):
     L.psx_lsm_adjoint.argtypes = [_vp, _dbl, _dbl, _dbl, _i32, _vp, _vp]
     L.psx_lsm_profile.argtypes = [_vp, _i32, _vp]
     L.psx_lsm_batch_moduli.argtypes = [C.POINTER(LSMParams), _i32, _vp, _vp, _dbl, _i32, _vp, _vp, _vp]
-    L.psx_lsm_reference_nodes.argtypes = [_vp, _i32, _vp, _vp, _vp]
+    L.psx_lsm_reference_nodes.argtypes = [_vp, _i32, _vp, _vp, _vp, _vp]
     L.psx_lsm_get_verlet.argtypes = [_vp, _i32, _vp, _vp, _vp]
     L.psx_lsm_set_verlet.argtypes = [_vp, _i32, _vp, _vp, _vp]
     L.psx_cpl_create.argtypes = [_vp, _dbl, _dbl, _dbl, _dbl, _vp, _vp, _vp, _vp, _vp]

@@ -1424,13 +1424,15 @@ namespace {
 // (up to 8) voxels is solid, numbered z outer / x fastest.  Each reference
 // node maps to its periodic-grid node; the far layer (index n_p) is the twin
 // of layer 0, whose displacement is the grid value plus the jump.
-void reference_nodes(const psx_lsm *h, int lc, std::vector<int64_t> &grid, std::vector<int8_t> &twin) {
+void reference_nodes(const psx_lsm *h, int lc, std::vector<int64_t> &grid, std::vector<int8_t> &twin,
+                     std::vector<int32_t> *icors = nullptr) {
     const int pa = CASE_PD[lc][0];
     const int n[3] = {h->nx, h->ny, h->nz};
     int G[3] = {n[0], n[1], n[2]};
     G[pa] += 2;
     grid.clear();
     twin.clear();
+    if (icors) icors->clear();
     for (int k = 0; k < G[2]; k++)
         for (int j = 0; j < G[1]; j++)
             for (int i = 0; i < G[0]; i++) {
@@ -1449,6 +1451,8 @@ void reference_nodes(const psx_lsm *h, int lc, std::vector<int64_t> &grid, std::
                 if (!solid) continue;
                 grid.push_back(c[0] % n[0] + (int64_t)n[0] * (c[1] % n[1] + (int64_t)n[1] * (c[2] % n[2])));
                 twin.push_back(c[pa] == n[pa] ? 1 : 0);
+                if (icors)
+                    for (int a = 0; a < 3; a++) icors->push_back(c[a] + 1);
             }
 }
 
@@ -1462,16 +1466,19 @@ int verlet_buffers(psx_lsm *h) {
 }
 }  // namespace
 
-extern "C" int psx_lsm_reference_nodes(psx_lsm *h, int32_t lc, int64_t *count, int64_t *grid, int8_t *twin) {
+extern "C" int psx_lsm_reference_nodes(psx_lsm *h, int32_t lc, int64_t *count, int64_t *grid, int8_t *twin,
+                                       int32_t *icors) {
     PSX_CHECK_ARG(h && count, "psx_lsm_reference_nodes: NULL argument");
     RC(check_case(lc));
     PSX_GUARD_BEGIN
     std::vector<int64_t> g;
     std::vector<int8_t> t;
-    reference_nodes(h, lc, g, t);
+    std::vector<int32_t> ic;
+    reference_nodes(h, lc, g, t, &ic);
     *count = (int64_t)g.size();
     if (grid) std::memcpy(grid, g.data(), sizeof(int64_t) * g.size());
     if (twin) std::memcpy(twin, t.data(), t.size());
+    if (icors) std::memcpy(icors, ic.data(), sizeof(int32_t) * ic.size());
     return PSX_OK;
     PSX_GUARD_END
 }

@@ -199,13 +199,13 @@ module porescale
        integer(c_int32_t), value :: restart
      end function psx_lsm_verlet
 
-     function psx_lsm_reference_nodes(h, loadcase, count, grid, twin) bind(C, name='psx_lsm_reference_nodes')
+     function psx_lsm_reference_nodes(h, loadcase, count, grid, twin, icors) bind(C, name='psx_lsm_reference_nodes')
        import :: c_int, c_int32_t, c_int64_t, c_ptr
        integer(c_int) :: psx_lsm_reference_nodes
        type(c_ptr), value :: h
        integer(c_int32_t), value :: loadcase
        integer(c_int64_t) :: count
-       type(c_ptr), value :: grid, twin
+       type(c_ptr), value :: grid, twin, icors
      end function psx_lsm_reference_nodes
 
      ! iteration2 state (rdat, vdat, adat as (3,ns)) of the Verlet loop

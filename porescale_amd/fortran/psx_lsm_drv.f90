@@ -12,9 +12,11 @@
 !     mode  solve  (default) preconditioned CG to the relative residual of
 !                  line 17 (default 1e-10), then one stress_tensor block
 !           verlet the reference's damped Verlet relaxation: itm blocks of
-!                  its steps, a stress_tensor block and the iteration2 state
-!                  after each (XX_P.f:1377-1545, 1604-1609); icon = 1 restarts
-!                  from iteration2 (XX_P.f:1288-1294)
+!                  its steps, a stress_tensor block, the iteration2 state and
+!                  the field files LSM_coord_field, LSM_displ_field,
+!                  LSM_velocity_field, LSM_force_field and 'write' after each
+!                  (XX_P.f:1377-1609); icon = 1 restarts from iteration2
+!                  (XX_P.f:1288-1294)
 !   case all solves the six cases and writes the 6x6 stiffness and the Voigt
 !   bulk and shear moduli to the file 'moduli'.
 !
@@ -33,7 +35,10 @@ program psx_lsm_drv
   real(8) :: relres
   integer(c_int64_t) :: counts(4), nref
   real(c_double), allocatable :: rr(:, :), vv(:, :), aa(:, :)
-  integer :: j
+  integer(c_int32_t), allocatable, target :: icors(:, :)
+  integer(c_int32_t), allocatable :: mijks(:, :, :)
+  integer :: j, gdim(3)
+  integer, parameter :: pad_axis(0:5) = [1, 2, 3, 2, 3, 1]
   integer(c_int32_t), allocatable :: ind(:, :, :)
   integer(c_int32_t), allocatable, target :: cbon(:), cang(:)
   real(c_double), allocatable, target :: csb(:), cab(:)
@@ -131,9 +136,20 @@ program psx_lsm_drv
      call psx_check(psx_lsm_stress(h, int(lc, c_int32_t), asig, moduli), 'psx_lsm_stress')
      call write_block(0, 0)
   case ('verlet')
-     call psx_check(psx_lsm_reference_nodes(h, int(lc, c_int32_t), nref, c_null_ptr, c_null_ptr), &
+     call psx_check(psx_lsm_reference_nodes(h, int(lc, c_int32_t), nref, c_null_ptr, c_null_ptr, c_null_ptr), &
           'psx_lsm_reference_nodes')
      allocate(rr(3, max(nref, 1_c_int64_t)), vv(3, max(nref, 1_c_int64_t)), aa(3, max(nref, 1_c_int64_t)))
+     allocate(icors(3, max(nref, 1_c_int64_t)))
+     call psx_check(psx_lsm_reference_nodes(h, int(lc, c_int32_t), nref, c_null_ptr, c_null_ptr, c_loc(icors)), &
+          'psx_lsm_reference_nodes')
+     gdim = [ncx, ncy, ncz]
+     gdim(pad_axis(lc)) = gdim(pad_axis(lc)) + 2   ! the padded grid (XX_P.f:521)
+     allocate(mijks(gdim(1), gdim(2), gdim(3)))
+     mijks = 0
+     do i = 1, int(nref)
+        mijks(icors(1, i), icors(2, i), icors(3, i)) = i
+     end do
+     call write_settings()
      if (icon == 1) then
         open(newunit=u, file='iteration2', status='old', iostat=ios)
         if (ios /= 0) then
@@ -152,6 +168,7 @@ program psx_lsm_drv
         call psx_check(psx_lsm_stress(h, int(lc, c_int32_t), asig, moduli), 'psx_lsm_stress')
         call write_block(iti, its)
         call psx_check(psx_lsm_get_verlet(h, int(lc, c_int32_t), rr, vv, aa), 'psx_lsm_get_verlet')
+        call write_fields(iti, its + 1)
         open(newunit=u, file='iteration2', status='replace')
         write(u, *) ((rr(j, i), j = 1, 3), i = 1, nref)
         write(u, *) ((vv(j, i), j = 1, 3), i = 1, nref)
@@ -195,6 +212,81 @@ contains
   ! One block of the reference's stress_tensor file (XX_P.f:1938-1966,
   ! YZ_P.f:1938-1963): the summed cube stress per solid cube and the
   ! effective stiffness coefficients of the load case.
+  ! lsm_settings (XX_P.f:1314-1348): parameters and sizes on the padded
+  ! grid; the bond and angle table sizes of the reference are not formed here
+  ! (the element operator replaces the tables)
+  subroutine write_settings()
+    integer :: v
+    open(newunit=v, file='lsm_settings', status='replace')
+    write(v, *) 'Initial program settings'
+    write(v, *) 'Physical parameters'
+    write(v, *) 'Young modulus Em1 of component solid 1 = ', Em1
+    write(v, *) 'Poissons ratio nu1 = ', nu1
+    write(v, *) 'linear spring parameter alf = ', alf1
+    write(v, *) 'angular spring parameter bet = ', bet1
+    write(v, *) 'Young modulus Em1 of component solid 2 = ', Em2
+    write(v, *) 'Poissons ratio nu2 = ', nu2
+    write(v, *) 'linear spring parameter alf = ', alf2
+    write(v, *) 'angular spring parameter bet = ', bet2
+    write(v, *) 'solid medium density rs = ', rs
+    write(v, *) 'integration time step dt = ', dt
+    write(v, *) 'lattice unit step dl = ', dl
+    write(v, *) 'Computational domain properties'
+    write(v, *) 'x unit cell size ncx = ', gdim(1)
+    write(v, *) 'y unit cell size ncy = ', gdim(2)
+    write(v, *) 'z unit cell size ncz = ', gdim(3)
+    write(v, *) 'number of solid points ns = ', nref
+    write(v, *) 'LSM algorithm parameters'
+    write(v, *) 'simulation duration itm = ', itm
+    write(v, *) 'save parameter its = ', its
+    close(v)
+  end subroutine write_settings
+
+  ! The field files after a block of Verlet steps (XX_P.f:1546-1592).
+  subroutine write_fields(iti_, itj_)
+    integer, intent(in) :: iti_, itj_
+    integer :: uu(4), v, ii, jj, kk, l
+    real(8) :: fto(3), vto(3), rto(3), z3(3)
+    z3 = 0.d0
+    open(newunit=uu(1), file='LSM_coord_field', status='replace')
+    open(newunit=uu(2), file='LSM_displ_field', status='replace')
+    open(newunit=uu(3), file='LSM_velocity_field', status='replace')
+    open(newunit=uu(4), file='LSM_force_field', status='replace')
+    do kk = 1, gdim(3)
+       do jj = 1, gdim(2)
+          do ii = 1, gdim(1)
+             l = mijks(ii, jj, kk)
+             if (l /= 0) then
+                write(uu(1), *) rr(:, l)
+                write(uu(2), *) rr(:, l) - icors(:, l)
+                write(uu(3), *) vv(:, l)
+                write(uu(4), *) aa(:, l) * rs
+             else
+                write(uu(3), *) z3
+                write(uu(4), *) z3
+                write(uu(1), *) [ii, jj, kk]
+                write(uu(2), *) z3
+             end if
+          end do
+       end do
+    end do
+    do l = 1, 4
+       close(uu(l))
+    end do
+    fto = 0.d0; vto = 0.d0; rto = 0.d0
+    do l = 1, int(nref)
+       fto = fto + aa(:, l) * rs
+       vto = vto + vv(:, l)
+       rto = rto + rr(:, l)
+    end do
+    open(newunit=v, file='write', status='replace')
+    write(v, *) 'iti = ', iti_, 'itj = ', itj_
+    write(v, *) 'FTO = ', fto
+    write(v, *) 'VTO = ', vto
+    write(v, *) 'RTO = ', rto
+    close(v)
+  end subroutine write_fields
+
   subroutine write_block(iti_, its_)
     integer, intent(in) :: iti_, its_
     integer :: v, pd, dd

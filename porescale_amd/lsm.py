@@ -166,15 +166,18 @@ class LSM:
         """nsteps of the reference's damped velocity-Verlet loop."""
         _lib.check(self._L.psx_lsm_verlet(self._h, _case(case), int(nsteps), 1 if restart else 0))
 
-    def reference_nodes(self, case):
-        """(grid index, far-twin flag) of every node of the reference's node
-        list for ``case`` (XX:474-578)."""
+    def reference_nodes(self, case, with_icors: bool = False):
+        """(grid index, far-twin flag[, icors (ns, 3) 1-based on the padded
+        grid]) of every node of the reference's node list for ``case``
+        (XX:474-578)."""
         n = C.c_int64(0)
-        _lib.check(self._L.psx_lsm_reference_nodes(self._h, _case(case), C.byref(n), None, None))
+        _lib.check(self._L.psx_lsm_reference_nodes(self._h, _case(case), C.byref(n), None, None, None))
         grid = np.zeros(n.value, np.int64)
         twin = np.zeros(n.value, np.int8)
-        _lib.check(self._L.psx_lsm_reference_nodes(self._h, _case(case), C.byref(n), _lib.ptr(grid), _lib.ptr(twin)))
-        return grid, twin.astype(bool)
+        icors = np.zeros((n.value, 3), np.int32)
+        _lib.check(self._L.psx_lsm_reference_nodes(self._h, _case(case), C.byref(n), _lib.ptr(grid), _lib.ptr(twin),
+                                                   _lib.ptr(icors)))
+        return (grid, twin.astype(bool), icors) if with_icors else (grid, twin.astype(bool))
 
     def get_verlet(self, case):
         """rdat, vdat, adat (ns, 3) of the Verlet state, reference node order

@@ -21,6 +21,9 @@ Fixtures (compressed .npz, inputs and reference outputs only):
   lsm_xx.npz      XX_P.f, two-component 10x12x14 pack, 100 Verlet steps,
                   nthread=3: r, v, a and stress_tensor moduli.
   lsm_yz.npz      YZ_P.f, same geometry, nthread=1.
+  lsm_xx_fields.npz, lsm_yz_fields.npz  the field files of the last block
+                  (LSM_coord_field, LSM_displ_field, LSM_velocity_field,
+                  LSM_force_field, and the sums of 'write').
   lsm_xx_restart.npz, lsm_yz_restart.npz  icon=1 from the state above
                   (iteration2), 50 more steps: r, v, a and the moduli.
   ref_tables.npz  the reference's velocities19/velocities9 and cube_* files
@@ -93,7 +96,8 @@ def main():
     nz, ny, nx = ind.shape
     for case, nth in (("xx", 3), ("yz", 1)):
         inp = pio.LSMInput(ncx=nx, ncy=ny, ncz=nz, nthread=nth, **LSM_INP)
-        r, v, a, st = ob.run_ref_lsm(case, ind, inp)
+        r, v, a, st, fl = ob.run_ref_lsm(case, ind, inp, files=True)
+        np.savez_compressed(os.path.join(HERE, f"lsm_{case}_fields.npz"), **fl)
         mods = ob.parse_stress_tensor(st)
         np.savez_compressed(os.path.join(HERE, f"lsm_{case}.npz"), r=r, v=v, a=a, steps=100, nthread=nth,
                             **{k: np.float64(v_) for k, v_ in mods.items()})

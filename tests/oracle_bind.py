@@ -365,8 +365,26 @@ def read_field_files(d: str, it1: int, shape) -> dict:
             "pressure": pre.reshape(shape), "phase": ph, "iter": it}
 
 
-def run_ref_lsm(case: str, ind: np.ndarray, inp: pio.LSMInput, restart=None, timeout=600.0):
-    """Run XX_P.f ('xx') or YZ_P.f ('yz'); returns (r, v, a, stress_tensor text)."""
+def read_lsm_block_files(d: str) -> dict:
+    """The field files XX_P.f writes after each block (XX:1546-1592): coord,
+    displ, vel, force as (rows, 3) over the padded grid (x fastest) and the
+    FTO/VTO/RTO sums of 'write'."""
+    out = {}
+    for key, name in (("coord", "LSM_coord_field"), ("displ", "LSM_displ_field"), ("vel", "LSM_velocity_field"),
+                      ("force", "LSM_force_field")):
+        with open(os.path.join(d, name)) as fh:
+            out[key] = pio.parse_fortran_numbers(fh.read()).reshape(-1, 3)
+    with open(os.path.join(d, "write")) as fh:
+        lines = fh.read().splitlines()
+    for key in ("FTO", "VTO", "RTO"):
+        line = next(l for l in lines if l.strip().startswith(key))
+        out[key] = pio.parse_fortran_numbers(line.split("=", 1)[1])
+    return out
+
+
+def run_ref_lsm(case: str, ind: np.ndarray, inp: pio.LSMInput, restart=None, timeout=600.0, files=False):
+    """Run XX_P.f ('xx') or YZ_P.f ('yz'); returns (r, v, a, stress_tensor text)
+    and, with ``files``, the last block's field files (read_lsm_block_files)."""
     exe = {"xx": "xx_ref", "yz": "yz_ref"}[case]
     d = _prep_dir(ind)
     try:
@@ -380,6 +398,8 @@ def run_ref_lsm(case: str, ind: np.ndarray, inp: pio.LSMInput, restart=None, tim
         with open(os.path.join(d, "icors")) as fh:
             ns = sum(1 for line in fh if line.strip())
         r, v, a = pio.read_lsm_restart(os.path.join(d, "iteration2"), ns)
+        if files:
+            return r, v, a, st, read_lsm_block_files(d)
         return r, v, a, st
     finally:
         shutil.rmtree(d, ignore_errors=True)

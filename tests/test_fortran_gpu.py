@@ -134,8 +134,15 @@ def test_lsm_driver_verlet_matches_reference_golden():
         _run([_drv("psx_lsm_drv"), "xx", "verlet"], d)
         st = open(os.path.join(d, "stress_tensor")).read()
         r, v, a = pio.read_lsm_restart(os.path.join(d, "iteration2"), g["r"].shape[0])
+        files = ob.read_lsm_block_files(d)
+        settings = open(os.path.join(d, "lsm_settings")).read()
     finally:
         shutil.rmtree(d, ignore_errors=True)
+    gf = np.load(os.path.join(GOLD, "lsm_xx_fields.npz"))
+    for key in ("coord", "displ", "vel", "force", "FTO", "VTO", "RTO"):
+        want = gf[key]
+        np.testing.assert_allclose(files[key], want, rtol=0, atol=1e-12 * max(np.abs(want).max(), 1.0))
+    assert f"{g['r'].shape[0]}" in settings
     m = ob.parse_stress_tensor(st)
     assert m["Exxxx"] == pytest.approx(float(g["Exxxx"]), rel=1e-11)
     assert m["Eyyxx"] == pytest.approx(float(g["Eyyxx"]), rel=1e-10)

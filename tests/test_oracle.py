@@ -87,6 +87,36 @@ def test_lsm_restart_bitexact_vs_reference(case):
     assert np.array_equal(r, g1["r"]) and np.array_equal(v, g1["v"]) and np.array_equal(a, g1["a"])
 
 
+@pytest.mark.parametrize("case", ["xx", "yz"])
+def test_lsm_block_files_match_reference(case):
+    """The field files after a block (XX:1546-1592) from the golden state:
+    LSM_coord_field (r at solid nodes, the node indices elsewhere),
+    LSM_displ_field (r - icors, as the reference writes it),
+    LSM_velocity_field, LSM_force_field (a rs) and the sums of 'write'."""
+    ind = lsm_geometry()
+    g, f = _gold(f"lsm_{case}.npz"), _gold(f"lsm_{case}_fields.npz")
+    o = ob.OracleLSM(ind, case, alf=(1.0, 0.1))
+    icors, mijks = o.nodes()
+    Gz, Gy, Gx = mijks.shape
+    kk, jj, ii = np.meshgrid(np.arange(Gz), np.arange(Gy), np.arange(Gx), indexing="ij")
+    ids = mijks.reshape(-1)
+    sol = ids > 0
+    coord = np.stack([ii.reshape(-1) + 1, jj.reshape(-1) + 1, kk.reshape(-1) + 1], 1).astype(float)
+    coord[sol] = g["r"][ids[sol] - 1]
+    displ = np.zeros_like(coord)
+    displ[sol] = g["r"][ids[sol] - 1] - icors[ids[sol] - 1]
+    vel = np.zeros_like(coord)
+    vel[sol] = g["v"][ids[sol] - 1]
+    force = np.zeros_like(coord)
+    force[sol] = g["a"][ids[sol] - 1] * 1.0
+    for key, want in (("coord", coord), ("displ", displ), ("vel", vel), ("force", force)):
+        assert np.array_equal(f[key], want), key
+    fto = np.zeros(3)
+    for row in g["a"]:
+        fto = fto + row * 1.0
+    assert np.array_equal(f["FTO"], fto)
+
+
 def test_lsm_tables_counts_consistent():
     ind = lsm_geometry()
     o = ob.OracleLSM(ind, "xx", alf=1.0, bet=0.1)
