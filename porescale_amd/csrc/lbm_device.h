@@ -47,28 +47,6 @@ struct StoreF64 {
     }
 };
 
-struct StoreF32 {
-    using phi_t = double;  // the colour field stays FP64: its differences drive the surface force
-    float *__restrict__ p;  // T[19][nl] then R[19][nl]
-    int64_t nl;
-    __device__ __forceinline__ void load_c(const LbmConst &c, int q, int64_t i, double &fr, double &fb) const {
-        double f = c.w0[q] + (double)p[(int64_t)q * nl + i];
-        double r = (double)p[(int64_t)(Q + q) * nl + i];
-        fr = r * f;
-        fb = f - fr;
-    }
-    __device__ __forceinline__ void store(const LbmConst &c, int q, int64_t j, double fr, double fb) const {
-        double f = fr + fb;
-        float r;
-        if (fb == 0.0) r = 1.0f;
-        else if (fr == 0.0) r = 0.0f;
-        else if (f == 0.0) r = 0.5f;
-        else r = (float)(fr / f);
-        p[(int64_t)q * nl + j] = (float)(f - c.w0[q]);
-        p[(int64_t)(Q + q) * nl + j] = r;
-    }
-};
-
 // FP32 pull layout: post-collision populations in AoSoA blocks of 64 nodes,
 // block-major: element (slot k, node n) at (n/64)*38*64 + k*64 + n%64, slots
 // 0..18 hold T, 19..37 hold R.  A node pulls direction q from the node src
@@ -80,7 +58,7 @@ __host__ __device__ __forceinline__ int64_t aos(int k, int64_t n) {
 __host__ __device__ __forceinline__ int64_t aos_pad(int64_t nl) { return (nl + AOS - 1) / AOS * AOS; }
 
 struct PullF32 {
-    using phi_t = double;
+    using phi_t = float;  // the FP32 path keeps the colour field in FP32 as well
     const float *__restrict__ p;
     const int32_t *__restrict__ src;  // [18][nl], -1 = bounce-back
     int64_t nl;
@@ -102,7 +80,7 @@ struct PullF32 {
 template <class St>
 __device__ __forceinline__ void load_pop(const St &s, const LbmConst &c, int q, int64_t i, double &fr,
                                          double &fb) {
-    if constexpr (std::is_same<St, StoreF32>::value || std::is_same<St, PullF32>::value) s.load_c(c, q, i, fr, fb);
+    if constexpr (std::is_same<St, PullF32>::value) s.load_c(c, q, i, fr, fb);
     else s.load(q, i, fr, fb);
 }
 
