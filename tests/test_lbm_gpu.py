@@ -302,3 +302,51 @@ def test_fields_after_gpu_steps_match_oracle():
     cf = want["color"]
     near = (np.abs(cf - 0.9) < 1e-5) | (np.abs(cf + 0.9) < 1e-5) | (np.abs(cf) < 1e-5)
     assert np.array_equal(g32["phase"][~near], want["phase"][~near])
+
+
+def _diag_arrays(d):
+    return np.array([d.permr, d.permb, *d.Qr, *d.Qb, d.nr, d.nb, d.tmr, d.tmb, d.tm, *d.outm])
+
+
+@pytest.mark.parametrize("shape, kind", [((5, 6, 7), "solid"), ((9, 8, 10), "fluid"), ((3, 3, 3), "pack")])
+def test_edge_domains_match_oracle(shape, kind):
+    """Edge geometries, FP64 bit-identical to the oracle: an all-solid domain
+    (no fluid node: every kernel has an empty grid), a domain with no solid
+    (every link streams, through the periodic wrap on all three axes, no
+    bounce-back) and the smallest grid the stencil allows (3 voxels a side,
+    where a node's -1 and +1 neighbours along an axis are one another's)."""
+    nz, ny, nx = shape
+    if kind == "solid":
+        ind = np.zeros(shape, np.int32)
+    elif kind == "fluid":
+        ind = np.ones(shape, np.int32)
+    else:
+        ind = np.ones(shape, np.int32)
+        ind[1, 1, 1] = 0
+        ind[0, 2, 1] = 0
+    inp = pio.LBMInput(B=0.7, sig=0.02, vir=0.1, vib=0.2, wet=0.3, ncx=nx, ncy=ny, ncz=nz, dt=1.0,
+                       bfr=[1e-5, 2e-6, 0], bfb=[0, 1e-5, 3e-6])
+    blobs = geo.phase_blobs(ind, 0.5, 3, radius=1.5) if kind != "solid" else np.zeros(shape, bool)
+    state = ob.equilibrium_state(inp, ob.fluid_order_mask(ind, blobs))
+    o = ob.OracleLBM(inp, ind)
+    o.set_state(*state)
+    o.step(12)
+    want, dwant, fwant = o.state(), _diag_arrays(o.check_state(1, 12)), o.save_fields()
+    _say(f"case edge domain {kind} {shape} fp64, nl={o.nl}")
+    with LBM(inp, ind) as lb:
+        lb.set_state(*state)
+        lb.step(12)
+        got, dgot, fgot = lb.get_state(), _diag_arrays(lb.check_state(1, 12)), lb.save_fields()
+    assert got[0].shape == want[0].shape == (o.nl, 19)
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+    np.testing.assert_allclose(dgot, dwant, rtol=1e-12, atol=1e-18, equal_nan=True)
+    for k in ("velocity", "color", "pressure", "phase"):
+        assert np.array_equal(fgot[k], fwant[k]), k
+    _say(f"case edge domain {kind} {shape} fp32")
+    with LBM(inp, ind, precision="f32") as lb:
+        lb.set_state(*state)
+        lb.step(12)
+        g32 = lb.get_state()
+    assert g32[0].shape == (o.nl, 19)
+    np.testing.assert_allclose(g32[0], want[0], rtol=0, atol=5e-8)
+    np.testing.assert_allclose(g32[1], want[1], rtol=0, atol=5e-8)

@@ -229,3 +229,46 @@ def test_multigrid_preconditioner_cuts_iterations(monkeypatch):
         out[mg] = (it, m[1, 2])
     assert out["1"][0] * 3 < out["0"][0]
     assert out["1"][1] == pytest.approx(out["0"][1], rel=1e-9)
+
+
+def test_edge_volumes_single_and_batched():
+    """Edge volumes, alone and together in one batch with an ordinary pack:
+    an all-pore volume (no solid node: every load case is the zero system
+    and the moduli are zero, as the reference's empty node loops give), the
+    smallest volume (2 voxels a side, all solid: the closed form), and a
+    solid island that touches no face (a floating cluster: its right-hand
+    side is zero, so the singular block does not disturb the solve)."""
+    a, b = 1.0, 0.1
+    c11, c12, c44 = 3 * a + 8 * b, a - 4 * b, a + 6 * b
+    closed = np.zeros((6, 6))
+    closed[:3, :3] = c12
+    np.fill_diagonal(closed[:3, :3], c11)
+    closed[3, 3] = closed[4, 4] = closed[5, 5] = c44
+    pore = np.ones((6, 7, 5), np.int32)
+    tiny = np.zeros((2, 2, 2), np.int32)
+    island = geo.sphere_pack((10, 9, 11), 0.4, 2.5, seed=41)
+    island[4:6, 4:6, 4:6] = 0
+    island[3, 3:7, 3:7] = 1
+    island[6, 3:7, 3:7] = 1
+    island[3:7, 3, 3:7] = 1
+    island[3:7, 6, 3:7] = 1
+    island[3:7, 3:7, 3] = 1
+    island[3:7, 3:7, 6] = 1
+    vols = [pore, tiny, island]
+    singles = []
+    for v, name in zip(vols, ("all-pore", "2x2x2 solid", "island")):
+        for prec in ("f64", "f32"):
+            _say(f"case edge volume {name} {prec}")
+            with LSM(v, a, b, precision=prec) as s:
+                singles.append(s.moduli(rtol=1e-10, maxit=20000))
+    (Cp, Kp, Gp), (Cp32, _, _) = singles[0], singles[1]
+    assert np.all(Cp == 0.0) and Kp == 0.0 and Gp == 0.0 and np.all(Cp32 == 0.0)
+    for C6, _, _ in singles[2:4]:
+        np.testing.assert_allclose(C6, closed, atol=1e-9)
+    np.testing.assert_allclose(singles[5][0], singles[4][0], atol=1e-8 * np.abs(singles[4][0]).max())
+    for prec in ("f64", "f32"):
+        _say(f"case edge volumes in one batch {prec}")
+        Cb, KG, ms, its = batch_moduli(vols, a, b, a, b, precision=prec, rtol=1e-10, maxit=20000)
+        assert np.all(Cb[0] == 0.0) and np.all(KG[0] == 0.0)
+        np.testing.assert_allclose(Cb[1], closed, atol=1e-9)
+        np.testing.assert_allclose(Cb[2], singles[4][0], atol=1e-8 * np.abs(singles[4][0]).max())
