@@ -153,6 +153,18 @@ int psx_lsm_create(const psx_lsm_params *p, int32_t nx, int32_t ny, int32_t nz,
 void psx_lsm_destroy(psx_lsm *h);
 /* counts[0..3] = solid nodes (periodic grid), solid cubes, nx*ny*nz, reserved */
 int psx_lsm_counts(psx_lsm *h, int64_t *counts);
+/* Host only, no handle or device: the sizes of the reference's bond and angle
+   tables for load case lc (counts[0] = nbo, counts[1] = nna, counts[2] = ns),
+   and its partition of the padded node grid into 2*nthread groups of z
+   layers: layers[2*nthread], and layb, lib, lia[2*nthread+1], the node, bond
+   and angle counts before each group (each may be NULL).  These are the
+   table sizes XX_P.f / YZ_P.f write to lsm_settings (XX:1314-1348: nbo, nna,
+   layers, layb, lib, lia; tables built at XX:522-548, 826-1018, 1140-1262);
+   the element operator itself uses no tables.  ind is nx*ny*nz voxel codes,
+   x fastest (0 and 3 solid). */
+int psx_lsm_table_counts(int32_t nx, int32_t ny, int32_t nz, const int32_t *ind, int32_t loadcase,
+                         int32_t nthread, int64_t *counts, int32_t *layers, int64_t *layb,
+                         int64_t *lib, int64_t *lia);
 /* Displacement field of a load case on the periodic node grid, (3,nx*ny*nz). */
 int psx_lsm_set_u(psx_lsm *h, int32_t loadcase, const double *u);
 int psx_lsm_get_u(psx_lsm *h, int32_t loadcase, double *u);

@@ -367,6 +367,14 @@ int64_t or_lsm_count(const or_lsm *h, int32_t which) {
     }
 }
 
+void or_lsm_layers(const or_lsm *h, int64_t *layb, int64_t *lib, int64_t *lia) {
+    for (int l = 0; l <= h->nth2; l++) {
+        if (layb) layb[l] = h->layb[l];
+        if (lib) lib[l] = h->lib[l];
+        if (lia) lia[l] = h->lia[l];
+    }
+}
+
 void or_lsm_get_nodes(const or_lsm *h, int32_t *icors, int32_t *mijks) {
     if (icors) memcpy(icors, h->icors, 3 * h->ns * sizeof(int32_t));
     if (mijks) memcpy(mijks, h->mijks, (size_t)h->G[0] * h->G[1] * h->G[2] * sizeof(int32_t));

@@ -1261,6 +1261,152 @@ extern "C" int psx_lsm_counts(psx_lsm *h, int64_t *c) {
 }
 
 namespace {
+// Lattice data of the reference's input files: velocities19 (1-based index q
+// -> c_q), velocities9 (the nine bond vectors, XX:422-427) and the 48 angle
+// types as rings of arm directions, type t pairing ring[m] with ring[m + 1]
+// (XX:309-362): three 8-rings of 45-degree angles, six 4-rings of 60-degree
+// angles.
+const int TAB_V19[19][3] = {{0, 0, 0},  {1, 0, 0},   {-1, 0, 0}, {0, 1, 0},  {0, -1, 0}, {0, 0, 1},  {0, 0, -1},
+                            {1, 1, 0},  {-1, 1, 0},  {1, -1, 0}, {-1, -1, 0}, {1, 0, 1},  {-1, 0, 1}, {1, 0, -1},
+                            {-1, 0, -1}, {0, 1, 1},  {0, -1, 1}, {0, 1, -1}, {0, -1, -1}};
+const int TAB_B9[9][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, -1, 0},
+                          {1, 0, 1}, {1, 0, -1}, {0, 1, 1}, {0, 1, -1}};
+const int TAB_RING8[3][8] = {{2, 8, 4, 9, 3, 11, 5, 10}, {2, 12, 6, 13, 3, 15, 7, 14}, {4, 16, 6, 17, 5, 19, 7, 18}};
+const int TAB_RING4[6][4] = {{12, 16, 13, 17}, {14, 18, 15, 19}, {8, 12, 10, 14},
+                             {9, 13, 11, 15},  {8, 16, 9, 18},   {10, 17, 11, 19}};
+
+// The reference's padded geometry of one load case (0-based indices):
+//  - nodes on a grid of n + 2 along the load axis p and n on the others; a
+//    node is solid when one of its 8 voxels is, the voxels along p being
+//    0..n-1 with nothing beyond (XX:474-518);
+//  - the voxel of a bond or angle cube is looked up cyclically: period n on
+//    the other axes, period n + 1 along p, whose last voxel is the added pore
+//    layer (XX:446-461 and the iic wrap of XX:614-622).
+struct PaddedGeom {
+    int n[3], G[3], pa;
+    const int32_t *ind;
+    bool code_solid(int64_t v) const { return ind[v] == 0 || ind[v] == 3; }
+    bool node_solid(const int (&c)[3]) const {
+        for (int corner = 0; corner < 8; corner++) {
+            int v[3];
+            bool ok = true;
+            for (int a = 0; a < 3; a++) {
+                v[a] = c[a] - ((corner >> a) & 1);
+                if (a == pa) ok = ok && v[a] >= 0 && v[a] < n[a];
+                else v[a] = (v[a] % n[a] + n[a]) % n[a];
+            }
+            if (ok && code_solid(v[0] + (int64_t)n[0] * (v[1] + (int64_t)n[1] * v[2]))) return true;
+        }
+        return false;
+    }
+    bool node_solid_wrapped(const int (&c)[3]) const {
+        int w[3];
+        for (int a = 0; a < 3; a++) w[a] = (c[a] % G[a] + G[a]) % G[a];
+        return node_solid(w);
+    }
+    bool cube_solid(const int (&v)[3]) const {
+        int w[3];
+        for (int a = 0; a < 3; a++) {
+            const int per = a == pa ? n[a] + 1 : n[a];
+            w[a] = (v[a] % per + per) % per;
+            if (w[a] == n[a]) return false;  // the pore layer along p
+        }
+        return code_solid(w[0] + (int64_t)n[0] * (w[1] + (int64_t)n[1] * w[2]));
+    }
+};
+}  // namespace
+
+// Sizes of the reference's bond and angle tables and its z-layer partition
+// for load case lc, from the voxel codes alone (host only).  The rules are
+// the reference's: a bond from solid node i along bond vector j exists when
+// the node it reaches is solid and one of the cubes sharing the bond is
+// (XX:826-1018); an angle of type t at i exists when both arm nodes are solid
+// and its cube (two diagonal arms) or one of the two cubes on either side of
+// its plane (an axis arm) is solid (XX:1140-1262).  Nodes are visited in the
+// reference's order, z outer and x fastest, in 2*nthread groups of z layers
+// (XX:522-548).
+extern "C" int psx_lsm_table_counts(int32_t nx, int32_t ny, int32_t nz, const int32_t *ind, int32_t lc,
+                                    int32_t nthread, int64_t *counts, int32_t *layers, int64_t *layb,
+                                    int64_t *lib, int64_t *lia) {
+    PSX_CHECK_ARG(ind && counts, "psx_lsm_table_counts: NULL argument");
+    PSX_CHECK_ARG(nx >= 2 && ny >= 2 && nz >= 2, "psx_lsm_table_counts: every dimension must be >= 2");
+    PSX_CHECK_ARG(nthread >= 1, "psx_lsm_table_counts: nthread must be >= 1");
+    PSX_CHECK_ARG(lc >= 0 && lc <= 5, "psx_lsm_table_counts: load case must be 0..5");
+    PSX_GUARD_BEGIN
+    PaddedGeom g;
+    g.n[0] = nx; g.n[1] = ny; g.n[2] = nz;
+    g.pa = CASE_PD[lc][0];
+    for (int a = 0; a < 3; a++) g.G[a] = g.n[a] + (a == g.pa ? 2 : 0);
+    g.ind = ind;
+    int nta[48][2], t = 0;
+    for (auto &ring : TAB_RING8)
+        for (int m = 0; m < 8; m++, t++) { nta[t][0] = ring[m]; nta[t][1] = ring[(m + 1) % 8]; }
+    for (auto &ring : TAB_RING4)
+        for (int m = 0; m < 4; m++, t++) { nta[t][0] = ring[m]; nta[t][1] = ring[(m + 1) % 4]; }
+    const int nth2 = 2 * nthread;
+    std::vector<int32_t> lay(nth2, g.G[2] / nth2);
+    for (int l = 0; l < g.G[2] % nth2; l++) lay[l]++;
+    std::vector<int64_t> lb(nth2 + 1, 0), bb(nth2 + 1, 0), ab(nth2 + 1, 0);
+    int64_t ns = 0, nbo = 0, nna = 0;
+    int kend = 0;
+    for (int l = 0; l < nth2; l++) {
+        const int k0 = kend;
+        kend = k0 + lay[l];
+        for (int k = k0; k < kend; k++)
+            for (int j = 0; j < g.G[1]; j++)
+                for (int i = 0; i < g.G[0]; i++) {
+                    const int c[3] = {i, j, k};
+                    if (!g.node_solid(c)) continue;
+                    ns++;
+                    for (int b = 0; b < 9; b++) {
+                        int nb[3], lo[3], hi[3];
+                        for (int a = 0; a < 3; a++) {
+                            nb[a] = c[a] + TAB_B9[b][a];
+                            lo[a] = TAB_B9[b][a] > 0 ? c[a] : c[a] - 1;
+                            hi[a] = TAB_B9[b][a] < 0 ? c[a] - 1 : c[a];
+                        }
+                        if (!g.node_solid_wrapped(nb)) continue;
+                        bool any = false;
+                        for (int x = lo[0]; x <= hi[0] && !any; x++)
+                            for (int y = lo[1]; y <= hi[1] && !any; y++)
+                                for (int z = lo[2]; z <= hi[2] && !any; z++) any = g.cube_solid({x, y, z});
+                        nbo += any;
+                    }
+                    for (int tt = 0; tt < 48; tt++) {
+                        const int *jv = TAB_V19[nta[tt][0] - 1], *kv = TAB_V19[nta[tt][1] - 1];
+                        int pj[3], pk[3], cube[3];
+                        for (int a = 0; a < 3; a++) {
+                            pj[a] = c[a] + jv[a];
+                            pk[a] = c[a] + kv[a];
+                            cube[a] = std::min(c[a], std::min(pj[a], pk[a]));
+                        }
+                        if (!g.node_solid_wrapped(pj) || !g.node_solid_wrapped(pk)) continue;
+                        bool exists;
+                        if (nta[tt][0] > 7 && nta[tt][1] > 7) {
+                            exists = g.cube_solid(cube);
+                        } else {
+                            const int ax = (jv[0] == 0 && kv[0] == 0) ? 0 : ((jv[1] == 0 && kv[1] == 0) ? 1 : 2);
+                            int other[3] = {cube[0], cube[1], cube[2]};
+                            other[ax] -= 1;
+                            exists = g.cube_solid(cube) || g.cube_solid(other);
+                        }
+                        nna += exists;
+                    }
+                }
+        lb[l + 1] = ns;
+        bb[l + 1] = nbo;
+        ab[l + 1] = nna;
+    }
+    counts[0] = nbo; counts[1] = nna; counts[2] = ns;
+    if (layers) std::copy(lay.begin(), lay.end(), layers);
+    if (layb) std::copy(lb.begin(), lb.end(), layb);
+    if (lib) std::copy(bb.begin(), bb.end(), lib);
+    if (lia) std::copy(ab.begin(), ab.end(), lia);
+    return PSX_OK;
+    PSX_GUARD_END
+}
+
+namespace {
 int ensure_solver(psx_lsm *h, int R) {
     if (h->sv.R == R) return PSX_OK;
     solver_free(h->sv);

@@ -182,6 +182,18 @@ module porescale
        integer(c_int64_t) :: counts(4)
      end function psx_lsm_counts
 
+     ! host only: bond/angle table sizes and the z-layer partition (lsm_settings)
+     function psx_lsm_table_counts(nx, ny, nz, ind, loadcase, nthread, counts, layers, layb, lib, lia) &
+          bind(C, name='psx_lsm_table_counts')
+       import :: c_int, c_int32_t, c_int64_t
+       integer(c_int) :: psx_lsm_table_counts
+       integer(c_int32_t), value :: nx, ny, nz, loadcase, nthread
+       integer(c_int32_t) :: ind(*)
+       integer(c_int64_t) :: counts(3)
+       integer(c_int32_t) :: layers(*)
+       integer(c_int64_t) :: layb(*), lib(*), lia(*)
+     end function psx_lsm_table_counts
+
      function psx_lsm_get_u(h, loadcase, u) bind(C, name='psx_lsm_get_u')
        import :: c_int, c_int32_t, c_double, c_ptr
        integer(c_int) :: psx_lsm_get_u

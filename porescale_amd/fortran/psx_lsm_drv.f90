@@ -212,22 +212,45 @@ contains
   ! One block of the reference's stress_tensor file (XX_P.f:1938-1966,
   ! YZ_P.f:1938-1963): the summed cube stress per solid cube and the
   ! effective stiffness coefficients of the load case.
-  ! lsm_settings (XX_P.f:1314-1348): parameters and sizes on the padded
-  ! grid; the bond and angle table sizes of the reference are not formed here
-  ! (the element operator replaces the tables)
+  ! lsm_settings (XX_P.f:1314-1348, YZ_P.f:1309-1341): parameters, sizes on
+  ! the padded grid, and the sizes of the reference's bond and angle tables
+  ! with their z-layer partition, counted on the host by the library (the
+  ! element operator itself uses no tables)
   subroutine write_settings()
-    integer :: v
+    integer :: v, l
+    integer(c_int64_t) :: tc(3)
+    integer(c_int32_t), allocatable :: lay4(:)
+    integer, allocatable :: layers(:)
+    integer(8), allocatable :: layb(:), lib(:), lia(:)
+    integer(8) :: nbo, nna
+    allocate(lay4(2 * nthread), layers(2 * nthread))
+    allocate(layb(2 * nthread + 1), lib(2 * nthread + 1), lia(2 * nthread + 1))
+    call psx_check(psx_lsm_table_counts(int(ncx, c_int32_t), int(ncy, c_int32_t), int(ncz, c_int32_t), ind, &
+         int(lc, c_int32_t), int(nthread, c_int32_t), tc, lay4, layb, lib, lia), 'psx_lsm_table_counts')
+    nbo = tc(1); nna = tc(2)
+    layers = lay4
     open(newunit=v, file='lsm_settings', status='replace')
     write(v, *) 'Initial program settings'
     write(v, *) 'Physical parameters'
-    write(v, *) 'Young modulus Em1 of component solid 1 = ', Em1
-    write(v, *) 'Poissons ratio nu1 = ', nu1
-    write(v, *) 'linear spring parameter alf = ', alf1
-    write(v, *) 'angular spring parameter bet = ', bet1
-    write(v, *) 'Young modulus Em1 of component solid 2 = ', Em2
-    write(v, *) 'Poissons ratio nu2 = ', nu2
-    write(v, *) 'linear spring parameter alf = ', alf2
-    write(v, *) 'angular spring parameter bet = ', bet2
+    if (lc >= 3) then  ! the shear program's labels (YZ_P.f:1313-1320)
+       write(v, *) 'Young modulus Em1 = ', Em1
+       write(v, *) 'Poissons ratio nu1 = ', nu1
+       write(v, *) 'linear spring parameter alf1 = ', alf1
+       write(v, *) 'angular spring parameter bet1 = ', bet1
+       write(v, *) 'Young modulus Em2 = ', Em2
+       write(v, *) 'Poissons ratio nu2 = ', nu2
+       write(v, *) 'linear spring parameter alf2 = ', alf2
+       write(v, *) 'angular spring parameter bet2 = ', bet2
+    else
+       write(v, *) 'Young modulus Em1 of component solid 1 = ', Em1
+       write(v, *) 'Poissons ratio nu1 = ', nu1
+       write(v, *) 'linear spring parameter alf = ', alf1
+       write(v, *) 'angular spring parameter bet = ', bet1
+       write(v, *) 'Young modulus Em1 of component solid 2 = ', Em2
+       write(v, *) 'Poissons ratio nu2 = ', nu2
+       write(v, *) 'linear spring parameter alf = ', alf2
+       write(v, *) 'angular spring parameter bet = ', bet2
+    end if
     write(v, *) 'solid medium density rs = ', rs
     write(v, *) 'integration time step dt = ', dt
     write(v, *) 'lattice unit step dl = ', dl
@@ -239,6 +262,12 @@ contains
     write(v, *) 'LSM algorithm parameters'
     write(v, *) 'simulation duration itm = ', itm
     write(v, *) 'save parameter its = ', its
+    write(v, *) 'nbo = ', nbo
+    write(v, *) 'nna = ', nna
+    write(v, *) 'layers = ', (layers(l), l = 1, 2 * nthread)
+    write(v, *) 'layb = ', (layb(l), l = 1, 2 * nthread + 1)
+    write(v, *) 'lib = ', (lib(l), l = 1, 2 * nthread + 1)
+    write(v, *) 'lia = ', (lia(l), l = 1, 2 * nthread + 1)
     close(v)
   end subroutine write_settings
 

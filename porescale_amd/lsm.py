@@ -230,6 +230,23 @@ class LSM:
         return tuple(ms)
 
 
+def table_counts(ind: np.ndarray, case, nthread: int = 1) -> dict:
+    """Sizes of the reference's bond and angle tables for ``case`` and its
+    z-layer partition (the lsm_settings entries nbo, nna, layers, layb, lib,
+    lia of XX:1314-1348), from the voxel codes ``ind`` [z, y, x] alone."""
+    L = _lib.lib()
+    nz, ny, nx = ind.shape
+    nth2 = 2 * int(nthread)
+    counts = np.zeros(3, np.int64)
+    layers = np.zeros(nth2, np.int32)
+    layb, lib_, lia = (np.zeros(nth2 + 1, np.int64) for _ in range(3))
+    _lib.check(L.psx_lsm_table_counts(nx, ny, nz, _lib.ptr(_lib.i32(ind).reshape(-1)), _case(case), int(nthread),
+                                      _lib.ptr(counts), _lib.ptr(layers), _lib.ptr(layb), _lib.ptr(lib_),
+                                      _lib.ptr(lia)))
+    return {"nbo": int(counts[0]), "nna": int(counts[1]), "ns": int(counts[2]), "layers": layers, "layb": layb,
+            "lib": lib_, "lia": lia}
+
+
 def batch_moduli(volumes: Sequence[np.ndarray], alf1: float, bet1: float, alf2: float, bet2: float,
                  gstrain: float = 0.01, precision: str = "f32", rtol: float = 1e-10, maxit: int = 20000):
     """Moduli of many sub-volumes, all six load cases solved together.

@@ -26,6 +26,10 @@ Fixtures (compressed .npz, inputs and reference outputs only):
                   LSM_force_field, and the sums of 'write').
   lsm_xx_restart.npz, lsm_yz_restart.npz  icon=1 from the state above
                   (iteration2), 50 more steps: r, v, a and the moduli.
+  lsm_settings_xx.txt, lsm_settings_yz.txt  the lsm_settings file of the
+                  lsm_xx / lsm_yz runs as the reference wrote it (table sizes
+                  nbo, nna and the z-layer partition included).
+                  `make_golden.py settings` rewrites only these.
   ref_tables.npz  the reference's velocities19/velocities9 and cube_* files
                   as read from /root/reference (data tables of the API).
 """
@@ -78,6 +82,18 @@ LSM_INP = dict(Em1=18.639, nu1=0.35065, Em2=94.529, nu2=0.0742, rs=1.0, dt=0.01,
                gstrain=0.01)
 
 
+def lsm_settings_golden():
+    import oracle_bind as ob
+    from porescale_amd import io as pio
+    ind = lsm_geometry()
+    nz, ny, nx = ind.shape
+    for case, nth in (("xx", 3), ("yz", 1)):
+        inp = pio.LSMInput(ncx=nx, ncy=ny, ncz=nz, nthread=nth, **LSM_INP)
+        text = ob.run_ref_lsm(case, ind, inp, settings=True)[-1]
+        with open(os.path.join(HERE, f"lsm_settings_{case}.txt"), "w") as fh:
+            fh.write(text)
+
+
 def main():
     (ind0, inp0), (ind1, inp1, blue) = lbm_cases()
     fr, fb, outm, it1 = ob.run_ref_lbm(inp0, ind0)
@@ -111,8 +127,12 @@ def main():
     np.savez_compressed(os.path.join(HERE, "ref_tables.npz"), velocities19=pio.read_velocities(
         os.path.join(ref, "velocities19"), 19), velocities9=pio.read_velocities(os.path.join(ref, "velocities9"), 9),
         cube_bon=b, cube_ang=an, cube_sbond=sb, cube_abond=ab)
+    lsm_settings_golden()
     print("golden fixtures written to", HERE)
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["settings"]:
+        lsm_settings_golden()
+    else:
+        main()

@@ -83,6 +83,7 @@ def lib():
         L.or_lsm_count.restype = C.c_int64
         L.or_lsm_count.argtypes = [vp, C.c_int32]
         L.or_lsm_get_nodes.argtypes = [vp, vp, vp]
+        L.or_lsm_layers.argtypes = [vp, vp, vp, vp]
         L.or_lsm_get_tables.argtypes = [vp] + [vp] * 6
         L.or_lsm_get_state.argtypes = [vp, vp, vp, vp]
         L.or_lsm_set_state.argtypes = [vp, vp, vp, vp]
@@ -240,6 +241,13 @@ class OracleLSM:
         return {k: self.L.or_lsm_count(self.h, i)
                 for i, k in enumerate(["ns", "nbo", "nna", "Gx", "Gy", "Gz", "npair"])}
 
+    def layers(self):
+        """(layb, lib, lia): node, bond and angle offsets of the z-layer groups."""
+        n = 2 * self.p.nthread + 1
+        out = [np.zeros(n, np.int64) for _ in range(3)]
+        self.L.or_lsm_layers(self.h, *(_ptr(x) for x in out))
+        return tuple(out)
+
     def nodes(self):
         c = self.counts()
         icors = np.zeros((c["ns"], 3), np.int32)
@@ -382,9 +390,11 @@ def read_lsm_block_files(d: str) -> dict:
     return out
 
 
-def run_ref_lsm(case: str, ind: np.ndarray, inp: pio.LSMInput, restart=None, timeout=600.0, files=False):
+def run_ref_lsm(case: str, ind: np.ndarray, inp: pio.LSMInput, restart=None, timeout=600.0, files=False,
+                settings=False):
     """Run XX_P.f ('xx') or YZ_P.f ('yz'); returns (r, v, a, stress_tensor text)
-    and, with ``files``, the last block's field files (read_lsm_block_files)."""
+    and, with ``files``, the last block's field files (read_lsm_block_files);
+    with ``settings``, also the text of lsm_settings."""
     exe = {"xx": "xx_ref", "yz": "yz_ref"}[case]
     d = _prep_dir(ind)
     try:
@@ -398,11 +408,26 @@ def run_ref_lsm(case: str, ind: np.ndarray, inp: pio.LSMInput, restart=None, tim
         with open(os.path.join(d, "icors")) as fh:
             ns = sum(1 for line in fh if line.strip())
         r, v, a = pio.read_lsm_restart(os.path.join(d, "iteration2"), ns)
+        out = (r, v, a, st)
         if files:
-            return r, v, a, st, read_lsm_block_files(d)
-        return r, v, a, st
+            out += (read_lsm_block_files(d),)
+        if settings:
+            with open(os.path.join(d, "lsm_settings")) as fh:
+                out += (fh.read(),)
+        return out
     finally:
         shutil.rmtree(d, ignore_errors=True)
+
+
+def settings_tables(text: str) -> dict:
+    """nbo, nna and the layer arrays from the text of an lsm_settings file."""
+    out = {}
+    for line in text.splitlines():
+        key, sep, vals = line.partition("=")
+        key = key.strip()
+        if sep and key in ("nbo", "nna", "layers", "layb", "lib", "lia"):
+            out[key] = [int(x) for x in vals.split()]
+    return out
 
 
 def _copy(obj, **kw):

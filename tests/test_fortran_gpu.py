@@ -150,6 +150,26 @@ def test_lsm_driver_verlet_matches_reference_golden():
         np.testing.assert_allclose(x, g[key], rtol=0, atol=1e-13 * np.abs(g[key]).max())
 
 
+@pytest.mark.parametrize("case,nth", [("xx", 3), ("yz", 1)])
+def test_lsm_driver_settings_identical_to_reference(case, nth):
+    """lsm_settings as the driver writes it equals, character for character,
+    the file XX_P.f / YZ_P.f wrote for the same input (spring constants,
+    domain, node count, bond and angle table sizes and the z-layer
+    partition)."""
+    ind = lsm_geometry()
+    nz, ny, nx = ind.shape
+    d = _prep(ind)
+    try:
+        pio.write_lsm_input(os.path.join(d, "INPUT_PARAMETERS"),
+                            pio.LSMInput(ncx=nx, ncy=ny, ncz=nz, nthread=nth, **LSM_INP))
+        _run([_drv("psx_lsm_drv"), case, "verlet"], d)
+        got = open(os.path.join(d, "lsm_settings")).read()
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    want = open(os.path.join(GOLD, f"lsm_settings_{case}.txt")).read()
+    assert got == want
+
+
 @pytest.mark.parametrize("case", ["xx", "yz"])
 def test_lsm_driver_restart_matches_reference(case):
     """icon = 1: the driver reads the reference's iteration2 and continues

@@ -187,13 +187,20 @@ def test_lsm_fresh_case_vs_reference_binary():
     nz, ny, nx = ind.shape
     inp = pio.LSMInput(ncx=nx, ncy=ny, ncz=nz, Em1=3.0, nu1=0.2, Em2=7.0, nu2=0.1, rs=1.3, dt=0.02,
                        itm=1, its=60, gstrain=0.005, nthread=2)
-    r, v, a, st = ob.run_ref_lsm("xx", ind, inp)
+    r, v, a, st, settings = ob.run_ref_lsm("xx", ind, inp, settings=True)
     a1, b1 = spring_constants(inp.Em1, inp.nu1)
     a2, b2 = spring_constants(inp.Em2, inp.nu2)
     o = ob.OracleLSM(ind, "xx", alf=a1, bet=b1, alf2=a2, bet2=b2, rs=1.3, dt=0.02, gstrain=0.005, nthread=2)
     o.verlet(60)
     orr, ov, oa = o.state()
     assert np.array_equal(orr, r) and np.array_equal(ov, v) and np.array_equal(oa, a)
+    # the table sizes and layer partition the reference writes to lsm_settings
+    ref = ob.settings_tables(settings)
+    c = o.counts()
+    layb, lib, lia = o.layers()
+    assert ref["nbo"] == [c["nbo"]] and ref["nna"] == [c["nna"]]
+    assert ref["layb"] == layb.tolist() and ref["lib"] == lib.tolist() and ref["lia"] == lia.tolist()
+    assert sum(ref["layers"]) == c["Gz"] and len(ref["layers"]) == 4
 
 
 def test_lbm_save_fields_bitexact_vs_reference_files():
@@ -330,3 +337,25 @@ def test_coupled_two_pore_families_vs_reference_binary():
     for k in ("asigmap", "astrain"):
         assert np.array_equal(cab[k] / cab["ncub"], ab[k]), k
     assert cab["betai"].shape == (2,) and (cab["betai"] != 0).all()
+
+
+@pytest.mark.parametrize("seed", [3, 8])
+def test_library_table_counts_equal_oracle(seed):
+    """psx_lsm_table_counts (host code of the library, no device) gives the
+    oracle's bond and angle table sizes and z-layer partition for every load
+    case, thread count and a two-component, an all-solid and a sparse
+    geometry (the oracle's tables are pinned to the reference's Verlet
+    trajectories and its lsm_settings)."""
+    from porescale_amd.lsm import table_counts
+    vols = [geo.two_component(geo.sphere_pack((7, 9, 8), 0.35, 2.0, seed=seed), 0.3, seed=seed + 1, radius=1.5),
+            np.zeros((4, 3, 5), np.int32), geo.sphere_pack((6, 5, 7), 0.6, 1.5, seed=seed + 5)]
+    for v in vols:
+        for case in ("xx", "yy", "zz", "yz", "zx", "xy"):
+            for nth in (1, 3):
+                o = ob.OracleLSM(v, case=case, nthread=nth)
+                c = o.counts()
+                layb, lib, lia = o.layers()
+                t = table_counts(v, case, nth)
+                assert (t["nbo"], t["nna"], t["ns"]) == (c["nbo"], c["nna"], c["ns"]), (v.shape, case, nth)
+                assert np.array_equal(t["layb"], layb) and np.array_equal(t["lib"], lib)
+                assert np.array_equal(t["lia"], lia) and int(t["layers"].sum()) == c["Gz"]
