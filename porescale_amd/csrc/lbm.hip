@@ -328,9 +328,9 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
     *out = nullptr;
     int rc = psx::require_device();
     if (rc) return rc;
-    psx_lbm *h = nullptr;
     PSX_GUARD_BEGIN
-    h = new psx_lbm();
+    psx_lbm *h = new psx_lbm();
+    psx::CreateGuard<psx_lbm> guard(h, psx_lbm_destroy);
     h->p = *p;
     h->nx = nx; h->ny = ny; h->nz = nz;
     h->f32 = p->precision == PSX_PREC_F32;
@@ -353,7 +353,6 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
     if (h->f32 && (int64_t)aos_pad(nl > 0 ? nl : 1) * 2 * Q >= (int64_t)INT32_MAX) {
         psx::set_error("psx_lbm_create: %lld fluid nodes exceed the FP32 path's 32-bit population index",
                        (long long)nl);
-        delete h;
         return PSX_EINVAL;
     }
     h->dst_h.assign(18 * (size_t)nl, 0);
@@ -391,14 +390,12 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
         if (const char *eb = std::getenv("PSX_LBM_BRICK")) {
             if (std::sscanf(eb, "%d,%d,%d", &bd[0], &bd[1], &bd[2]) != 3 || bd[0] < 1 || bd[1] < 1 || bd[2] < 1) {
                 psx::set_error("psx_lbm_create: PSX_LBM_BRICK must be bx,by,bz");
-                delete h;
                 return PSX_EINVAL;
             }
         }
         for (int d = 0; d < 3; d++) h->box[d] = bd[d] + 2;
         if ((int64_t)h->box[0] * h->box[1] * h->box[2] * 4 > 65536) {
             psx::set_error("psx_lbm_create: brick halo box exceeds 64 KB of LDS");
-            delete h;
             return PSX_EINVAL;
         }
         build_bricks(h, bd, vox, bricks, hnode, hcell);
@@ -501,7 +498,7 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
         hipLaunchKernelGGL(k_fill<double>, dim3((unsigned)gb), dim3(256), 0, h->stream, (double *)h->d_phi, nv, p->wet);
     PSX_HIP(hipGetLastError());
     PSX_HIP(hipStreamSynchronize(h->stream));
-    *out = h;
+    *out = guard.release();
     return PSX_OK;
     PSX_GUARD_END
 }
@@ -1115,9 +1112,9 @@ extern "C" int psx_cpl_create(psx_lbm *lbm, double Em, double nu, double dl, dou
     PSX_CHECK_ARG(rs > 0.0 && dl > 0.0, "psx_cpl_create: rs and dl must be > 0");
     PSX_CHECK_ARG(5.0 - 10.0 * nu != 0.0 && 2 * nu * nu + nu - 1 != 0.0, "psx_cpl_create: nu gives no springs");
     *out = nullptr;
-    psx_cpl *c = nullptr;
     PSX_GUARD_BEGIN
-    c = new psx_cpl();
+    psx_cpl *c = new psx_cpl();
+    psx::CreateGuard<psx_cpl> guard(c, psx_cpl_destroy);
     c->lbm = lbm;
     c->rs = rs;
     c->dl = dl;
@@ -1141,7 +1138,7 @@ extern "C" int psx_cpl_create(psx_lbm *lbm, double Em, double nu, double dl, dou
     lp.gstrain = 0.0;
     lp.precision = PSX_PREC_F64;
     int rc = psx_lsm_create(&lp, nx, ny, nz, ind.data(), nullptr, cube_bon, cube_ang, cube_sbond, cube_abond, &c->lsm);
-    if (rc) { delete c; return rc; }
+    if (rc) return rc;
     c->N = psx_lsm_cpl::grid_nodes(c->lsm);
     coupling_nodes(lbm, c->mijks);
     for (int64_t g = 0; g < nv; g++)
@@ -1151,7 +1148,7 @@ extern "C" int psx_cpl_create(psx_lbm *lbm, double Em, double nu, double dl, dou
     const int64_t n2 = lbm->npb2, n4 = lbm->npb4;
     std::vector<int32_t> ipb2(4 * (size_t)n2), ipb4(6 * (size_t)n4);
     rc = psx_lbm_tables(lbm, nullptr, nullptr, ipb2.data(), ipb4.data());
-    if (rc) { psx_cpl_destroy(c); return rc; }
+    if (rc) return rc;
     std::vector<int32_t> inv;
     if (!lbm->perm.empty()) {
         inv.resize(lbm->perm.size());
@@ -1160,7 +1157,6 @@ extern "C" int psx_cpl_create(psx_lbm *lbm, double Em, double nu, double dl, dou
     const int64_t nlink = n2 + n4;
     if (nlink >= ((int64_t)1 << CPL_QSHIFT)) {
         psx::set_error("psx_cpl_create: %lld bounce-back links exceed the coupling table", (long long)nlink);
-        psx_cpl_destroy(c);
         return PSX_EINVAL;
     }
     std::vector<int32_t> lnode((size_t)std::max<int64_t>(nlink, 1)), cnt((size_t)c->ns + 1, 0);
@@ -1267,8 +1263,8 @@ extern "C" int psx_cpl_create(psx_lbm *lbm, double Em, double nu, double dl, dou
         PSX_HIP(hipStreamSynchronize(st));
     }
     rc = psx_lsm_cpl::reset(c->lsm);
-    if (rc) { psx_cpl_destroy(c); return rc; }
-    *out = c;
+    if (rc) return rc;
+    *out = guard.release();
     return PSX_OK;
     PSX_GUARD_END
 }

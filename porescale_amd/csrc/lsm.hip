@@ -1214,15 +1214,15 @@ extern "C" int psx_lsm_create(const psx_lsm_params *p, int32_t nx, int32_t ny, i
     }
     int rc = psx::require_device();
     if (rc) return rc;
-    psx_lsm *h = nullptr;
     PSX_GUARD_BEGIN
-    h = new psx_lsm();
+    psx_lsm *h = new psx_lsm();
+    psx::CreateGuard<psx_lsm> guard(h, psx_lsm_destroy);
     h->p = *p;
     h->nx = nx; h->ny = ny; h->nz = nz;
     PSX_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     int32_t dims[3] = {nx, ny, nz};
     rc = batch_build(h->B, h->stream, *p, 1, dims, ind, stiff);
-    if (rc) { psx_lsm_destroy(h); return rc; }
+    if (rc) return rc;
     const int64_t N = h->B.Ntot;
     for (int64_t i = 0; i < N; i++) h->ncubes += h->B.comp_h[i] != 0;
     for (int z = 0; z < nz; z++)
@@ -1236,8 +1236,8 @@ extern "C" int psx_lsm_create(const psx_lsm_params *p, int32_t nx, int32_t ny, i
                 h->nsolid_nodes += s;
             }
     rc = dalloc(&h->d_U, (size_t)MAXR * 3 * N);
-    if (rc) { psx_lsm_destroy(h); return rc; }
-    *out = h;
+    if (rc) return rc;
+    *out = guard.release();
     return PSX_OK;
     PSX_GUARD_END
 }
@@ -1891,19 +1891,26 @@ extern "C" int psx_lsm_batch_moduli(const psx_lsm_params *p, int32_t nvol, const
     int rc = psx::require_device();
     if (rc) return rc;
     PSX_GUARD_BEGIN
-    hipStream_t st;
-    PSX_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    Batch B;
-    Solver S;
-    auto cleanup = [&]() {
-        batch_free(B);
-        solver_free(S);
-        (void)hipStreamDestroy(st);
-    };
+    // freed on every way out of the call (HIP errors return early)
+    struct Res {
+        hipStream_t st = nullptr;
+        Batch B;
+        Solver S;
+        ~Res() {
+            if (st) (void)hipStreamSynchronize(st);
+            batch_free(B);
+            solver_free(S);
+            if (st) (void)hipStreamDestroy(st);
+        }
+    } res;
+    PSX_HIP(hipStreamCreateWithFlags(&res.st, hipStreamNonBlocking));
+    hipStream_t st = res.st;
+    Batch &B = res.B;
+    Solver &S = res.S;
     rc = batch_build(B, st, *p, nvol, dims, ind_concat, nullptr);
-    if (rc) { cleanup(); return rc; }
+    if (rc) return rc;
     rc = solver_alloc(S, B, MAXR, true);
-    if (rc) { cleanup(); return rc; }
+    if (rc) return rc;
     int cs[MAXR] = {0, 1, 2, 3, 4, 5};
     set_jumps(S, cs, MAXR, p->gstrain);
     PSX_HIP(hipMemcpyAsync(S.d_jump, &S.jump_h, sizeof(Jump), hipMemcpyHostToDevice, st));
@@ -1922,14 +1929,13 @@ extern "C" int psx_lsm_batch_moduli(const psx_lsm_params *p, int32_t nvol, const
     float ms = 0;
     PSX_HIP(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
-    if (rc) { cleanup(); return rc; }
+    if (rc) return rc;
     for (int v = 0; v < nvol; v++)
         moduli_from_stress(&asig[(size_t)v * MAXR * 9], B.vols[v], p->gstrain, &C_out[36 * v], &KG_out[2 * v],
                            &KG_out[2 * v + 1]);
     if (ms_out) { ms_out[0] = ms; ms_out[1] = it; }
     bool allc = true;
     for (auto &s : sst) allc &= s.conv != 0;
-    cleanup();
     if (!allc) {
         psx::set_error("psx_lsm_batch_moduli: some systems did not reach rtol %.1e", rtol);
         return PSX_ECONV;

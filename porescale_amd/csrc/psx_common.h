@@ -19,6 +19,30 @@ int require_device();
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Owns a half-built handle inside a create call: an early return (a failed
+// HIP call, an argument error) or a host exception leaving the call destroys
+// it with the handle's own destroy function; release() hands it over on
+// success.  The destroy functions accept partly initialised handles.
+template <class T>
+class CreateGuard {
+  public:
+    CreateGuard(T *h, void (*destroy)(T *)) : h_(h), destroy_(destroy) {}
+    ~CreateGuard() {
+        if (h_) destroy_(h_);
+    }
+    CreateGuard(const CreateGuard &) = delete;
+    CreateGuard &operator=(const CreateGuard &) = delete;
+    T *release() {
+        T *h = h_;
+        h_ = nullptr;
+        return h;
+    }
+
+  private:
+    T *h_;
+    void (*destroy_)(T *);
+};
+
 // Remaps a linear block id so that each XCD (blocks are dealt round-robin to
 // the 8 XCDs) works on one contiguous slice of the index space; neighbouring
 // nodes then share an XCD's L2.
