@@ -787,14 +787,19 @@ int autotune_tiles(Batch &B, hipStream_t st) {
                             tile_shape_code(4, 3, 1), tile_shape_code(5, 2, 1), tile_shape_code(3, 2, 3),
                             tile_shape_code(6, 1, 1)};
         const int R = B.Ntot * MAXR * 3 * 4 < ((int64_t)512 << 20) ? MAXR : 1;
+        psx::CallScope scope;
         float *u = nullptr, *y = nullptr;
         RC(dalloc(&u, (size_t)R * 3 * B.Ntot));
+        scope.buffer(u);
         RC(dalloc(&y, (size_t)R * 3 * B.Ntot));
+        scope.buffer(y);
         int64_t n = (int64_t)R * 3 * B.Ntot;
         hipLaunchKernelGGL(k_zero<float>, dim3((unsigned)psx::ceil_div(n, 256)), dim3(256), 0, st, u, n);
         hipEvent_t e0, e1;
         PSX_HIP(hipEventCreate(&e0));
+        scope.event(e0);
         PSX_HIP(hipEventCreate(&e1));
+        scope.event(e1);
         float best_ms = 1e30f;
         best = 0;
         for (int shape : cand) {
@@ -818,10 +823,6 @@ int autotune_tiles(Batch &B, hipStream_t st) {
                 std::fprintf(stderr, "tile shape %d: %d tiles, %.4f ms per apply (R=%d)\n", shape, B.ntiles, ms / 5, R);
             if (ms < best_ms) { best_ms = ms; best = shape; }
         }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        (void)hipFree(u);
-        (void)hipFree(y);
         std::lock_guard<std::mutex> lk(mu);
         cache[key] = best;
     }
@@ -1144,15 +1145,17 @@ int stress_batch(hipStream_t st, const Batch &B, Solver &S, const double *U, std
     int g = grid_for(B.ntiles);
     hipLaunchKernelGGL(k_stress, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles, S.R, B.d_ka,
                        B.d_kb, U, B.Ntot, S.d_jump, S.ppq);
+    psx::CallScope scope;
     double *d_out = S.prz;  // reuse: needs S*9 doubles <= ntiles*R
-    bool tmp = (size_t)S.S * 9 > (size_t)B.ntiles * S.R;
-    if (tmp) RC(dalloc(&d_out, (size_t)S.S * 9));
+    if ((size_t)S.S * 9 > (size_t)B.ntiles * S.R) {
+        RC(dalloc(&d_out, (size_t)S.S * 9));
+        scope.buffer(d_out);
+    }
     hipLaunchKernelGGL(k_stress_reduce, dim3(S.S * 9), dim3(256), 0, st, B.d_vols, S.R, S.ppq, d_out);
     PSX_HIP(hipGetLastError());
     out.resize((size_t)S.S * 9);
     PSX_HIP(hipMemcpyAsync(out.data(), d_out, sizeof(double) * S.S * 9, hipMemcpyDeviceToHost, st));
     PSX_HIP(hipStreamSynchronize(st));
-    if (tmp) (void)hipFree(d_out);
     return PSX_OK;
 }
 
@@ -1808,8 +1811,14 @@ extern "C" int psx_lsm_adjoint(psx_lsm *h, double wK, double wG, double rtol, in
             W[J * 9 + d + 3 * c] += dMdC[I + 6 * J] / (V * eps);
         }
     }
+    psx::CallScope scope;
     double *d_W = nullptr, *d_tA = nullptr, *d_tB = nullptr;
-    RC(dalloc(&d_W, W.size())); RC(dalloc(&d_tA, N)); RC(dalloc(&d_tB, N));
+    RC(dalloc(&d_W, W.size()));
+    scope.buffer(d_W);
+    RC(dalloc(&d_tA, N));
+    scope.buffer(d_tA);
+    RC(dalloc(&d_tB, N));
+    scope.buffer(d_tB);
     PSX_HIP(hipMemcpyAsync(d_W, W.data(), sizeof(double) * W.size(), hipMemcpyHostToDevice, h->stream));
     int cs[MAXR] = {0, 1, 2, 3, 4, 5};
     set_jumps(h->sv, cs, MAXR, h->p.gstrain);
@@ -1824,7 +1833,7 @@ extern "C" int psx_lsm_adjoint(psx_lsm *h, double wK, double wG, double rtol, in
     std::vector<SysState> st;
     int it = 0;
     int rc = solve_batch(h->stream, h->B, h->sv, h->p.precision == PSX_PREC_F32, rtol, maxit, 1, &st, &it);
-    if (rc) { (void)hipFree(d_W); (void)hipFree(d_tA); (void)hipFree(d_tB); return rc; }
+    if (rc) return rc;
     hipLaunchKernelGGL(k_adj_voxel, dim3(g), dim3(TILE), 0, h->stream, h->B.d_vols, h->B.d_tvol, h->B.d_tstart,
                        h->B.ntiles, h->B.d_ka, h->B.d_kb, h->d_U, h->sv.X, d_W, N, h->sv.d_jump, d_tA, d_tB);
     PSX_HIP(hipGetLastError());
@@ -1832,7 +1841,6 @@ extern "C" int psx_lsm_adjoint(psx_lsm *h, double wK, double wG, double rtol, in
     PSX_HIP(hipMemcpyAsync(tA.data(), d_tA, sizeof(double) * N, hipMemcpyDeviceToHost, h->stream));
     PSX_HIP(hipMemcpyAsync(tB.data(), d_tB, sizeof(double) * N, hipMemcpyDeviceToHost, h->stream));
     PSX_HIP(hipStreamSynchronize(h->stream));
-    (void)hipFree(d_W); (void)hipFree(d_tA); (void)hipFree(d_tB);
     double g4[4] = {0, 0, 0, 0};
     for (int64_t i = 0; i < N; i++) {
         int c = h->B.comp_h[i];
@@ -1861,8 +1869,14 @@ extern "C" int psx_lsm_profile(psx_lsm *h, int32_t n, double *ms) {
     PSX_CHECK_ARG(h && ms && n > 0, "psx_lsm_profile: bad argument");
     PSX_GUARD_BEGIN
     RC(ensure_solver(h, 1));
+    psx::CallScope scope;
     hipEvent_t e0, e1, e2;
-    PSX_HIP(hipEventCreate(&e0)); PSX_HIP(hipEventCreate(&e1)); PSX_HIP(hipEventCreate(&e2));
+    PSX_HIP(hipEventCreate(&e0));
+    scope.event(e0);
+    PSX_HIP(hipEventCreate(&e1));
+    scope.event(e1);
+    PSX_HIP(hipEventCreate(&e2));
+    scope.event(e2);
     PSX_HIP(hipEventRecord(e0, h->stream));
     for (int i = 0; i < n; i++)
         launch_apply<double, false, false, false>(h->stream, h->B, 1, h->B.d_ka, h->B.d_kb, h->sv.P, h->sv.Qv, nullptr,
@@ -1876,7 +1890,6 @@ extern "C" int psx_lsm_profile(psx_lsm *h, int32_t n, double *ms) {
     float a = 0, b = 0;
     PSX_HIP(hipEventElapsedTime(&a, e0, e1));
     PSX_HIP(hipEventElapsedTime(&b, e1, e2));
-    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2);
     ms[0] = a / n;
     ms[1] = b / n;
     return PSX_OK;
@@ -1915,8 +1928,12 @@ extern "C" int psx_lsm_batch_moduli(const psx_lsm_params *p, int32_t nvol, const
     set_jumps(S, cs, MAXR, p->gstrain);
     PSX_HIP(hipMemcpyAsync(S.d_jump, &S.jump_h, sizeof(Jump), hipMemcpyHostToDevice, st));
     PSX_HIP(hipStreamSynchronize(st));
+    psx::CallScope scope;
     hipEvent_t e0, e1;
-    PSX_HIP(hipEventCreate(&e0)); PSX_HIP(hipEventCreate(&e1));
+    PSX_HIP(hipEventCreate(&e0));
+    scope.event(e0);
+    PSX_HIP(hipEventCreate(&e1));
+    scope.event(e1);
     PSX_HIP(hipEventRecord(e0, st));
     rc = build_rhs(st, B, S);
     std::vector<SysState> sst;
@@ -1928,7 +1945,6 @@ extern "C" int psx_lsm_batch_moduli(const psx_lsm_params *p, int32_t nvol, const
     PSX_HIP(hipEventSynchronize(e1));
     float ms = 0;
     PSX_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
     if (rc) return rc;
     for (int v = 0; v < nvol; v++)
         moduli_from_stress(&asig[(size_t)v * MAXR * 9], B.vols[v], p->gstrain, &C_out[36 * v], &KG_out[2 * v],

@@ -833,45 +833,50 @@ int mg_build(hipStream_t st, const Batch &B, int R, const MGParams &prm, MG &mg)
     }
     for (int l = 0; l < L; l++) RC(upload_level(st, mg.lv[l]));
     // operators: A0 assembled in FP64, Galerkin products down the levels
-    double *Af = nullptr, *Ac = nullptr, *pmax = nullptr;
+    // Af: the FP64 operator of the level being processed (freed as soon as
+    // the next level's Galerkin product exists); freed on every way out
+    psx::DeviceBuf<double> Af, pmax_buf;
+    double *tmp = nullptr;
     int maxtiles = 0;
     for (auto &l : mg.lv) maxtiles = std::max(maxtiles, l.ntiles);
-    RC(dalloc(&pmax, maxtiles));
-    RC(dalloc(&Af, (size_t)NSTEN * l0.N));
+    RC(dalloc(&tmp, maxtiles));
+    pmax_buf.reset(tmp);
+    double *const pmax = tmp;
+    RC(dalloc(&tmp, (size_t)NSTEN * l0.N));
+    Af.reset(tmp);
     hipLaunchKernelGGL(k_assemble_fine, dim3(grid_for(l0.ntiles)), dim3(256), 0, st, l0.d_vols, l0.d_tvol,
-                       l0.d_tstart, l0.ntiles, B.d_ka, B.d_kb, Af, l0.N);
+                       l0.d_tstart, l0.ntiles, B.d_ka, B.d_kb, Af.get(), l0.N);
     for (int l = 0; l < L; l++) {
         MGLevel &lv = mg.lv[l];
         RC(dalloc(&lv.lam, nvol));
         if (l == 0) {
             hipLaunchKernelGGL(k_level_prep, dim3(grid_for(lv.ntiles)), dim3(256), 0, st, lv.d_vols, lv.d_tvol,
-                               lv.d_tstart, lv.ntiles, (const double *)Af, lv.N, (const double *)B.d_minv,
+                               lv.d_tstart, lv.ntiles, (const double *)Af.get(), lv.N, (const double *)B.d_minv,
                                (float *)nullptr, (float *)nullptr, pmax);
         } else {
             RC(dalloc(&lv.A, (size_t)NSTEN * lv.N));
             RC(dalloc(&lv.minv, (size_t)6 * lv.N));
             lv.own_minv = true;
             hipLaunchKernelGGL(k_level_prep, dim3(grid_for(lv.ntiles)), dim3(256), 0, st, lv.d_vols, lv.d_tvol,
-                               lv.d_tstart, lv.ntiles, (const double *)Af, lv.N, (const double *)nullptr, lv.A,
+                               lv.d_tstart, lv.ntiles, (const double *)Af.get(), lv.N, (const double *)nullptr, lv.A,
                                lv.minv, pmax);
         }
         hipLaunchKernelGGL(k_vol_max, dim3(nvol), dim3(256), 0, st, lv.d_vols, (const double *)pmax, lv.lam);
         PSX_HIP(hipGetLastError());
         if (l + 1 < L) {
             MGLevel &cv = mg.lv[l + 1];
-            RC(dalloc(&Ac, (size_t)NSTEN * cv.N));
+            RC(dalloc(&tmp, (size_t)NSTEN * cv.N));
+            psx::DeviceBuf<double> Ac(tmp);
             hipLaunchKernelGGL(k_galerkin, dim3(grid_for(cv.ntiles)), dim3(256), 0, st, lv.d_vols, cv.d_vols,
-                               cv.d_tvol, cv.d_tstart, cv.ntiles, (const double *)Af, lv.N, Ac, cv.N);
+                               cv.d_tvol, cv.d_tstart, cv.ntiles, (const double *)Af.get(), lv.N, Ac.get(), cv.N);
             PSX_HIP(hipGetLastError());
             PSX_HIP(hipStreamSynchronize(st));
-            (void)hipFree(Af);
-            Af = Ac;
-            Ac = nullptr;
+            Af = std::move(Ac);
         }
     }
     PSX_HIP(hipStreamSynchronize(st));
-    (void)hipFree(Af);
-    (void)hipFree(pmax);
+    Af.reset();
+    pmax_buf.reset();
     if (env_int("PSX_MG_DEBUG", 0)) {
         for (int l = 0; l < L; l++) {
             std::vector<float> lam(nvol);
@@ -883,8 +888,14 @@ int mg_build(hipStream_t st, const Batch &B, int R, const MGParams &prm, MG &mg)
     for (int l = 0; l < L; l++) RC(alloc_level_vectors(mg.lv[l], R, l > 0));
     // sharpen the bounds: power iteration on D^-1 A per volume and level
     if (prm.power_steps > 0) {
+        psx::CallScope scope;
         double *pn = nullptr, *n0 = nullptr, *n1 = nullptr;
-        RC(dalloc(&pn, maxtiles)); RC(dalloc(&n0, nvol)); RC(dalloc(&n1, nvol));
+        RC(dalloc(&pn, maxtiles));
+        scope.buffer(pn);
+        RC(dalloc(&n0, nvol));
+        scope.buffer(n0);
+        RC(dalloc(&n1, nvol));
+        scope.buffer(n1);
         for (int l = 0; l < L; l++) {
             MGLevel &lv = mg.lv[l];
             LevelArgs A = args_of(lv, B, prm.ratio);
@@ -907,7 +918,6 @@ int mg_build(hipStream_t st, const Batch &B, int R, const MGParams &prm, MG &mg)
             PSX_HIP(hipGetLastError());
         }
         PSX_HIP(hipStreamSynchronize(st));
-        (void)hipFree(pn); (void)hipFree(n0); (void)hipFree(n1);
     }
     if (env_int("PSX_MG_DEBUG", 0)) {
         for (int l = 0; l < L; l++) {

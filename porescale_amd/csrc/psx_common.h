@@ -6,7 +6,9 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <memory>
 #include <string>
+#include <vector>
 
 #include "porescale.h"
 
@@ -18,6 +20,39 @@ void set_error(const char *fmt, ...);
 int require_device();
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Device buffers and events that live for one library call, released on
+// every way out of it (HIP errors return early).  hipFree waits for the
+// device, so work still queued on them completes first.
+class CallScope {
+  public:
+    CallScope() = default;
+    CallScope(const CallScope &) = delete;
+    CallScope &operator=(const CallScope &) = delete;
+    ~CallScope() {
+        for (hipEvent_t e : events_) (void)hipEventDestroy(e);
+        for (void *b : bufs_) (void)hipFree(b);
+    }
+    void buffer(void *b) {
+        if (b) bufs_.push_back(b);
+    }
+    void event(hipEvent_t e) {
+        if (e) events_.push_back(e);
+    }
+
+  private:
+    std::vector<void *> bufs_;
+    std::vector<hipEvent_t> events_;
+};
+
+// Owning pointer to device memory (hipFree on reset or scope exit).
+struct DeviceFree {
+    void operator()(void *b) const {
+        if (b) (void)hipFree(b);
+    }
+};
+template <class T>
+using DeviceBuf = std::unique_ptr<T, DeviceFree>;
 
 // Owns a half-built handle inside a create call: an early return (a failed
 // HIP call, an argument error) or a host exception leaving the call destroys
