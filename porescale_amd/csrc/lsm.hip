@@ -1098,8 +1098,12 @@ int solve_batch(hipStream_t st, const Batch &B, Solver &S, bool f32, double tol,
     std::vector<SysState> ho(S.S);
     const int max_outer = 16;
     for (int outer = 0; outer <= max_outer; outer++) {
-        // r = b - K x
-        launch_apply<double, false, false, false>(st, B, S.R, B.d_ka, B.d_kb, S.X, S.Qv, nullptr, S.d_jump, nullptr);
+        // r = b - K x; on the first pass x = 0, so K x is a zero fill
+        if (outer == 0)
+            hipLaunchKernelGGL(k_zero<double>, dim3(gb), dim3(256), 0, st, S.Qv, n);
+        else
+            launch_apply<double, false, false, false>(st, B, S.R, B.d_ka, B.d_kb, S.X, S.Qv, nullptr, S.d_jump,
+                                                      nullptr);
         hipLaunchKernelGGL(k_refine_residual, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles,
                            S.R, B.d_ka, B.d_kb, S.B, S.Qv, S.Rr, B.Ntot, S.prr, S.ppq);
         hipLaunchKernelGGL(k_outer_reduce, dim3(S.S), dim3(256), 0, st, B.d_vols, S.R, S.prr, S.ppq, S.outer,
