@@ -350,11 +350,6 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
                 }
             }
     h->nl = nl;
-    if (h->f32 && (int64_t)aos_pad(nl > 0 ? nl : 1) * 2 * Q >= (int64_t)INT32_MAX) {
-        psx::set_error("psx_lbm_create: %lld fluid nodes exceed the FP32 path's 32-bit population index",
-                       (long long)nl);
-        return PSX_EINVAL;
-    }
     h->dst_h.assign(18 * (size_t)nl, 0);
     std::vector<int32_t> vox(nl);
     for (int64_t n = 0; n < nl; n++) {
@@ -384,6 +379,11 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
     if (h->f32 && nl > 0) {  // opt-in: measured slower than the two-kernel step (occupancy 2)
         const char *ef = std::getenv("PSX_LBM_FUSED");
         h->fused = ef && std::atoi(ef) != 0;
+    }
+    if (h->fused && (int64_t)aos_pad(nl) * 2 * Q >= (int64_t)INT32_MAX) {
+        psx::set_error("psx_lbm_create: %lld fluid nodes exceed the fused step's 32-bit population index "
+                       "(the two-kernel step has no such limit)", (long long)nl);
+        return PSX_EINVAL;
     }
     if (h->fused) {
         int bd[3] = {8, 8, 16};  // brick (x, y, z); PSX_LBM_BRICK="bx,by,bz"

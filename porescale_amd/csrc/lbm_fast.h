@@ -17,6 +17,9 @@ struct Brick {
 // pull table is src16 (int16 residuals, -32768 = bounce-back) with the
 // per-plane bases pbase[3][nx] when src16 is not null, else src [18][nl]
 // (-1 = bounce-back).
+// true when the collide kernel needs 64-bit pull offsets (the population
+// buffer of nl nodes holds 2^31 floats or more; PSX_LBM_WIDE=1 forces it)
+bool wide_index(int64_t nl);
 hipError_t collide(const psx_lbm_dev::LbmConst &c, const float *in, float *out, const int32_t *src,
                    const int16_t *src16, const int32_t *pbase, const int32_t *vox, const float *phi, int xcd,
                    hipStream_t st);

@@ -11,6 +11,9 @@
 // one 9.7 KB span.  This translation unit allows FMA contraction and
 // replaces the per-node divisions of the recolouring step (L1:3798-3816) and
 // of calcFs by one reciprocal each.
+#include <cstdlib>
+#include <type_traits>
+
 #include "lbm_fast.h"
 
 using namespace psx_lbm_dev;
@@ -192,7 +195,7 @@ __device__ __forceinline__ void collide_fast(const LbmConst &c, double t1, doubl
 
 // Pull + collide + recolour of node i (voxel v) into its own slots of out.
 // fs(x, y, z, Cg, Fs) evaluates calcFs at the node's voxel.
-template <int M, class CalcFs>
+template <int M, class CalcFs, bool WIDE = false>
 __device__ __forceinline__ void collide_pull(const LbmConst &c, const float *__restrict__ in,
                                              float *__restrict__ out, const PullIdx &P, int64_t i, int32_t v,
                                              const CalcFs &fs) {
@@ -200,7 +203,10 @@ __device__ __forceinline__ void collide_pull(const LbmConst &c, const float *__r
     int32_t pb[3];
     load_bases<M>(c, P, v, pb);
     double f[Q];
-    int32_t at[Q];  // where direction q is pulled from (AoSoA index; < 2^31 for nl < 5.6e7)
+    // where direction q is pulled from: an AoSoA element index, 32-bit while
+    // the population buffer holds fewer than 2^31 floats (nl < 5.6e7), 64-bit
+    // beyond (WIDE: 19 more registers)
+    typename std::conditional<WIDE, int64_t, int32_t>::type at[Q];
     double rr = 0.0, rb = 0.0;
     bool allred = true, allblue = true;
 #pragma unroll
@@ -208,7 +214,7 @@ __device__ __forceinline__ void collide_pull(const LbmConst &c, const float *__r
         int64_t j;
         int k;
         pull_from<M>(P, pb, nl, i, q, j, k);
-        at[q] = (int32_t)aos(k, j);
+        at[q] = (decltype(at[0] + 0))aos(k, j);
         double t = c.w0[q] + (double)in[at[q]];
         float r = in[at[q] + Q * AOS];
         double a = (double)r * t;
@@ -311,7 +317,7 @@ __device__ __forceinline__ double colour_pull(const LbmConst &c, const float *__
     return (rr - rb) / (rr + rb);
 }
 
-template <int M>
+template <int M, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void k_collide_f32(const LbmConst c, const float *__restrict__ in,
                                                        float *__restrict__ out, const PullIdx P,
                                                        const int32_t *__restrict__ vox,
@@ -320,7 +326,7 @@ __global__ __launch_bounds__(BLOCK) void k_collide_f32(const LbmConst c, const f
     int64_t i = b * blockDim.x + threadIdx.x;
     if (i >= c.nl) return;
     auto fs = [&](int x, int y, int z, double *Cg, double *Fs) { calc_fs<float, false>(c, phi, x, y, z, Cg, Fs); };
-    collide_pull<M>(c, in, out, P, i, vox[i], fs);
+    collide_pull<M, decltype(fs), WIDE>(c, in, out, P, i, vox[i], fs);
 }
 
 template <int M>
@@ -373,6 +379,11 @@ __global__ __launch_bounds__(BLOCK) void k_step_f32(const LbmConst c, const floa
 
 namespace psx_lbm_fast {
 
+bool wide_index(int64_t nl) {
+    const char *e = std::getenv("PSX_LBM_WIDE");  // 1: 64-bit pull offsets at any size (testing)
+    return (e && std::atoi(e) != 0) || aos_pad(nl > 0 ? nl : 1) * 2 * Q >= (int64_t)INT32_MAX;
+}
+
 hipError_t collide(const LbmConst &c, const float *in, float *out, const int32_t *src, const int16_t *src16,
                    const int32_t *pbase, const int32_t *vox, const float *phi, int xcd, hipStream_t st) {
     int64_t nb = psx::ceil_div(c.nl, BLOCK);
@@ -382,10 +393,20 @@ hipError_t collide(const LbmConst &c, const float *in, float *out, const int32_t
     // indices need no dependent load of a plane base
     if (src16 && !pbase) {
         const PullIdx P{src, src16, nullptr};
-        hipLaunchKernelGGL(k_collide_f32<2>, dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, out, P, vox, phi, xcd);
+        if (wide_index(c.nl))
+            hipLaunchKernelGGL((k_collide_f32<2, true>), dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, out, P, vox,
+                               phi, xcd);
+        else
+            hipLaunchKernelGGL((k_collide_f32<2, false>), dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, out, P, vox,
+                               phi, xcd);
     } else {
         const PullIdx P{src, nullptr, nullptr};
-        hipLaunchKernelGGL(k_collide_f32<0>, dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, out, P, vox, phi, xcd);
+        if (wide_index(c.nl))
+            hipLaunchKernelGGL((k_collide_f32<0, true>), dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, out, P, vox,
+                               phi, xcd);
+        else
+            hipLaunchKernelGGL((k_collide_f32<0, false>), dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, out, P, vox,
+                               phi, xcd);
     }
     return hipGetLastError();
 }

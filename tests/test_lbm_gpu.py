@@ -193,6 +193,30 @@ def test_fp32_compressed_pull_table_is_exact(monkeypatch):
         assert np.array_equal(out["0"][0], out[flag][0]) and np.array_equal(out["0"][1], out[flag][1])
 
 
+def test_fp32_wide_pull_offsets_are_exact(monkeypatch):
+    """The collide kernel's 64-bit pull offsets (taken automatically once the
+    population buffer reaches 2^31 floats, about 56 M fluid nodes; forced
+    here with PSX_LBM_WIDE=1) give the same bits as the 32-bit ones, with the
+    int32 and the compressed pull tables."""
+    ind = geo.sphere_pack((18, 14, 16), 0.4, 3.0, seed=29)
+    nz, ny, nx = ind.shape
+    inp = pio.LBMInput(B=0.7, sig=0.02, vir=0.1, vib=0.2, wet=0.3, ncx=nx, ncy=ny, ncz=nz, dt=1.0,
+                       bfr=[1e-5, 2e-6, 0], bfb=[0, 1e-5, 1e-6])
+    state = ob.equilibrium_state(inp, ob.fluid_order_mask(ind, geo.phase_blobs(ind, 0.5, 4, radius=2.5)))
+    monkeypatch.setenv("PSX_LBM_FUSED", "0")
+    for idx16 in ("0", "1"):
+        monkeypatch.setenv("PSX_LBM_IDX16", idx16)
+        out = {}
+        for wide in ("0", "1"):
+            monkeypatch.setenv("PSX_LBM_WIDE", wide)
+            _say(f"case fp32 wide={wide} int16={idx16}")
+            with LBM(inp, ind, precision="f32") as lb:
+                lb.set_state(*state)
+                lb.step(25)
+                out[wide] = lb.get_state()
+        assert np.array_equal(out["0"][0], out["1"][0]) and np.array_equal(out["0"][1], out["1"][1])
+
+
 @pytest.mark.parametrize("brick", ["8,8,16", "4,4,4", "3,5,7", "16,16,16"])
 def test_fp32_fused_brick_step_is_exact(monkeypatch, brick):
     """The fused step (colour field of each brick and its halo in LDS, then
