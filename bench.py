@@ -22,6 +22,7 @@ Also reported (objects in the same line):
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-lsm] [--no-cpu] [--no-coupled] [--full]
     python bench.py --lsm-only         # the LSM rows alone (profiling)
+    python bench.py --precision-steps N  # FP32 storage against FP64 over N steps
     python bench.py --calib-copy MIB   # plain float32 elementwise kernel (torch.neg)
                                        # over MIB MiB in and out, ten launches:
                                        # scale factor for FETCH_SIZE/WRITE_SIZE
@@ -122,6 +123,43 @@ def run_coupled(ind, inp, blue, steps=20):
             "solid_ms_per_step": (t_c - t_l) / steps * 1e3, "calc_ab_s": t_ab,
             "max_disp": float(np.abs(r).max()), "alphai_diag": np.diag(ab["alphai"]).tolist(),
             "betai": ab["betai"].tolist()}
+
+
+def run_precision(steps, chunk=2000):
+    """FP32 storage against FP64 on the headline workload over ``steps`` steps:
+    permeabilities, blue saturation and colour masses of three runs: FP64 and
+    FP32 from the same start, and, as the control, FP64 from that start with
+    every population scaled by 1 + 1e-7 u (u uniform in [-1, 1], seed 0): how
+    far a perturbation of the size of FP32 storage rounding alone moves the
+    observables of this transient."""
+    import numpy as np
+    from porescale_amd.lbm import LBM, mrt_rates
+    ind, inp, blue = lbm_workload()
+    fr0, fb0 = initial_state(inp, ind, blue)
+    coll = ("mrt", mrt_rates(inp.vir), mrt_rates(inp.vib))
+    rng = np.random.default_rng(0)
+    fr1 = fr0 * (1.0 + 1e-7 * rng.uniform(-1.0, 1.0, fr0.shape))
+    fb1 = fb0 * (1.0 + 1e-7 * rng.uniform(-1.0, 1.0, fb0.shape))
+    out = {"workload": "lbm-d3q19-twophase-mrt-256^3", "steps": steps}
+    for name, prec, st in (("f64", "f64", (fr0, fb0)), ("f64_perturbed_start", "f64", (fr1, fb1)),
+                           ("f32", "f32", (fr0, fb0))):
+        with LBM(inp, ind, collision=coll, precision=prec) as lb:
+            lb.set_state(*st)
+            done = 0
+            while done < steps:
+                n = min(chunk, steps - done)
+                lb.step(n)
+                lb.sync()
+                done += n
+                log(f"precision run {name}: {done}/{steps} steps")
+            d = lb.check_state(1, steps)
+            out[name] = {"permr": d.permr, "permb": d.permb, "sat_b": d.outm[11], "mass_r": d.tmr,
+                         "mass_b": d.tmb}
+    for k in ("permr", "permb", "sat_b", "mass_r", "mass_b"):
+        ref = out["f64"][k]
+        out[f"rel_{k}"] = {"f32": abs(out["f32"][k] - ref) / abs(ref),
+                           "f64_perturbed_start": abs(out["f64_perturbed_start"][k] - ref) / abs(ref)}
+    return out
 
 
 def cpu_baseline(ind, inp, blue, budget_s=12.0):
@@ -294,6 +332,8 @@ def main():
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB")
     ap.add_argument("--full", action="store_true", help="50-step calibration in the LSM rows")
     ap.add_argument("--lsm-only", action="store_true", help="only the LSM rows (for profiling them)")
+    ap.add_argument("--precision-steps", type=int, default=0, metavar="N",
+                    help="FP32 storage against FP64 on the headline workload over N steps (own JSON line)")
     args = ap.parse_args()
     ws = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or ws != 1:
@@ -306,6 +346,9 @@ def main():
         return
     if args.lsm_only:
         print(json.dumps({"lsm": run_lsm(args.full)}), flush=True)
+        return
+    if args.precision_steps:
+        print(json.dumps({"precision": run_precision(args.precision_steps)}), flush=True)
         return
     from porescale_amd import _lib
     dev, ncu, gb = _lib.device_info()

@@ -75,10 +75,8 @@ __device__ __forceinline__ void put(const LbmConst &c, float *__restrict__ out, 
     float r;
     if (fb == 0.0) r = 1.0f;
     else if (fr == 0.0) r = 0.0f;
-    else {
-        float ff = (float)f;
-        r = ff != 0.0f ? (float)fr * __builtin_amdgcn_rcpf(ff) : 0.5f;
-    }
+    else if (f == 0.0) r = 0.5f;
+    else r = (float)(fr / f);  // correctly rounded: an approximate reciprocal biases R and drifts colour mass
     out[aos(q, j)] = (float)(f - c.w0[q]);
     out[aos(Q + q, j)] = r;
 }
