@@ -26,8 +26,7 @@
 //                zero minority density as the reference's branch test rb<=ee
 //                requires.
 //   vox[nl]      voxel index of each node in the phi grid.
-//   phi grid     dense nx*ny*nz, z fastest (index z + nz*(y + ny*x)), FP64 in
-//                FP64 mode and FP32 in FP32 mode:
+//   phi grid     dense nx*ny*nz FP64, z fastest (index z + nz*(y + ny*x)):
 //                colour field cf at fluid voxels, `wet` at solid voxels, so
 //                the 27-point stencil of calcFs is plain loads.
 //
@@ -199,7 +198,7 @@ int launch_density(psx_lbm *h, int buf) {
     if (nb == 0) return PSX_OK;
     if (h->f32) {
         PSX_HIP(psx_lbm_fast::density(h->c, (const float *)h->d_pop[buf], h->d_dst, h->d_src16, h->d_pbase,
-                                      h->d_vox, (float *)h->d_phi, h->xcd, h->stream));
+                                      h->d_vox, (double *)h->d_phi, h->xcd, h->stream));
     } else {
         StoreF64 s{(double *)h->d_pop[buf], h->nl};
         hipLaunchKernelGGL(k_density<StoreF64>, dim3((unsigned)nb), dim3(256), 0, h->stream, h->c, s,
@@ -218,7 +217,7 @@ int launch_collide(psx_lbm *h) {
                                    h->d_bricks, h->nbricks, h->d_hnode, h->d_hcell, h->box, h->xcd, h->stream));
     } else if (h->f32) {
         PSX_HIP(psx_lbm_fast::collide(h->c, (const float *)h->d_pop[a], (float *)h->d_pop[b], h->d_dst, h->d_src16,
-                                      h->d_pbase, h->d_vox, (const float *)h->d_phi, h->xcd, h->stream));
+                                      h->d_pbase, h->d_vox, (const double *)h->d_phi, h->xcd, h->stream));
     } else {
         StoreF64 si{(double *)h->d_pop[a], h->nl}, so{(double *)h->d_pop[b], h->nl};
         hipLaunchKernelGGL(k_collide<StoreF64>, dim3((unsigned)nb), dim3(256), 0, h->stream, h->c, si, so,
@@ -394,7 +393,7 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
             }
         }
         for (int d = 0; d < 3; d++) h->box[d] = bd[d] + 2;
-        if ((int64_t)h->box[0] * h->box[1] * h->box[2] * 4 > 65536) {
+        if ((int64_t)h->box[0] * h->box[1] * h->box[2] * 8 > 65536) {
             psx::set_error("psx_lbm_create: brick halo box exceeds 64 KB of LDS");
             return PSX_EINVAL;
         }
@@ -469,7 +468,7 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
             PSX_HIP(hipMemcpyAsync(h->d_pbase, pbase.data(), sizeof(int32_t) * 3 * nx, hipMemcpyHostToDevice, h->stream));
         }
     }
-    size_t phib = (size_t)nv * (h->f32 ? sizeof(float) : sizeof(double));
+    size_t phib = (size_t)nv * sizeof(double);
     PSX_HIP(hipMalloc(&h->d_phi, phib));
     int64_t nb = std::max<int64_t>(1, psx::ceil_div(nl, 256));
     PSX_HIP(hipMalloc(&h->d_partial, sizeof(double) * NDIAG * nb));
@@ -491,11 +490,7 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
         PSX_HIP(hipMemcpyAsync(h->d_vox, vox.data(), sizeof(int32_t) * nl, hipMemcpyHostToDevice, h->stream));
     }
     int64_t gb = psx::ceil_div(nv, 256);
-    if (h->f32)
-        hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)gb), dim3(256), 0, h->stream, (float *)h->d_phi, nv,
-                           (float)p->wet);
-    else
-        hipLaunchKernelGGL(k_fill<double>, dim3((unsigned)gb), dim3(256), 0, h->stream, (double *)h->d_phi, nv, p->wet);
+    hipLaunchKernelGGL(k_fill<double>, dim3((unsigned)gb), dim3(256), 0, h->stream, (double *)h->d_phi, nv, p->wet);
     PSX_HIP(hipGetLastError());
     PSX_HIP(hipStreamSynchronize(h->stream));
     *out = guard.release();
@@ -710,7 +705,7 @@ extern "C" int psx_lbm_fields(psx_lbm *h, double *u, double *color, double *pres
         const unsigned nb = (unsigned)psx::ceil_div(nl, 256);
         if (h->f32) {
             e = psx_lbm_fast::fields(h->c, (const float *)h->d_pop[h->cur], h->d_dst, h->d_vox,
-                                     (const float *)h->d_phi, d_out, d_ph, h->stream);
+                                     (const double *)h->d_phi, d_out, d_ph, h->stream);
         } else {
             StoreF64 st{(double *)h->d_pop[h->cur], nl};
             hipLaunchKernelGGL(k_fields<StoreF64>, dim3(nb), dim3(256), 0, h->stream, h->c, st, h->d_vox,
@@ -823,7 +818,7 @@ extern "C" int psx_lbm_check_state(psx_lbm *h, int64_t it1, int64_t itw, psx_lbm
         }
         if (h->f32) {
             PSX_HIP(psx_lbm_fast::diag(h->c, (const float *)h->d_pop[h->cur], h->d_dst, h->d_vox,
-                                       (const float *)h->d_phi, h->d_partial, h->stream));
+                                       (const double *)h->d_phi, h->d_partial, h->stream));
         } else {
             StoreF64 st{(double *)h->d_pop[h->cur], h->nl};
             hipLaunchKernelGGL(k_diag<StoreF64>, dim3((unsigned)nb), dim3(256), 0, h->stream, h->c, st, h->d_vox,

@@ -21,11 +21,11 @@ struct Brick {
 // buffer of nl nodes holds 2^31 floats or more; PSX_LBM_WIDE=1 forces it)
 bool wide_index(int64_t nl);
 hipError_t collide(const psx_lbm_dev::LbmConst &c, const float *in, float *out, const int32_t *src,
-                   const int16_t *src16, const int32_t *pbase, const int32_t *vox, const float *phi, int xcd,
+                   const int16_t *src16, const int32_t *pbase, const int32_t *vox, const double *phi, int xcd,
                    hipStream_t st);
 // Pulled densities -> colour field phi at every node's voxel.
 hipError_t density(const psx_lbm_dev::LbmConst &c, const float *in, const int32_t *src, const int16_t *src16,
-                   const int32_t *pbase, const int32_t *vox, float *phi, int xcd, hipStream_t st);
+                   const int32_t *pbase, const int32_t *vox, double *phi, int xcd, hipStream_t st);
 // Fused pull + colour + collide step over bricks (k_step_f32): one launch
 // per time step.  box = the LDS box (brick + halo) dimensions (x, y, z).
 hipError_t step(const psx_lbm_dev::LbmConst &c, const float *in, float *out, const int32_t *src,
@@ -35,10 +35,10 @@ hipError_t step(const psx_lbm_dev::LbmConst &c, const float *in, float *out, con
 hipError_t link_force(const psx_lbm_dev::LbmConst &c, const float *pop, const psx_lbm_dev::CplLinks &L, hipStream_t st);
 // check_state partial sums (k_diag) of the pulled state.
 hipError_t diag(const psx_lbm_dev::LbmConst &c, const float *in, const int32_t *src, const int32_t *vox,
-                const float *phi, double *partial, hipStream_t st);
+                const double *phi, double *partial, hipStream_t st);
 
 // save_results fields (k_fields) of the pulled state: out [5][nl], phase [nl].
 hipError_t fields(const psx_lbm_dev::LbmConst &c, const float *in, const int32_t *src, const int32_t *vox,
-                  const float *phi, double *out, int32_t *phase, hipStream_t st);
+                  const double *phi, double *out, int32_t *phase, hipStream_t st);
 
 }  // namespace psx_lbm_fast

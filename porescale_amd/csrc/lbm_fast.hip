@@ -319,23 +319,23 @@ template <int M, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void k_collide_f32(const LbmConst c, const float *__restrict__ in,
                                                        float *__restrict__ out, const PullIdx P,
                                                        const int32_t *__restrict__ vox,
-                                                       const float *__restrict__ phi, int xcd) {
+                                                       const double *__restrict__ phi, int xcd) {
     int64_t b = xcd ? psx::xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
     int64_t i = b * blockDim.x + threadIdx.x;
     if (i >= c.nl) return;
-    auto fs = [&](int x, int y, int z, double *Cg, double *Fs) { calc_fs<float, false>(c, phi, x, y, z, Cg, Fs); };
+    auto fs = [&](int x, int y, int z, double *Cg, double *Fs) { calc_fs<double, false>(c, phi, x, y, z, Cg, Fs); };
     collide_pull<M, decltype(fs), WIDE>(c, in, out, P, i, vox[i], fs);
 }
 
 template <int M>
 __global__ __launch_bounds__(BLOCK) void k_density_f32(const LbmConst c, const float *__restrict__ in,
                                                        const PullIdx P, const int32_t *__restrict__ vox,
-                                                       float *__restrict__ phi, int xcd) {
+                                                       double *__restrict__ phi, int xcd) {
     int64_t b = xcd ? psx::xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
     int64_t i = b * blockDim.x + threadIdx.x;
     if (i >= c.nl) return;
     const int32_t v = vox[i];
-    phi[v] = (float)colour_pull<M>(c, in, P, i, v);
+    phi[v] = colour_pull<M>(c, in, P, i, v);
 }
 
 // One whole step over one brick of voxels per workgroup: the colour field of
@@ -350,21 +350,21 @@ __global__ __launch_bounds__(BLOCK) void k_step_f32(const LbmConst c, const floa
                                                     const int32_t *__restrict__ hnode,
                                                     const int16_t *__restrict__ hcell, int lx, int ly, int lz,
                                                     int xcd) {
-    extern __shared__ float phs[];
+    extern __shared__ double phs[];
     const int64_t t = xcd ? psx::xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
     const psx_lbm_fast::Brick B = bricks[t];
     const PullIdx P{src, nullptr, nullptr};
     const int ncell = lx * ly * lz;
-    for (int k = threadIdx.x; k < ncell; k += BLOCK) phs[k] = (float)c.wet;
+    for (int k = threadIdx.x; k < ncell; k += BLOCK) phs[k] = c.wet;
     __syncthreads();
     for (int k = threadIdx.x; k < B.hcount; k += BLOCK) {
         const int64_t i = hnode[B.hstart + k];
-        phs[hcell[B.hstart + k]] = (float)colour_pull<0>(c, in, P, i, vox[i]);
+        phs[hcell[B.hstart + k]] = colour_pull<0>(c, in, P, i, vox[i]);
     }
     __syncthreads();
     auto fs = [&](int x, int y, int z, double *Cg, double *Fs) {
         const int cell = (z - B.z0) + lz * ((y - B.y0) + ly * (x - B.x0));  // the cell of offset (-1,-1,-1)
-        auto phi3 = [&](int a, int b2, int e) -> double { return (double)phs[cell + e + lz * (b2 + ly * a)]; };
+        auto phi3 = [&](int a, int b2, int e) -> double { return phs[cell + e + lz * (b2 + ly * a)]; };
         calc_fs_window<false>(c, phi3, Cg, Fs);
     };
     for (int k = threadIdx.x; k < B.ncount; k += BLOCK) {
@@ -383,7 +383,7 @@ bool wide_index(int64_t nl) {
 }
 
 hipError_t collide(const LbmConst &c, const float *in, float *out, const int32_t *src, const int16_t *src16,
-                   const int32_t *pbase, const int32_t *vox, const float *phi, int xcd, hipStream_t st) {
+                   const int32_t *pbase, const int32_t *vox, const double *phi, int xcd, hipStream_t st) {
     int64_t nb = psx::ceil_div(c.nl, BLOCK);
     if (nb == 0) return hipSuccess;
     // the collide kernel is latency-bound (occupancy 3): it takes the direct
@@ -410,7 +410,7 @@ hipError_t collide(const LbmConst &c, const float *in, float *out, const int32_t
 }
 
 hipError_t density(const LbmConst &c, const float *in, const int32_t *src, const int16_t *src16,
-                   const int32_t *pbase, const int32_t *vox, float *phi, int xcd, hipStream_t st) {
+                   const int32_t *pbase, const int32_t *vox, double *phi, int xcd, hipStream_t st) {
     int64_t nb = psx::ceil_div(c.nl, BLOCK);
     if (nb == 0) return hipSuccess;
     const PullIdx P{src, src16, pbase};
@@ -427,7 +427,7 @@ hipError_t step(const LbmConst &c, const float *in, float *out, const int32_t *s
                 const Brick *bricks, int nbricks, const int32_t *hnode, const int16_t *hcell, const int (&box)[3],
                 int xcd, hipStream_t st) {
     if (nbricks == 0) return hipSuccess;
-    const size_t lds = sizeof(float) * (size_t)box[0] * box[1] * box[2];
+    const size_t lds = sizeof(double) * (size_t)box[0] * box[1] * box[2];
     hipLaunchKernelGGL(k_step_f32, dim3((unsigned)nbricks), dim3(BLOCK), lds, st, c, in, out, src, vox, bricks,
                        hnode, hcell, box[0], box[1], box[2], xcd);
     return hipGetLastError();
@@ -454,7 +454,7 @@ hipError_t link_force(const LbmConst &c, const float *pop, const CplLinks &L, hi
     return hipGetLastError();
 }
 
-hipError_t diag(const LbmConst &c, const float *in, const int32_t *src, const int32_t *vox, const float *phi,
+hipError_t diag(const LbmConst &c, const float *in, const int32_t *src, const int32_t *vox, const double *phi,
                 double *partial, hipStream_t st) {
     int64_t nb = psx::ceil_div(c.nl, BLOCK);
     if (nb == 0) return hipSuccess;
@@ -463,7 +463,7 @@ hipError_t diag(const LbmConst &c, const float *in, const int32_t *src, const in
     return hipGetLastError();
 }
 
-hipError_t fields(const LbmConst &c, const float *in, const int32_t *src, const int32_t *vox, const float *phi,
+hipError_t fields(const LbmConst &c, const float *in, const int32_t *src, const int32_t *vox, const double *phi,
                   double *out, int32_t *phase, hipStream_t st) {
     int64_t nb = psx::ceil_div(c.nl, BLOCK);
     if (nb == 0) return hipSuccess;
