@@ -89,6 +89,22 @@ def run_lbm(steps, warmup):
     return out, ind, inp, blue
 
 
+def run_lbm_f64(ind, inp, blue, steps=50, warmup=5):
+    """The same workload on the FP64 storage path, whose states are
+    bit-identical to the reference's: what exact parity costs."""
+    from porescale_amd.lbm import LBM, mrt_rates
+    coll = ("mrt", mrt_rates(inp.vir), mrt_rates(inp.vib))
+    with LBM(inp, ind, collision=coll, precision="f64") as lb:
+        lb.set_state(*initial_state(inp, ind, blue))
+        lb.step(warmup)
+        lb.sync()
+        tot, tc, td = lb.profile(steps)
+        nl = lb.nl
+    return {"storage": "f64 (bit-identical to the reference)", "steps": steps, "ms_per_step": tot / steps,
+            "collide_ms": tc, "density_ms": td, "mlups": nl * steps / (tot * 1e-3) / 1e6,
+            "algorithmic_bytes_per_node": 624.0, "collide_gbs": 624.0 * nl / (tc * 1e-3) / 1e9}
+
+
 def run_coupled(ind, inp, blue, steps=20):
     """L1's coupled model on the same 256^3 workload: each main-loop step is
     the FP32 LBM step, the bounce-back momentum exchange and lsm_step of the
@@ -387,6 +403,8 @@ def main():
                      "mean_launch_ms": lbm["coll_ms"]},
         "lbm_diag": {"permr": lbm["permr"], "permb": lbm["permb"], "sat_b": lbm["sat_b"]},
     }
+    out["lbm_f64"] = run_lbm_f64(ind, inp, blue)
+    log(f"LBM f64 storage {json.dumps(out['lbm_f64'])}")
     if args.coupled:
         out["coupled"] = run_coupled(ind, inp, blue)
         log(f"coupled {json.dumps(out['coupled'])}")
