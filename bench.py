@@ -9,6 +9,8 @@ is one pull+collide launch and one pull+density launch; the roofline entry is
 the collide kernel, timed with HIP events on the library's own stream.
 
 Also reported (objects in the same line):
+  lbm_f64      the same LBM workload with FP64 storage (states bit-identical
+               to the reference): ms per step and MLUPS.
   lsm          64^3 single-solid sphere pack (porosity 0.35), all six load
                cases at rtol 1e-10 and 1e-5: plain FP64 PCG one case at a time
                versus the batched FP32 multigrid-PCG / FP64-refinement solver,
