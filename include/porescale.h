@@ -106,6 +106,8 @@ int psx_lbm_init_random(psx_lbm *h, double sat);
    the content of iteration2 (L1:3951-3961, read L1:2845-2850). */
 int psx_lbm_set_state(psx_lbm *h, const double *fr, const double *fb);
 int psx_lbm_get_state(psx_lbm *h, double *fr, double *fb);
+/* Red and blue densities and the colour field of the current state, nl each
+   in the reference's node order (rr, rb, cf of L1:3436-3447). */
 int psx_lbm_get_fields(psx_lbm *h, double *rr, double *rb, double *cf);
 /* nsteps time steps of L1:3634-3947 (pre-steps included: the caller adds 2). */
 int psx_lbm_step(psx_lbm *h, int64_t nsteps);
@@ -118,6 +120,8 @@ int psx_lbm_check_state(psx_lbm *h, int64_t it1, int64_t itw, psx_lbm_diag *out)
    ind_red_blue code (L1:2080-2088).  Solid voxels get u = 0, color = 1.2,
    pressure = 0, phase = 0.  Any output may be NULL. */
 int psx_lbm_fields(psx_lbm *h, double *u, double *color, double *pressure, int32_t *phase);
+/* Waits for the work queued on the handle's stream (psx_lbm_step returns as
+   soon as the steps are queued). */
 int psx_lbm_sync(psx_lbm *h);
 /* Runs nsteps and times each kernel with HIP events on the handle's stream.
    ms[0] = total wall ms (event to event), ms[1] = mean collide kernel ms,
@@ -209,7 +213,10 @@ int psx_lsm_adjoint(psx_lsm *h, double wK, double wG, double rtol, int32_t maxit
 int psx_lsm_profile(psx_lsm *h, int32_t n, double *ms);
 
 /* Batched moduli: nvol sub-volumes, each with its own dims and voxel codes
-   (concatenated), all six load cases solved together in persistent launches.
+   (concatenated), all six load cases solved together: each kernel launch
+   serves every active (volume, load case) system.  ms_out[2] (may be NULL):
+   device ms of right-hand sides, solve and stresses, and the inner
+   iterations.
    C_out: nvol*36 (column-major 6x6 each); KG_out: nvol*2. */
 int psx_lsm_batch_moduli(const psx_lsm_params *p, int32_t nvol, const int32_t *dims,
                          const int32_t *ind_concat, double rtol, int32_t maxit,
