@@ -181,10 +181,15 @@ def run_precision(steps, chunk=2000):
 
 
 def cpu_baseline(ind, inp, blue, budget_s=12.0):
-    """Oracle (1 thread) on the same 256^3 workload: steps until the budget."""
+    """Oracle on the same 256^3 workload, its node and link loops on this
+    process's cores (at most 16, the host share of one GPU): steps until the
+    budget."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind as ob
     from porescale_amd.lbm import mrt_rates
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    ob.lib().or_set_threads(threads)
+    threads = ob.lib().or_num_threads()
     S = (mrt_rates(inp.vir), mrt_rates(inp.vib))
     o = ob.OracleLBM(inp, ind, S=S)
     o.set_state(*initial_state(inp, ind, blue))
@@ -194,8 +199,8 @@ def cpu_baseline(ind, inp, blue, budget_s=12.0):
         o.step(1)
         t += time.perf_counter() - t0
         n += 1
-    return {"value": o.nl * n / t / 1e6, "unit": "MLUPS", "cores": 1, "kind": "port",
-            "sample": f"{n} steps of the same 256^3 two-phase MRT workload, oracle in C, 1 thread"}
+    return {"value": o.nl * n / t / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"{n} steps of the same 256^3 two-phase MRT workload, oracle in C, {threads} OpenMP threads"}
 
 
 def _plain_rate(ind, cases, rtol, pars=(1.0, 0.1), maxit=200000, stiff=None, end_to_end=False):

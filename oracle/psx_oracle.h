@@ -103,6 +103,10 @@ or_lsm *or_lsm_create(const or_lsm_params *p, const int32_t *ind, const int32_t 
                       const int32_t *cube_ang, const double *cube_sbond,
                       const double *cube_abond);
 void or_lsm_destroy(or_lsm *h);
+/* threads the LBM step loops use (OpenMP; 1 without it): results do not
+   depend on it */
+int or_num_threads(void);
+void or_set_threads(int n);
 int64_t or_lsm_count(const or_lsm *h, int32_t which); /* 0 ns,1 nbo,2 nna,3 Gx,4 Gy,5 Gz */
 /* the z-layer groups (XX:522-548): node, bond and angle offsets, 2*nthread+1 each */
 void or_lsm_layers(const or_lsm *h, int64_t *layb, int64_t *lib, int64_t *lia);

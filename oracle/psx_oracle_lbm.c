@@ -7,6 +7,9 @@
  * without FP contraction, so the state after N steps is bit-identical to the
  * flang-compiled reference on the same inputs.
  */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include "psx_oracle.h"
 #include <math.h>
 #include <stdlib.h>
@@ -529,6 +532,7 @@ void or_lbm_fluid_force(const or_lbm *h, double rsi, double *af) {
 
 /* density / colour field pass (L1:3436-3447, 3934-3945) */
 void or_lbm_densities(or_lbm *h) {
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < h->nl; i++) {
         const double *a = &h->fr[19 * i], *b = &h->fb[19 * i];
         double sr = a[0], sb = b[0];
@@ -553,6 +557,9 @@ static void lbm_one_step(or_lbm *h) {
         Fpr[q] = rk[q] * sr * dt2;
         Fpb[q] = rk[q] * sb * dt2;
     }
+    /* every loop below writes only its own node's (or its own link's target)
+       slots, so threads change no result */
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < h->nl; i++) {
         double *fri = &h->fr[19 * i], *fbi = &h->fb[19 * i];
         double *pri = &h->pcfr[19 * i], *pbi = &h->pcfb[19 * i];
@@ -607,18 +614,21 @@ static void lbm_one_step(or_lbm *h) {
             }
         }
     }
+#pragma omp parallel for schedule(static)
     for (int64_t s = 0; s < h->nps; s++) {
         const int32_t *e = &h->ips[3 * s];
         int64_t from = e[0] - 1, to = e[1] - 1, q = e[2] - 1;
         h->fr[19 * to + q] = h->pcfr[19 * from + q];
         h->fb[19 * to + q] = h->pcfb[19 * from + q];
     }
+#pragma omp parallel for schedule(static)
     for (int64_t s = 0; s < h->npb2; s++) {
         const int32_t *e = &h->ipb2[4 * s];
         int64_t n = e[0] - 1, qb = e[1] - 1, q = h->niv[qb];
         h->fr[19 * n + qb] = h->pcfr[19 * n + q];
         h->fb[19 * n + qb] = h->pcfb[19 * n + q];
     }
+#pragma omp parallel for schedule(static)
     for (int64_t s = 0; s < h->npb4; s++) {
         const int32_t *e = &h->ipb4[6 * s];
         int64_t n = e[0] - 1, qb = e[1] - 1, q = h->niv[qb];
@@ -626,6 +636,22 @@ static void lbm_one_step(or_lbm *h) {
         h->fb[19 * n + qb] = h->pcfb[19 * n + q];
     }
     or_lbm_densities(h);
+}
+
+void or_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+int or_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 void or_lbm_step(or_lbm *h, int64_t nsteps) {

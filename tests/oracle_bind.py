@@ -80,6 +80,9 @@ def lib():
         L.or_lsm_create.argtypes = [C.POINTER(LSMParams), _P(np.int32), _P(np.int32), _P(np.int32),
                                     _P(np.float64), _P(np.float64)]
         L.or_lsm_destroy.argtypes = [vp]
+        L.or_num_threads.restype = C.c_int
+        L.or_num_threads.argtypes = []
+        L.or_set_threads.argtypes = [C.c_int]
         L.or_lsm_count.restype = C.c_int64
         L.or_lsm_count.argtypes = [vp, C.c_int32]
         L.or_lsm_get_nodes.argtypes = [vp, vp, vp]

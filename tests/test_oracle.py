@@ -359,3 +359,27 @@ def test_library_table_counts_equal_oracle(seed):
                 assert (t["nbo"], t["nna"], t["ns"]) == (c["nbo"], c["nna"], c["ns"]), (v.shape, case, nth)
                 assert np.array_equal(t["layb"], layb) and np.array_equal(t["lib"], lib)
                 assert np.array_equal(t["lia"], lia) and int(t["layers"].sum()) == c["Gz"]
+
+
+def test_oracle_lbm_threads_change_no_bit():
+    """The oracle's LBM step runs its node and link loops with OpenMP (for the
+    CPU baseline on all host cores); every loop writes only its own slots, so
+    1 and 4 threads give the same bits."""
+    ind = geo.sphere_pack((14, 12, 16), 0.45, 2.5, seed=5)
+    nz, ny, nx = ind.shape
+    inp = pio.LBMInput(B=0.7, sig=0.02, vir=0.1, vib=0.2, wet=0.3, ncx=nx, ncy=ny, ncz=nz, dt=1.0,
+                       bfr=[1e-5, 0, 2e-6], bfb=[0, 1e-5, 0])
+    state = ob.equilibrium_state(inp, ob.fluid_order_mask(ind, geo.phase_blobs(ind, 0.5, 3, radius=2.0)))
+    L = ob.lib()
+    out = []
+    try:
+        for n in (1, 4):
+            L.or_set_threads(n)
+            assert L.or_num_threads() == n
+            o = ob.OracleLBM(inp, ind)
+            o.set_state(*state)
+            o.step(15)
+            out.append(o.state())
+    finally:
+        L.or_set_threads(os.cpu_count() or 1)
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
