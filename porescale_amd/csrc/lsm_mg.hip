@@ -433,6 +433,82 @@ __global__ __launch_bounds__(TILE) void k_cheb_step(LevelArgs L, int R, const Sy
     }
 }
 
+// The same STEP on a stencil level for all right-hand sides at once: each
+// thread reads its node's 27 stencil blocks (243 floats) once and applies
+// them to every active right-hand side.  One grid row per right-hand side
+// would stream the stencil R times per step; at 128^3 the first coarse
+// level's stencil alone is 255 MB.
+template <bool NEWD>
+__global__ __launch_bounds__(TILE) void k_cheb_step_rall(LevelArgs L, int R, const SysState *__restrict__ sys,
+                                                         int k, float *__restrict__ xv, float *__restrict__ rv,
+                                                         const float *__restrict__ dv, float *__restrict__ dout) {
+    for (int t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
+        const int v = L.tvol[t];
+        const VolDesc vd = L.vols[v];
+        int x, y, z;
+        bool ok;
+        tile_node(vd, L.tstart[t], x, y, z, ok);
+        if (!ok) continue;
+        const int64_t n = node_index(vd, x, y, z);
+        bool act[MAXR];
+        bool any = false;
+#pragma unroll
+        for (int r = 0; r < MAXR; r++) {
+            act[r] = r < R && sys_active(sys, v, R, r);
+            any |= act[r];
+        }
+        if (!any) continue;
+        float q[MAXR][3];
+#pragma unroll
+        for (int r = 0; r < MAXR; r++) q[r][0] = q[r][1] = q[r][2] = 0.f;
+#pragma unroll 1
+        for (int dz = -1; dz <= 1; dz++) {
+            const int zz = wrap1(z + dz, vd.nz);
+#pragma unroll 1
+            for (int dy = -1; dy <= 1; dy++) {
+                const int yy = wrap1(y + dy, vd.ny);
+#pragma unroll
+                for (int dx = -1; dx <= 1; dx++) {
+                    const int xx = wrap1(x + dx, vd.nx);
+                    const int s = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+                    const int64_t nb = node_index(vd, xx, yy, zz);
+                    float Ab[9];
+#pragma unroll
+                    for (int j = 0; j < 9; j++) Ab[j] = L.A[(int64_t)(9 * s + j) * L.N + n];
+#pragma unroll
+                    for (int r = 0; r < MAXR; r++) {
+                        if (!act[r]) continue;
+                        const float *u = dv + (int64_t)r * 3 * L.N;
+                        const float u0 = u[nb], u1 = u[L.N + nb], u2 = u[2 * L.N + nb];
+#pragma unroll
+                        for (int e = 0; e < 3; e++) q[r][e] += Ab[3 * e] * u0 + Ab[3 * e + 1] * u1 + Ab[3 * e + 2] * u2;
+                    }
+                }
+            }
+        }
+        float c1 = 0.f, c2 = 0.f;
+        if (NEWD) cheb_coef(L.lam[v], L.ratio, k, c1, c2);
+#pragma unroll
+        for (int r = 0; r < MAXR; r++) {
+            if (!act[r]) continue;
+            const int64_t base = (int64_t)r * 3 * L.N;
+            float rr[3], dd[3];
+            for (int e = 0; e < 3; e++) {
+                const int64_t i = base + e * L.N + n;
+                dd[e] = dv[i];
+                xv[i] += dd[e];
+                rr[e] = rv[i] - q[r][e];
+                rv[i] = rr[e];
+            }
+            if (NEWD) {
+                float zz[3];
+                apply_minv(L.minv, L.N, n, rr, zz);
+                for (int e = 0; e < 3; e++) dout[base + e * L.N + n] = c1 * dd[e] + c2 * zz[e];
+            }
+        }
+    }
+}
+
 // Level-0 restart that also ends the V-cycle (smoothing degree 1 on the
 // finest level): x += e; r -= A e; z = x + D^-1 r / theta, written to x (the
 // caller's z), with the (tile, rhs) partials of r_cg.z into prz.
@@ -678,12 +754,20 @@ LevelArgs args_of(const MGLevel &l, const Batch &B, float ratio) {
     return a;
 }
 
+int env_int(const char *name, int def);
+
 template <bool NEWD>
 void launch_step(hipStream_t st, const LevelArgs &L, bool elem, int R, const SysState *sys, int k, float *x, float *r,
                  const float *d, float *dout) {
     int g = grid_for(L.ntiles);
+    // All right-hand sides per thread only where the level alone fills the
+    // chip: with fewer tiles the R grid rows are the parallelism (64^3:
+    // 2x slower with it on every level, 128^3: 15% faster on the 64^3 level).
+    static const int rall_tiles = env_int("PSX_MG_RALL_TILES", 1024);
     if (elem)
         hipLaunchKernelGGL((k_cheb_step<true, NEWD>), dim3(g, R), dim3(TILE), 0, st, L, R, sys, k, x, r, d, dout);
+    else if (rall_tiles > 0 && L.ntiles >= rall_tiles)
+        hipLaunchKernelGGL((k_cheb_step_rall<NEWD>), dim3(g), dim3(TILE), 0, st, L, R, sys, k, x, r, d, dout);
     else
         hipLaunchKernelGGL((k_cheb_step<false, NEWD>), dim3(g, R), dim3(TILE), 0, st, L, R, sys, k, x, r, d, dout);
 }
