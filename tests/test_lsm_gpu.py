@@ -141,6 +141,26 @@ def test_moduli_symmetry_and_batch_equals_single():
             assert kg[0] == pytest.approx(K, rel=1e-8) and kg[1] == pytest.approx(G, rel=1e-8)
 
 
+def test_all_rhs_coarse_step_batch_equals_single():
+    """Eight 64^3 volumes in one FP32 multigrid batch: the first coarse level
+    holds 8 x 32^3 nodes = 1024 tiles, so its Chebyshev steps apply each
+    stencil block to all six right-hand sides in one thread; each volume alone
+    (128 tiles there) takes one grid row per right-hand side.  Both converge
+    to the same FP64-refined moduli."""
+    vols = [geo.two_component(geo.sphere_pack((64, 64, 64), 0.35, 5.0, seed=60 + i), 0.5, seed=70 + i,
+                              radius=4.0) for i in range(8)]
+    pars = (1.0, 0.1, 0.3, 0.02)
+    singles = []
+    for v in vols:
+        with LSM(v, *pars, precision="f32") as s:
+            singles.append(s.moduli(rtol=1e-10))
+    _say("case batch of eight 64^3 volumes (all right-hand sides per thread on level 1)")
+    Cb, KG, ms, its = batch_moduli(vols, *pars, precision="f32", rtol=1e-10)
+    for (C6, K, G), cb, kg in zip(singles, Cb, KG):
+        np.testing.assert_allclose(cb, C6, atol=1e-8 * np.abs(C6).max())
+        assert kg[0] == pytest.approx(K, rel=1e-8) and kg[1] == pytest.approx(G, rel=1e-8)
+
+
 def test_adjoint_closed_form_homogeneous():
     a, b = 1.0, 0.1
     ind = np.zeros((6, 6, 6), np.int32)
