@@ -291,7 +291,8 @@ __device__ inline void op_node(const LevelArgs &L, const VolDesc &vd, int x, int
 #pragma unroll 1
         for (int dz = -1; dz <= 1; dz++) {
             int zz = wrap1(z + dz, vd.nz);
-#pragma unroll(UYZ >= 3 ? 3 : 1)
+            constexpr int kUnrollY = UYZ >= 3 ? 3 : 1;
+#pragma unroll kUnrollY
             for (int dy = -1; dy <= 1; dy++) {
                 int yy = wrap1(y + dy, vd.ny);
 #pragma unroll
@@ -338,7 +339,7 @@ __global__ __launch_bounds__(TILE) void k_cheb_first(LevelArgs L, int R, const S
             apply_minv(L.minv, L.N, n, bb, zz);
             for (int e = 0; e < 3; e++) {
                 rv[base + e * L.N + n] = bb[e];
-                xv[base + e * L.N + n] = 0.f;
+                if (xv) xv[base + e * L.N + n] = 0.f;
                 dv[base + e * L.N + n] = c2 * zz[e];
             }
         }
@@ -384,7 +385,7 @@ __global__ __launch_bounds__(TILE) void k_cg_update_first(LevelArgs L, int R, co
             for (int e = 0; e < 3; e++) {
                 const int64_t i = base + e * L.N + n;
                 rv[i] = bb[e];
-                xv[i] = 0.f;
+                if (xv) xv[i] = 0.f;
                 dv[i] = c2 * zz[e];
             }
         }
@@ -393,9 +394,10 @@ __global__ __launch_bounds__(TILE) void k_cg_update_first(LevelArgs L, int R, co
     }
 }
 
-// STEP k: x += d; r -= A d; (NEWD) d' = c1 d + c2 D^-1 r.  k = -1 restarts
-// the recurrence (d' = D^-1 r / theta) after a coarse correction d.
-template <bool ELEM, bool NEWD, int UYZ = 1>
+// STEP k: x += d (X0: x = d, the pass's first step from x = 0); r -= A d;
+// (NEWD) d' = c1 d + c2 D^-1 r.  k = -1 restarts the recurrence
+// (d' = D^-1 r / theta) after a coarse correction d.
+template <bool ELEM, bool NEWD, int UYZ = 1, bool X0 = false>
 __global__ __launch_bounds__(TILE) void k_cheb_step(LevelArgs L, int R, const SysState *__restrict__ sys, int k,
                                                     float *__restrict__ xv, float *__restrict__ rv,
                                                     const float *__restrict__ dv, float *__restrict__ dout) {
@@ -422,7 +424,8 @@ __global__ __launch_bounds__(TILE) void k_cheb_step(LevelArgs L, int R, const Sy
             for (int e = 0; e < 3; e++) {
                 const int64_t i = base + e * L.N + n;
                 dd[e] = dv[i];
-                xv[i] += dd[e];
+                if (X0) xv[i] = dd[e];
+                else xv[i] += dd[e];
                 rr[e] = rv[i] - q[e];
                 rv[i] = rr[e];
             }
@@ -440,7 +443,7 @@ __global__ __launch_bounds__(TILE) void k_cheb_step(LevelArgs L, int R, const Sy
 // them to every active right-hand side.  One grid row per right-hand side
 // would stream the stencil R times per step; at 128^3 the first coarse
 // level's stencil alone is 255 MB.
-template <bool NEWD>
+template <bool NEWD, bool X0 = false>
 __global__ __launch_bounds__(TILE) void k_cheb_step_rall(LevelArgs L, int R, const SysState *__restrict__ sys,
                                                          int k, float *__restrict__ xv, float *__restrict__ rv,
                                                          const float *__restrict__ dv, float *__restrict__ dout) {
@@ -498,7 +501,8 @@ __global__ __launch_bounds__(TILE) void k_cheb_step_rall(LevelArgs L, int R, con
             for (int e = 0; e < 3; e++) {
                 const int64_t i = base + e * L.N + n;
                 dd[e] = dv[i];
-                xv[i] += dd[e];
+                if (X0) xv[i] = dd[e];
+                else xv[i] += dd[e];
                 rr[e] = rv[i] - q[r][e];
                 rv[i] = rr[e];
             }
@@ -601,7 +605,7 @@ __global__ __launch_bounds__(TILE) void k_restrict_first(LevelArgs F, LevelArgs 
             for (int e = 0; e < 3; e++) {
                 bc[base + e * C.N + I] = s[e];
                 rc[base + e * C.N + I] = s[e];
-                xc[base + e * C.N + I] = 0.f;
+                if (xc) xc[base + e * C.N + I] = 0.f;
                 dc[base + e * C.N + I] = c2 * zz[e];
             }
         }
@@ -758,9 +762,9 @@ LevelArgs args_of(const MGLevel &l, const Batch &B, float ratio) {
 
 int env_int(const char *name, int def);
 
-template <bool NEWD>
-void launch_step(hipStream_t st, const LevelArgs &L, bool elem, int R, const SysState *sys, int k, float *x, float *r,
-                 const float *d, float *dout) {
+template <bool NEWD, bool X0>
+void launch_step_t(hipStream_t st, const LevelArgs &L, bool elem, int R, const SysState *sys, int k, float *x,
+                   float *r, const float *d, float *dout) {
     int g = grid_for(L.ntiles);
     // All right-hand sides per thread only where the level alone fills the
     // chip: with fewer tiles the R grid rows are the parallelism (64^3:
@@ -768,17 +772,34 @@ void launch_step(hipStream_t st, const LevelArgs &L, bool elem, int R, const Sys
     static const int rall_tiles = env_int("PSX_MG_RALL_TILES", 1024);
     static const int uyz = env_int("PSX_MG_UYZ", 3);
     if (elem)
-        hipLaunchKernelGGL((k_cheb_step<true, NEWD>), dim3(g, R), dim3(TILE), 0, st, L, R, sys, k, x, r, d, dout);
+        hipLaunchKernelGGL((k_cheb_step<true, NEWD, 1, X0>), dim3(g, R), dim3(TILE), 0, st, L, R, sys, k, x, r, d,
+                           dout);
     else if (rall_tiles > 0 && L.ntiles >= rall_tiles)
-        hipLaunchKernelGGL((k_cheb_step_rall<NEWD>), dim3(g), dim3(TILE), 0, st, L, R, sys, k, x, r, d, dout);
+        hipLaunchKernelGGL((k_cheb_step_rall<NEWD, X0>), dim3(g), dim3(TILE), 0, st, L, R, sys, k, x, r, d, dout);
     else if (uyz >= 3)
-        hipLaunchKernelGGL((k_cheb_step<false, NEWD, 3>), dim3(g, R), dim3(TILE), 0, st, L, R, sys, k, x, r, d, dout);
+        hipLaunchKernelGGL((k_cheb_step<false, NEWD, 3, X0>), dim3(g, R), dim3(TILE), 0, st, L, R, sys, k, x, r, d,
+                           dout);
     else
-        hipLaunchKernelGGL((k_cheb_step<false, NEWD>), dim3(g, R), dim3(TILE), 0, st, L, R, sys, k, x, r, d, dout);
+        hipLaunchKernelGGL((k_cheb_step<false, NEWD, 1, X0>), dim3(g, R), dim3(TILE), 0, st, L, R, sys, k, x, r, d,
+                           dout);
 }
 
-// V-cycle from level l, whose b, r, x, dA hold a started smoothing pass
-// (r = b, x = 0, dA = d_0).  Returns the pending last update of x.
+// x0: the level's x is zero (the first step of a smoothing pass from x = 0),
+// so the step stores x = d without reading x
+template <bool NEWD>
+void launch_step(hipStream_t st, const LevelArgs &L, bool elem, int R, const SysState *sys, int k, float *x, float *r,
+                 const float *d, float *dout, bool x0) {
+    if (x0) launch_step_t<NEWD, true>(st, L, elem, R, sys, k, x, r, d, dout);
+    else launch_step_t<NEWD, false>(st, L, elem, R, sys, k, x, r, d, dout);
+}
+
+// V-cycle from level l, whose b, r, dA hold a started smoothing pass
+// (r = b, dA = d_0) and whose x is taken as 0: the first step stores x = d,
+// so x needs zeroing only on a level that takes no step (a coarsest level of
+// degree 1, see x_untouched).  Returns the pending last update of x.
+// Level l takes no smoothing step in a V-cycle, so its x must start zeroed.
+bool x_untouched(const MG &mg, int l) { return l + 1 == (int)mg.lv.size() && mg.prm.coarse_deg < 2; }
+
 float *vcycle(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, int l, const float *rcg = nullptr,
               double *prz = nullptr, bool *finished = nullptr) {
     MGLevel &lv = mg.lv[l];
@@ -787,9 +808,11 @@ float *vcycle(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, int l
     const MGParams &p = mg.prm;
     LevelArgs L = args_of(lv, B, coarsest ? p.coarse_ratio : p.ratio);
     float *dA = lv.dA, *dB = lv.dB;
+    bool x0 = true;
     if (coarsest) {
         for (int k = 0; k + 1 < p.coarse_deg; k++) {
-            launch_step<true>(st, L, elem, mg.R, sys, k, lv.x, lv.r, dA, dB);
+            launch_step<true>(st, L, elem, mg.R, sys, k, lv.x, lv.r, dA, dB, x0);
+            x0 = false;
             std::swap(dA, dB);
         }
         return dA;
@@ -797,16 +820,17 @@ float *vcycle(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, int l
     const int nu = l == 0 ? p.nu0 : p.nu;
     // pre-smoothing: nu updates, the last one with the residual update
     for (int k = 0; k + 1 < nu; k++) {
-        launch_step<true>(st, L, elem, mg.R, sys, k, lv.x, lv.r, dA, dB);
+        launch_step<true>(st, L, elem, mg.R, sys, k, lv.x, lv.r, dA, dB, x0);
+        x0 = false;
         std::swap(dA, dB);
     }
-    launch_step<false>(st, L, elem, mg.R, sys, 0, lv.x, lv.r, dA, nullptr);
+    launch_step<false>(st, L, elem, mg.R, sys, 0, lv.x, lv.r, dA, nullptr, x0);
     // coarse correction
     MGLevel &cv = mg.lv[l + 1];
     const bool c_coarsest = l + 2 == (int)mg.lv.size();
     LevelArgs C = args_of(cv, B, c_coarsest ? p.coarse_ratio : p.ratio);
     hipLaunchKernelGGL(k_restrict_first, dim3(grid_for(C.ntiles), mg.R), dim3(TILE), 0, st, L, C, mg.R, sys, lv.r, cv.b,
-                       cv.x, cv.r, cv.dA);
+                       x_untouched(mg, l + 1) ? cv.x : nullptr, cv.r, cv.dA);
     float *pend = vcycle(st, B, mg, sys, l + 1);
     hipLaunchKernelGGL(k_prolong, dim3(grid_for(L.ntiles), mg.R), dim3(TILE), 0, st, L, C, mg.R, sys, cv.x,
                        (const float *)pend, dA);
@@ -817,10 +841,10 @@ float *vcycle(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, int l
         *finished = true;
         return nullptr;
     }
-    launch_step<true>(st, L, elem, mg.R, sys, -1, lv.x, lv.r, dA, dB);
+    launch_step<true>(st, L, elem, mg.R, sys, -1, lv.x, lv.r, dA, dB, false);
     std::swap(dA, dB);
     for (int k = 0; k + 1 < nu; k++) {
-        launch_step<true>(st, L, elem, mg.R, sys, k, lv.x, lv.r, dA, dB);
+        launch_step<true>(st, L, elem, mg.R, sys, k, lv.x, lv.r, dA, dB, false);
         std::swap(dA, dB);
     }
     return dA;
@@ -1028,10 +1052,11 @@ void mg_apply(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, const
     l0.x = z;
     if (upd)
         hipLaunchKernelGGL(k_cg_update_first, dim3(grid_for(L.ntiles), mg.R), dim3(TILE), 0, st, L, mg.R, sys, upd->X,
-                           const_cast<float *>(r), upd->P, upd->Q, upd->prr, z, l0.r, l0.dA);
+                           const_cast<float *>(r), upd->P, upd->Q, upd->prr, x_untouched(mg, 0) ? z : nullptr, l0.r,
+                           l0.dA);
     else
         hipLaunchKernelGGL(k_cheb_first, dim3(grid_for(L.ntiles), mg.R), dim3(TILE), 0, st, L, mg.R, sys,
-                           (const float *)r, z, l0.r, l0.dA);
+                           (const float *)r, x_untouched(mg, 0) ? z : nullptr, l0.r, l0.dA);
     bool finished = false;
     float *pend = vcycle(st, B, mg, sys, 0, r, prz, &finished);
     if (!finished)

@@ -161,6 +161,25 @@ def test_all_rhs_coarse_step_batch_equals_single():
         assert kg[0] == pytest.approx(K, rel=1e-8) and kg[1] == pytest.approx(G, rel=1e-8)
 
 
+@pytest.mark.parametrize("levels", ["1", "4"])
+def test_multigrid_coarsest_degree_one_same_moduli(monkeypatch, levels):
+    """A coarsest level of Chebyshev degree 1 takes no smoothing step, so its
+    x must be zeroed where the pass starts (every other level's first step
+    stores x = d).  One level: the CG update zeroes the finest x; four: the
+    restriction zeroes the coarsest."""
+    ind = geo.two_component(geo.sphere_pack((24, 20, 22), 0.35, 3.0, seed=3), 0.3, seed=4, radius=2.0)
+    pars = (1.0, 0.1, 0.3, 0.02)
+    with LSM(ind, *pars, precision="f32") as s:
+        C6, K, G = s.moduli(rtol=1e-10)
+    monkeypatch.setenv("PSX_MG_CDEG", "1")
+    monkeypatch.setenv("PSX_MG_LEVELS", levels)
+    _say(f"case coarsest degree 1, {levels} level(s)")
+    with LSM(ind, *pars, precision="f32") as s:
+        C1, K1, G1 = s.moduli(rtol=1e-10, maxit=200000)
+    np.testing.assert_allclose(C1, C6, atol=1e-8 * np.abs(C6).max())
+    assert K1 == pytest.approx(K, rel=1e-8) and G1 == pytest.approx(G, rel=1e-8)
+
+
 def test_adjoint_closed_form_homogeneous():
     a, b = 1.0, 0.1
     ind = np.zeros((6, 6, 6), np.int32)
