@@ -338,8 +338,10 @@ __global__ __launch_bounds__(TILE) void k_cheb_first(LevelArgs L, int R, const S
             for (int e = 0; e < 3; e++) bb[e] = bv[base + e * L.N + n];
             apply_minv(L.minv, L.N, n, bb, zz);
             for (int e = 0; e < 3; e++) {
-                rv[base + e * L.N + n] = bb[e];
-                if (xv) xv[base + e * L.N + n] = 0.f;
+                if (xv) {  // no step follows: the pass is x = 0, r = b
+                    xv[base + e * L.N + n] = 0.f;
+                    rv[base + e * L.N + n] = bb[e];
+                }
                 dv[base + e * L.N + n] = c2 * zz[e];
             }
         }
@@ -384,8 +386,10 @@ __global__ __launch_bounds__(TILE) void k_cg_update_first(LevelArgs L, int R, co
             apply_minv(L.minv, L.N, n, bb, zz);
             for (int e = 0; e < 3; e++) {
                 const int64_t i = base + e * L.N + n;
-                rv[i] = bb[e];
-                if (xv) xv[i] = 0.f;
+                if (xv) {  // no step follows: the pass is x = 0, r = b
+                    xv[i] = 0.f;
+                    rv[i] = bb[e];
+                }
                 dv[i] = c2 * zz[e];
             }
         }
@@ -400,7 +404,9 @@ __global__ __launch_bounds__(TILE) void k_cg_update_first(LevelArgs L, int R, co
 template <bool ELEM, bool NEWD, int UYZ = 1, bool X0 = false>
 __global__ __launch_bounds__(TILE) void k_cheb_step(LevelArgs L, int R, const SysState *__restrict__ sys, int k,
                                                     float *__restrict__ xv, float *__restrict__ rv,
-                                                    const float *__restrict__ dv, float *__restrict__ dout) {
+                                                    const float *__restrict__ dv, float *__restrict__ dout,
+                                                    const float *rin) {
+    const float *rs = X0 && rin ? rin : rv;
     for (int t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
         const int v = L.tvol[t];
         const VolDesc vd = L.vols[v];
@@ -426,7 +432,7 @@ __global__ __launch_bounds__(TILE) void k_cheb_step(LevelArgs L, int R, const Sy
                 dd[e] = dv[i];
                 if (X0) xv[i] = dd[e];
                 else xv[i] += dd[e];
-                rr[e] = rv[i] - q[e];
+                rr[e] = rs[i] - q[e];
                 rv[i] = rr[e];
             }
             if (NEWD) {
@@ -446,7 +452,9 @@ __global__ __launch_bounds__(TILE) void k_cheb_step(LevelArgs L, int R, const Sy
 template <bool NEWD, bool X0 = false>
 __global__ __launch_bounds__(TILE) void k_cheb_step_rall(LevelArgs L, int R, const SysState *__restrict__ sys,
                                                          int k, float *__restrict__ xv, float *__restrict__ rv,
-                                                         const float *__restrict__ dv, float *__restrict__ dout) {
+                                                         const float *__restrict__ dv, float *__restrict__ dout,
+                                                         const float *rin) {
+    const float *rs = X0 && rin ? rin : rv;
     for (int t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
         const int v = L.tvol[t];
         const VolDesc vd = L.vols[v];
@@ -503,7 +511,7 @@ __global__ __launch_bounds__(TILE) void k_cheb_step_rall(LevelArgs L, int R, con
                 dd[e] = dv[i];
                 if (X0) xv[i] = dd[e];
                 else xv[i] += dd[e];
-                rr[e] = rv[i] - q[r][e];
+                rr[e] = rs[i] - q[r][e];
                 rv[i] = rr[e];
             }
             if (NEWD) {
@@ -565,12 +573,12 @@ __global__ __launch_bounds__(TILE) void k_restart_final0(LevelArgs L, int R, con
     }
 }
 
-// coarse node I: b = P^T r_fine; r = b; x = 0; d = D^-1 b / theta
+// coarse node I: r = P^T r_fine (the level's right-hand side; no copy is
+// kept); x = 0 when xc is given; d = D^-1 r / theta
 __global__ __launch_bounds__(TILE) void k_restrict_first(LevelArgs F, LevelArgs C, int R,
                                                          const SysState *__restrict__ sys,
-                                                         const float *__restrict__ rf, float *__restrict__ bc,
-                                                         float *__restrict__ xc, float *__restrict__ rc,
-                                                         float *__restrict__ dc) {
+                                                         const float *__restrict__ rf, float *__restrict__ xc,
+                                                         float *__restrict__ rc, float *__restrict__ dc) {
     for (int t = blockIdx.x; t < C.ntiles; t += gridDim.x) {
         const int v = C.tvol[t];
         const VolDesc cd = C.vols[v], fd = F.vols[v];
@@ -603,7 +611,6 @@ __global__ __launch_bounds__(TILE) void k_restrict_first(LevelArgs F, LevelArgs 
             apply_minv(C.minv, C.N, I, s, zz);
             const int64_t base = (int64_t)r * 3 * C.N;
             for (int e = 0; e < 3; e++) {
-                bc[base + e * C.N + I] = s[e];
                 rc[base + e * C.N + I] = s[e];
                 if (xc) xc[base + e * C.N + I] = 0.f;
                 dc[base + e * C.N + I] = c2 * zz[e];
@@ -764,7 +771,7 @@ int env_int(const char *name, int def);
 
 template <bool NEWD, bool X0>
 void launch_step_t(hipStream_t st, const LevelArgs &L, bool elem, int R, const SysState *sys, int k, float *x,
-                   float *r, const float *d, float *dout) {
+                   float *r, const float *d, float *dout, const float *rin) {
     int g = grid_for(L.ntiles);
     // All right-hand sides per thread only where the level alone fills the
     // chip: with fewer tiles the R grid rows are the parallelism (64^3:
@@ -773,24 +780,27 @@ void launch_step_t(hipStream_t st, const LevelArgs &L, bool elem, int R, const S
     static const int uyz = env_int("PSX_MG_UYZ", 3);
     if (elem)
         hipLaunchKernelGGL((k_cheb_step<true, NEWD, 1, X0>), dim3(g, R), dim3(TILE), 0, st, L, R, sys, k, x, r, d,
-                           dout);
+                           dout, rin);
     else if (rall_tiles > 0 && L.ntiles >= rall_tiles)
-        hipLaunchKernelGGL((k_cheb_step_rall<NEWD, X0>), dim3(g), dim3(TILE), 0, st, L, R, sys, k, x, r, d, dout);
+        hipLaunchKernelGGL((k_cheb_step_rall<NEWD, X0>), dim3(g), dim3(TILE), 0, st, L, R, sys, k, x, r, d, dout,
+                           rin);
     else if (uyz >= 3)
         hipLaunchKernelGGL((k_cheb_step<false, NEWD, 3, X0>), dim3(g, R), dim3(TILE), 0, st, L, R, sys, k, x, r, d,
-                           dout);
+                           dout, rin);
     else
         hipLaunchKernelGGL((k_cheb_step<false, NEWD, 1, X0>), dim3(g, R), dim3(TILE), 0, st, L, R, sys, k, x, r, d,
-                           dout);
+                           dout, rin);
 }
 
 // x0: the level's x is zero (the first step of a smoothing pass from x = 0),
-// so the step stores x = d without reading x
+// so the step stores x = d without reading x; rin (first step only, else
+// null): the pass's right-hand side, read in place of r, which the step
+// then writes
 template <bool NEWD>
 void launch_step(hipStream_t st, const LevelArgs &L, bool elem, int R, const SysState *sys, int k, float *x, float *r,
-                 const float *d, float *dout, bool x0) {
-    if (x0) launch_step_t<NEWD, true>(st, L, elem, R, sys, k, x, r, d, dout);
-    else launch_step_t<NEWD, false>(st, L, elem, R, sys, k, x, r, d, dout);
+                 const float *d, float *dout, bool x0, const float *rin = nullptr) {
+    if (x0) launch_step_t<NEWD, true>(st, L, elem, R, sys, k, x, r, d, dout, rin);
+    else launch_step_t<NEWD, false>(st, L, elem, R, sys, k, x, r, d, dout, nullptr);
 }
 
 // V-cycle from level l, whose b, r, dA hold a started smoothing pass
@@ -809,9 +819,11 @@ float *vcycle(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, int l
     LevelArgs L = args_of(lv, B, coarsest ? p.coarse_ratio : p.ratio);
     float *dA = lv.dA, *dB = lv.dB;
     bool x0 = true;
+    // level 0's pass starts from r = b without a copy: its first step reads b
+    const float *rin = l == 0 ? lv.b : nullptr;
     if (coarsest) {
         for (int k = 0; k + 1 < p.coarse_deg; k++) {
-            launch_step<true>(st, L, elem, mg.R, sys, k, lv.x, lv.r, dA, dB, x0);
+            launch_step<true>(st, L, elem, mg.R, sys, k, lv.x, lv.r, dA, dB, x0, rin);
             x0 = false;
             std::swap(dA, dB);
         }
@@ -820,16 +832,16 @@ float *vcycle(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, int l
     const int nu = l == 0 ? p.nu0 : p.nu;
     // pre-smoothing: nu updates, the last one with the residual update
     for (int k = 0; k + 1 < nu; k++) {
-        launch_step<true>(st, L, elem, mg.R, sys, k, lv.x, lv.r, dA, dB, x0);
+        launch_step<true>(st, L, elem, mg.R, sys, k, lv.x, lv.r, dA, dB, x0, rin);
         x0 = false;
         std::swap(dA, dB);
     }
-    launch_step<false>(st, L, elem, mg.R, sys, 0, lv.x, lv.r, dA, nullptr, x0);
+    launch_step<false>(st, L, elem, mg.R, sys, 0, lv.x, lv.r, dA, nullptr, x0, x0 ? rin : nullptr);
     // coarse correction
     MGLevel &cv = mg.lv[l + 1];
     const bool c_coarsest = l + 2 == (int)mg.lv.size();
     LevelArgs C = args_of(cv, B, c_coarsest ? p.coarse_ratio : p.ratio);
-    hipLaunchKernelGGL(k_restrict_first, dim3(grid_for(C.ntiles), mg.R), dim3(TILE), 0, st, L, C, mg.R, sys, lv.r, cv.b,
+    hipLaunchKernelGGL(k_restrict_first, dim3(grid_for(C.ntiles), mg.R), dim3(TILE), 0, st, L, C, mg.R, sys, lv.r,
                        x_untouched(mg, l + 1) ? cv.x : nullptr, cv.r, cv.dA);
     float *pend = vcycle(st, B, mg, sys, l + 1);
     hipLaunchKernelGGL(k_prolong, dim3(grid_for(L.ntiles), mg.R), dim3(TILE), 0, st, L, C, mg.R, sys, cv.x,
@@ -852,7 +864,7 @@ float *vcycle(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, int l
 
 int alloc_level_vectors(MGLevel &l, int R, bool bx) {
     size_t n = (size_t)R * 3 * l.N;
-    if (bx) { RC(dalloc(&l.b, n)); RC(dalloc(&l.x, n)); l.own_bx = true; }
+    if (bx) { RC(dalloc(&l.x, n)); l.own_bx = true; }  // a coarse level keeps no b (restriction writes r)
     RC(dalloc(&l.r, n)); RC(dalloc(&l.dA, n)); RC(dalloc(&l.dB, n));
     return PSX_OK;
 }
