@@ -448,7 +448,10 @@ __global__ __launch_bounds__(TILE) void k_cheb_step(LevelArgs L, int R, const Sy
 // thread reads its node's 27 stencil blocks (243 floats) once and applies
 // them to every active right-hand side.  One grid row per right-hand side
 // would stream the stencil R times per step; at 128^3 the first coarse
-// level's stencil alone is 255 MB.
+// level's stencil alone is 255 MB.  Reading only the 14 forward blocks and
+// taking each backward block as the transpose of the neighbour's forward
+// block (the Galerkin operator is symmetric) was measured 4-8 % slower: the
+// neighbour's blocks miss the cache across tile edges.
 template <bool NEWD, bool X0 = false>
 __global__ __launch_bounds__(TILE) void k_cheb_step_rall(LevelArgs L, int R, const SysState *__restrict__ sys,
                                                          int k, float *__restrict__ xv, float *__restrict__ rv,
@@ -776,8 +779,10 @@ void launch_step_t(hipStream_t st, const LevelArgs &L, bool elem, int R, const S
     // All right-hand sides per thread only where the level alone fills the
     // chip: with fewer tiles the R grid rows are the parallelism (64^3:
     // 2x slower with it on every level, 128^3: 15% faster on the 64^3 level).
-    static const int rall_tiles = env_int("PSX_MG_RALL_TILES", 1024);
-    static const int uyz = env_int("PSX_MG_UYZ", 3);
+    // read at every launch (cheap next to one; graphs are captured once), so
+    // a process can compare the variants
+    const int rall_tiles = env_int("PSX_MG_RALL_TILES", 1024);
+    const int uyz = env_int("PSX_MG_UYZ", 3);
     if (elem)
         hipLaunchKernelGGL((k_cheb_step<true, NEWD, 1, X0>), dim3(g, R), dim3(TILE), 0, st, L, R, sys, k, x, r, d,
                            dout, rin);
