@@ -60,7 +60,7 @@ __global__ __launch_bounds__(TILE) void k_apply(const VolDesc *__restrict__ vols
                                                 const SysState *__restrict__ sys, const Jump *__restrict__ jp,
                                                 double *__restrict__ partial) {
     __shared__ double sh[TILE / 64];
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int t = lsm_tile0(); t < ntiles; t += gridDim.x) {
         const int v = tvol[t];
         const VolDesc vd = vols[v];
         int x, yy, z;
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(TILE) void k_apply_cverlet(const VolDesc *__restric
                                                         const double *__restrict__ u, double *__restrict__ V,
                                                         double *__restrict__ A, const double *__restrict__ AF,
                                                         int64_t Ntot, double rsi, double damp, double dt) {
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int t = lsm_tile0(); t < ntiles; t += gridDim.x) {
         const VolDesc vd = vols[tvol[t]];
         int x, yy, z;
         bool ok;
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(TILE) void k_diag_inv(const VolDesc *__restrict__ v
                                                    const int32_t *__restrict__ tstart, int ntiles,
                                                    const double *__restrict__ ka, const double *__restrict__ kb,
                                                    T *__restrict__ minv, int64_t Ntot) {
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int t = lsm_tile0(); t < ntiles; t += gridDim.x) {
         const VolDesc vd = vols[tvol[t]];
         int x, y, z;
         bool ok;
@@ -242,7 +242,7 @@ __global__ __launch_bounds__(TILE) void k_cg_update(const VolDesc *__restrict__ 
                                                     const T *__restrict__ Qv, const T *__restrict__ minv, int64_t Ntot,
                                                     double *__restrict__ prz, double *__restrict__ prr, int init) {
     __shared__ double sh[TILE / 64];
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int t = lsm_tile0(); t < ntiles; t += gridDim.x) {
         const int v = tvol[t];
         const VolDesc vd = vols[v];
         int x, y, z;
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(TILE) void k_cg_pupdate(const VolDesc *__restrict__
                                                      const int32_t *__restrict__ tstart, int ntiles, int R,
                                                      const SysState *__restrict__ sys, T *__restrict__ P,
                                                      const T *__restrict__ Z, int64_t Ntot, int init) {
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int t = lsm_tile0(); t < ntiles; t += gridDim.x) {
         const int v = tvol[t];
         const VolDesc vd = vols[v];
         int x, y, z;
@@ -326,7 +326,7 @@ __global__ __launch_bounds__(TILE) void k_refine_residual(
     const double *__restrict__ KX, double *__restrict__ R64, int64_t Ntot, double *__restrict__ prr,
     double *__restrict__ pmean) {
     __shared__ double sh[TILE / 64];
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int t = lsm_tile0(); t < ntiles; t += gridDim.x) {
         const VolDesc vd = vols[tvol[t]];
         int x, y, z;
         bool ok;
@@ -375,7 +375,7 @@ __global__ __launch_bounds__(TILE) void k_scale_sys(const VolDesc *__restrict__ 
                                                     const SysState *__restrict__ outer, const double *__restrict__ ka,
                                                     const double *__restrict__ kb, float *__restrict__ R32,
                                                     float *__restrict__ X32, const double *__restrict__ R64, int64_t Ntot) {
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int t = lsm_tile0(); t < ntiles; t += gridDim.x) {
         const int v = tvol[t];
         const VolDesc vd = vols[v];
         int x, y, z;
@@ -404,7 +404,7 @@ __global__ __launch_bounds__(TILE) void k_unscale_add(const VolDesc *__restrict_
                                                       const int32_t *__restrict__ tstart, int ntiles, int R,
                                                       const SysState *__restrict__ outer, double *__restrict__ X64,
                                                       const float *__restrict__ X32, int64_t Ntot) {
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int t = lsm_tile0(); t < ntiles; t += gridDim.x) {
         const int v = tvol[t];
         const VolDesc vd = vols[v];
         int x, y, z;
@@ -497,7 +497,7 @@ __global__ __launch_bounds__(TILE) void k_stress(const VolDesc *__restrict__ vol
                                                  const double *__restrict__ u, int64_t Ntot, const Jump *__restrict__ jp,
                                                  double *__restrict__ partial) {
     __shared__ double sh[TILE / 64];
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int t = lsm_tile0(); t < ntiles; t += gridDim.x) {
         const VolDesc vd = vols[tvol[t]];
         int x, y, z;
         bool ok;
@@ -594,7 +594,7 @@ __global__ __launch_bounds__(TILE) void k_adj_rhs(const VolDesc *__restrict__ vo
                                                   const double *__restrict__ W /* [6 loads][9 comps] */,
                                                   double *__restrict__ H, int64_t Ntot, double *__restrict__ mag) {
     __shared__ double sh[TILE / 64];
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int t = lsm_tile0(); t < ntiles; t += gridDim.x) {
         const VolDesc vd = vols[tvol[t]];
         int x, y, z;
         bool ok;
@@ -639,7 +639,7 @@ __global__ __launch_bounds__(TILE) void k_adj_voxel(const VolDesc *__restrict__ 
                                                     const double *__restrict__ U, const double *__restrict__ LAM,
                                                     const double *__restrict__ W, int64_t Ntot, const Jump *__restrict__ jp,
                                                     double *__restrict__ tA, double *__restrict__ tB) {
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int t = lsm_tile0(); t < ntiles; t += gridDim.x) {
         const VolDesc vd = vols[tvol[t]];
         int x, y, z;
         bool ok;

@@ -110,6 +110,17 @@ inline int64_t tiles_for(const VolDesc &d, int shape) {
     return nbx * nby * nbz;
 }
 
+// First tile of a workgroup's grid-stride loop over tiles.  The hardware
+// deals workgroups to the 8 XCDs round robin; remapped, the workgroups of
+// one XCD take a contiguous range of tiles, so the 27-point halo of a tile
+// mostly lies in tiles whose data the same L2 holds.
+#ifndef PSX_LSM_XCD
+#define PSX_LSM_XCD 1
+#endif
+__device__ __forceinline__ int lsm_tile0() {
+    return PSX_LSM_XCD ? (int)psx::xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+}
+
 __device__ __forceinline__ void tile_node(const VolDesc &vd, int start, int &x, int &y, int &z, bool &ok) {
     if (vd.pad_ & 1) {
         const int s = vd.pad_;

@@ -83,7 +83,7 @@ __global__ __launch_bounds__(256) void k_assemble_fine(const VolDesc *__restrict
                                                        const int32_t *__restrict__ tstart, int ntiles,
                                                        const double *__restrict__ ka, const double *__restrict__ kb,
                                                        double *__restrict__ A0, int64_t N) {
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int t = lsm_tile0(); t < ntiles; t += gridDim.x) {
         const VolDesc vd = vols[tvol[t]];
         int x, y, z;
         bool ok;
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(256) void k_galerkin(const VolDesc *__restrict__ fv
                                                   const int32_t *__restrict__ ctstart, int cntiles,
                                                   const double *__restrict__ Af, int64_t Nf, double *__restrict__ Ac,
                                                   int64_t Nc) {
-    for (int t = blockIdx.x; t < cntiles; t += gridDim.x) {
+    for (int t = lsm_tile0(); t < cntiles; t += gridDim.x) {
         const int v = ctvol[t];
         const VolDesc fd = fvols[v], cd = cvols[v];
         int X, Y, Z;
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(256) void k_level_prep(const VolDesc *__restrict__ 
                                                     const double *__restrict__ minv_in, float *__restrict__ A32,
                                                     float *__restrict__ minv32, double *__restrict__ pmax) {
     __shared__ double sh[4];
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int t = lsm_tile0(); t < ntiles; t += gridDim.x) {
         const VolDesc vd = vols[tvol[t]];
         int x, y, z;
         bool ok;
@@ -320,7 +320,7 @@ __device__ inline bool sys_active(const SysState *sys, int v, int R, int r) {
 __global__ __launch_bounds__(TILE) void k_cheb_first(LevelArgs L, int R, const SysState *__restrict__ sys,
                                                      const float *__restrict__ bv, float *__restrict__ xv,
                                                      float *__restrict__ rv, float *__restrict__ dv) {
-    for (int t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
+    for (int t = lsm_tile0(); t < L.ntiles; t += gridDim.x) {
         const int v = L.tvol[t];
         const VolDesc vd = L.vols[v];
         int x, y, z;
@@ -356,7 +356,7 @@ __global__ __launch_bounds__(TILE) void k_cg_update_first(LevelArgs L, int R, co
                                                           double *__restrict__ prr, float *__restrict__ xv,
                                                           float *__restrict__ rv, float *__restrict__ dv) {
     __shared__ double sh[TILE / 64];
-    for (int t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
+    for (int t = lsm_tile0(); t < L.ntiles; t += gridDim.x) {
         const int v = L.tvol[t];
         const VolDesc vd = L.vols[v];
         int x, y, z;
@@ -407,7 +407,7 @@ __global__ __launch_bounds__(TILE) void k_cheb_step(LevelArgs L, int R, const Sy
                                                     const float *__restrict__ dv, float *__restrict__ dout,
                                                     const float *rin) {
     const float *rs = X0 && rin ? rin : rv;
-    for (int t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
+    for (int t = lsm_tile0(); t < L.ntiles; t += gridDim.x) {
         const int v = L.tvol[t];
         const VolDesc vd = L.vols[v];
         int x, y, z;
@@ -458,7 +458,7 @@ __global__ __launch_bounds__(TILE) void k_cheb_step_rall(LevelArgs L, int R, con
                                                          const float *__restrict__ dv, float *__restrict__ dout,
                                                          const float *rin) {
     const float *rs = X0 && rin ? rin : rv;
-    for (int t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
+    for (int t = lsm_tile0(); t < L.ntiles; t += gridDim.x) {
         const int v = L.tvol[t];
         const VolDesc vd = L.vols[v];
         int x, y, z;
@@ -534,7 +534,7 @@ __global__ __launch_bounds__(TILE) void k_restart_final0(LevelArgs L, int R, con
                                                          const float *__restrict__ ev, const float *__restrict__ rcg,
                                                          double *__restrict__ prz) {
     __shared__ double sh[TILE / 64];
-    for (int t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
+    for (int t = lsm_tile0(); t < L.ntiles; t += gridDim.x) {
         const int v = L.tvol[t];
         const VolDesc vd = L.vols[v];
         int x, y, z;
@@ -582,7 +582,7 @@ __global__ __launch_bounds__(TILE) void k_restrict_first(LevelArgs F, LevelArgs 
                                                          const SysState *__restrict__ sys,
                                                          const float *__restrict__ rf, float *__restrict__ xc,
                                                          float *__restrict__ rc, float *__restrict__ dc) {
-    for (int t = blockIdx.x; t < C.ntiles; t += gridDim.x) {
+    for (int t = lsm_tile0(); t < C.ntiles; t += gridDim.x) {
         const int v = C.tvol[t];
         const VolDesc cd = C.vols[v], fd = F.vols[v];
         int X, Y, Z;
@@ -626,7 +626,7 @@ __global__ __launch_bounds__(TILE) void k_restrict_first(LevelArgs F, LevelArgs 
 __global__ __launch_bounds__(TILE) void k_prolong(LevelArgs F, LevelArgs C, int R, const SysState *__restrict__ sys,
                                                   const float *__restrict__ xc, const float *__restrict__ dpend,
                                                   float *__restrict__ ef) {
-    for (int t = blockIdx.x; t < F.ntiles; t += gridDim.x) {
+    for (int t = lsm_tile0(); t < F.ntiles; t += gridDim.x) {
         const int v = F.tvol[t];
         const VolDesc fd = F.vols[v], cd = C.vols[v];
         int x, y, z;
@@ -665,7 +665,7 @@ __global__ __launch_bounds__(TILE) void k_final0(LevelArgs L, int R, const SysSt
                                                  const float *__restrict__ rcg, float *__restrict__ z,
                                                  const float *__restrict__ d, double *__restrict__ prz) {
     __shared__ double sh[TILE / 64];
-    for (int t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
+    for (int t = lsm_tile0(); t < L.ntiles; t += gridDim.x) {
         const int v = L.tvol[t];
         const VolDesc vd = L.vols[v];
         int x, y, zc;
@@ -700,7 +700,7 @@ __global__ __launch_bounds__(TILE) void k_final0(LevelArgs L, int R, const SysSt
 // Power iteration on D^-1 A (right-hand side slot 0 of the level's work
 // vectors): vout = D^-1 A vin / |vin|, per-tile partials of |vout|^2.
 __global__ __launch_bounds__(TILE) void k_pow_init(LevelArgs L, float *__restrict__ v) {
-    for (int t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
+    for (int t = lsm_tile0(); t < L.ntiles; t += gridDim.x) {
         const VolDesc vd = L.vols[L.tvol[t]];
         int x, y, z;
         bool ok;
@@ -719,7 +719,7 @@ template <bool ELEM>
 __global__ __launch_bounds__(TILE) void k_pow_step(LevelArgs L, const float *__restrict__ vin, float *__restrict__ vout,
                                                    const double *__restrict__ nin, double *__restrict__ pnorm) {
     __shared__ double sh[TILE / 64];
-    for (int t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
+    for (int t = lsm_tile0(); t < L.ntiles; t += gridDim.x) {
         const int v = L.tvol[t];
         const VolDesc vd = L.vols[v];
         int x, y, z;
