@@ -14,17 +14,19 @@
 //                bounce-back link: the population returns to the same node in
 //                the opposite direction).
 //                FP32 mode (lbm_fast.hip): per (q,node) T = (fr+fb) - tr_q*rr0
-//                and R = fr/(fr+fb) as floats, post-collision, in AoSoA blocks
+//                and the minority colour's fraction of fr+fb, its sign naming
+//                the colour, as floats, post-collision, in AoSoA blocks
 //                of 64 nodes, streamed by a pull through src[18][nl] (source
 //                node of each incoming link, -1 for bounce-back), held on the
 //                device as int16 rank differences within x-planes plus
 //                per-plane bases (36 B per node instead of 72) when every
 //                difference fits, as the int32 table otherwise.  T keeps the
 //                small deviation of the total population (momentum is exact to
-//                FP32 ulps of the deviation) and R is exactly 1 or 0 for
-//                single-colour populations, so pure-phase nodes keep exactly
-//                zero minority density as the reference's branch test rb<=ee
-//                requires.
+//                FP32 ulps of the deviation); the colour fraction keeps the
+//                minority colour to FP32 relative precision and is exact (-0
+//                or +0) for single-colour populations, so pure-phase nodes
+//                keep exactly zero minority density as the reference's branch
+//                test rb<=ee requires.
 //   vox[nl]      voxel index of each node in the phi grid.
 //   phi grid     dense nx*ny*nz FP64, z fastest (index z + nz*(y + ny*x)):
 //                colour field cf at fluid voxels, `wet` at solid voxels, so
@@ -301,15 +303,11 @@ void build_bricks(psx_lbm *h, const int (&bd)[3], std::vector<int32_t> &vox,
     }
 }
 
-// host-side conversion of one population pair into FP32 storage: T = f - w0,
-// R = fr / f, R exactly 1 or 0 for single-colour pairs
-inline void f32_encode(double w0, double fr, double fb, float &T, float &R) {
-    double f = fr + fb;
-    if (fb == 0.0) R = 1.0f;
-    else if (fr == 0.0) R = 0.0f;
-    else if (f == 0.0) R = 0.5f;
-    else R = (float)(fr / f);
-    T = (float)(f - w0);
+// host-side conversion of one population pair into FP32 storage: T = f - w0
+// and the signed minority-colour fraction s (frac_encode)
+inline void f32_encode(double w0, double fr, double fb, float &T, float &s) {
+    s = frac_encode(fr, fb);
+    T = (float)((fr + fb) - w0);
 }
 
 }  // namespace
@@ -643,10 +641,8 @@ extern "C" int psx_lbm_get_state(psx_lbm *h, double *fr, double *fb) {
                     else k = opp(q);
                 }
                 double f = h->c.w0[q] + (double)st[aos(k, j)];
-                double r = (double)st[aos(Q + k, j)];
                 const int64_t o = h->ref_node(i);
-                fr[Q * o + q] = r * f;
-                fb[Q * o + q] = f - r * f;
+                frac_split(f, st[aos(Q + k, j)], fr[Q * o + q], fb[Q * o + q]);
             }
     } else {
         std::vector<double> st((size_t)2 * Q * nl);

@@ -1,6 +1,7 @@
 // FP32-storage variant of the two-phase LBM step.  Populations are stored
 // per (direction, node) as
-//   T = (fr + fb) - tr_q*rr0   and   R = fr / (fr + fb)
+//   T = (fr + fb) - tr_q*rr0   and   s = the minority colour's fraction of
+//                                        fr + fb, signed by colour (frac_encode)
 // in FP32 (arithmetic in FP64 registers), post-collision and not yet
 // streamed, in AoSoA blocks of 64 nodes (lbm_device.h: aos()).  Streaming is
 // a pull: each node gathers direction q from its source node (src table) or,
@@ -71,14 +72,8 @@ __device__ __forceinline__ void pull_from(const PullIdx &P, const int32_t (&pb)[
 
 __device__ __forceinline__ void put(const LbmConst &c, float *__restrict__ out, int q, int64_t j, double fr,
                                     double fb) {
-    double f = fr + fb;
-    float r;
-    if (fb == 0.0) r = 1.0f;
-    else if (fr == 0.0) r = 0.0f;
-    else if (f == 0.0) r = 0.5f;
-    else r = (float)(fr / f);  // correctly rounded: an approximate reciprocal biases R and drifts colour mass
-    out[aos(q, j)] = (float)(f - c.w0[q]);
-    out[aos(Q + q, j)] = r;
+    out[aos(q, j)] = (float)((fr + fb) - c.w0[q]);
+    out[aos(Q + q, j)] = frac_encode(fr, fb);
 }
 
 // The collision of L1:2407-2620 with the relaxation vector selected per lane
@@ -214,13 +209,14 @@ __device__ __forceinline__ void collide_pull(const LbmConst &c, const float *__r
         pull_from<M>(P, pb, nl, i, q, j, k);
         at[q] = (decltype(at[0] + 0))aos(k, j);
         double t = c.w0[q] + (double)in[at[q]];
-        float r = in[at[q] + Q * AOS];
-        double a = (double)r * t;
+        float s = in[at[q] + Q * AOS];
+        double a, b;
+        frac_split(t, s, a, b);
         f[q] = t;
         rr += a;
-        rb += t - a;
-        allred &= r == 1.0f;
-        allblue &= r == 0.0f;
+        rb += b;
+        allred &= frac_all_red(s);
+        allblue &= frac_all_blue(s);
     }
     const double dt2 = c.dt2;
     double bf[3];
@@ -233,7 +229,11 @@ __device__ __forceinline__ void collide_pull(const LbmConst &c, const float *__r
         t1 = c.tr[0]; t2 = c.tr[1]; t3 = c.tr[7]; r = rr;
         if (!allred) {  // the colour split of a node that is not single-coloured
 #pragma unroll
-            for (int q = 0; q < Q; q++) f[q] = (double)in[at[q] + Q * AOS] * f[q];
+            for (int q = 0; q < Q; q++) {
+                double a, b;
+                frac_split(f[q], in[at[q] + Q * AOS], a, b);
+                f[q] = a;
+            }
         }
         bf[0] = c.bfr[0]; bf[1] = c.bfr[1]; bf[2] = c.bfr[2];
     } else if (rr <= c.ee) {
@@ -241,7 +241,11 @@ __device__ __forceinline__ void collide_pull(const LbmConst &c, const float *__r
         t1 = c.tb[0]; t2 = c.tb[1]; t3 = c.tb[7]; r = rb;
         if (!allblue) {
 #pragma unroll
-            for (int q = 0; q < Q; q++) f[q] = f[q] - (double)in[at[q] + Q * AOS] * f[q];
+            for (int q = 0; q < Q; q++) {
+                double a, b;
+                frac_split(f[q], in[at[q] + Q * AOS], a, b);
+                f[q] = b;
+            }
         }
         bf[0] = c.bfb[0]; bf[1] = c.bfb[1]; bf[2] = c.bfb[2];
     } else {
@@ -308,9 +312,10 @@ __device__ __forceinline__ double colour_pull(const LbmConst &c, const float *__
         pull_from<M>(P, pb, c.nl, i, q, j, k);
         int64_t a = aos(k, j);
         double t = c.w0[q] + (double)in[a];
-        double rt = (double)in[a + Q * AOS] * t;
-        rr += rt;
-        rb += t - rt;
+        double pr, pb;
+        frac_split(t, in[a + Q * AOS], pr, pb);
+        rr += pr;
+        rb += pb;
     }
     return (rr - rb) / (rr + rb);
 }

@@ -92,13 +92,6 @@ def test_fp64_collision_variants_bitexact(variant):
     np.testing.assert_allclose(d.Qr[:], od.Qr[:], rtol=1e-10, atol=1e-18)
 
 
-def test_trt_quarter_at_tau_one_is_bgk():
-    """With tau = 1 (nu = 1/6) and Lambda = 1/4 every rate is 1: the collision
-    returns f_eq + Fp exactly (closed form of the BGK limit)."""
-    Sr = trt_rates(1.0 / 6.0, 0.25)
-    assert np.allclose(Sr[[1, 2, 4, 6, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18]], 1.0)
-
-
 def test_single_phase_mass_conservation_and_rest_state():
     ind = geo.sphere_pack((16, 16, 16), 0.5, 3.0, seed=9)
     inp = pio.LBMInput(ncx=16, ncy=16, ncz=16, bfr=[0, 0, 0], bfb=[0, 0, 0], sat=0.0, sig=0.0)
