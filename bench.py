@@ -11,20 +11,33 @@ the collide kernel, timed with HIP events on the library's own stream.
 Also reported (objects in the same line):
   lbm_f64      the same LBM workload with FP64 storage (states bit-identical
                to the reference): ms per step and MLUPS.
+  trt_pack_64  the reference method as configured: 64^3 radius-6 pack, FP64,
+               TRT tau = 1 with magic parameter 1/4, 2 + 2000 steps: MLUPS,
+               k_r, k_b and the blue saturation.
+  precision    FP32 storage against FP64 on the headline workload over 20000
+               steps, with the FP64 control started from a 1e-7 perturbation:
+               permeabilities, saturation, colour masses at the end and
+               averaged over the second half.
+  coupled      L1's coupled model on the same 256^3 workload: ms per main-loop
+               step (LBM + momentum exchange + lsm_step) and calc_ab.
   lsm          64^3 single-solid sphere pack (porosity 0.35), all six load
                cases at rtol 1e-10 and 1e-5: plain FP64 PCG one case at a time
                versus the batched FP32 multigrid-PCG / FP64-refinement solver,
-               with the moduli error; 64 uneven two-component sub-volumes in
-               one batch against plain FP64 on every volume; adjoint gradients of (K, G) at 128^3 with a
-               lognormal stiffness field, and Gauss-Newton calibration steps.
-  coupled      L1's coupled model on the same 256^3 workload: ms per main-loop
-               step (LBM + momentum exchange + lsm_step) and calc_ab.
+               with the moduli error; the operator's roofline at 64^3 and
+               128^3; 64 uneven two-component sub-volumes in one batch against
+               plain FP64 on every volume; adjoint gradients of (K, G) at
+               128^3 with a lognormal stiffness field, and a 50-step
+               Gauss-Newton calibration on one handle.
   cpu_baseline the oracle (C restatement of the reference, OpenMP over up to
                16 host threads) timed on this host over a bounded sample of
-               the same LBM workload.
+               the headline workload; beside it the reference programs
+               themselves (flang builds of L1 and XX_P.f, one core) on the
+               64^3 cases, and the LSM oracle.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-lsm] [--no-cpu] [--no-coupled] [--full]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-lsm] [--no-cpu] [--no-coupled]
+                    [--no-precision] [--calib-steps N]
     python bench.py --lsm-only         # the LSM rows alone (profiling)
+    python bench.py --lsm-matvec       # the LSM operator alone at 64^3 and 128^3 (profiling)
     python bench.py --precision-steps N  # FP32 storage against FP64 over N steps
     python bench.py --calib-copy MIB   # plain float32 elementwise kernel (torch.neg)
                                        # over MIB MiB in and out, ten launches:
@@ -49,31 +62,22 @@ def log(msg):
 
 
 def lbm_workload(n=256):
-    from porescale_amd import geometry as geo
-    from porescale_amd import io as pio
-    ind = geo.sphere_pack((n, n, n), 0.30, (4, 10), seed=0)
-    inp = pio.LBMInput(B=0.7, sig=0.01, vir=1.0 / 6.0, vib=1.0 / 60.0, wet=0.70710678118654752, idir=3,
-                       ncx=n, ncy=n, ncz=n, dt=1.0, bfr=[0.0, 0.0, 1e-5], bfb=[0.0, 0.0, 1e-5])
-    blue = geo.phase_blobs(ind, 0.5, seed=0, radius=4.0)
+    from porescale_amd import cases
+    ind, inp, coll, S, blue = cases.twophase_mrt_pack(n)
     return ind, inp, blue
 
 
 def initial_state(inp, ind, blue):
-    import numpy as np
-    # rest equilibrium, all-red or all-blue per node (initdistr's state,
-    # lsm_lbm_static_alpha_beta_multipore.f:2834-2840) in the reference node
-    # order (x outer, z fastest)
-    dt = inp.dt
-    cb2 = inp.rr0 * inp.cr2 / inp.rb0
-    tr = np.array([1 - 2 * inp.cr2 * dt * dt] + [inp.cr2 * dt * dt / 6] * 6 + [inp.cr2 * dt * dt / 12] * 12)
-    tb = np.array([1 - 2 * cb2 * dt * dt] + [cb2 * dt * dt / 6] * 6 + [cb2 * dt * dt / 12] * 12)
-    t_ind = np.transpose(ind, (2, 1, 0))
-    bl = np.transpose(blue, (2, 1, 0))[t_ind == inp.iipore]
-    c = np.where(bl, 0.0, 1.0)
-    return tr[None, :] * c[:, None] * inp.rr0, tb[None, :] * (1 - c[:, None]) * inp.rb0
+    from porescale_amd import cases
+    return cases.initial_state(inp, ind, blue)
 
 
-def run_lbm(steps, warmup):
+def run_lbm(steps, warmup, before=None):
+    """The headline: ``warmup`` untimed steps, then ``steps`` steps with HIP
+    events around every kernel.  ``before(ind, inp, blue)`` runs once the
+    handle holds its start state, just ahead of the warm-up (the FP64
+    storage leg: its GPU work keeps the clocks up while the headline's host
+    set-up would otherwise leave the device idle); its result is returned."""
     from porescale_amd.lbm import LBM, mrt_rates
     ind, inp, blue = lbm_workload()
     fr0, fb0 = initial_state(inp, ind, blue)
@@ -81,6 +85,8 @@ def run_lbm(steps, warmup):
     out = {}
     with LBM(inp, ind, collision=coll, precision="f32") as lb:
         lb.set_state(fr0, fb0)
+        if before is not None:
+            out["before"] = before(ind, inp, blue)
         lb.step(warmup)
         lb.sync()
         t0 = time.perf_counter()
@@ -144,13 +150,40 @@ def run_coupled(ind, inp, blue, steps=20):
             "betai": ab["betai"].tolist()}
 
 
-def run_precision(steps, chunk=2000):
-    """FP32 storage against FP64 on the headline workload over ``steps`` steps:
-    permeabilities, blue saturation and colour masses of three runs: FP64 and
-    FP32 from the same start, and, as the control, FP64 from that start with
-    every population scaled by 1 + 1e-7 u (u uniform in [-1, 1], seed 0): how
-    far a perturbation of the size of FP32 storage rounding alone moves the
-    observables of this transient."""
+def run_trt_pack(n=64, steps=2000):
+    """The reference method as configured: FP64 storage (bit-identical to
+    the reference), TRT with tau = 1 and magic parameter 1/4 on the 64^3
+    radius-6 pack (porosity 0.35), A = 0.01, recolouring 0.7, body force
+    1e-5 along z, blob start S_b = 0.5; 2 pre-steps then ``steps`` steps.
+    Outputs of check_state (L1:3181-3182, 3280) after the last step."""
+    from porescale_amd import cases
+    from porescale_amd.lbm import LBM
+    ind, inp, coll, S, blue = cases.twophase_trt_pack(n)
+    out = {"workload": f"lbm-d3q19-twophase-trt-{n}^3 (tau 1, magic 1/4, radius-6 pack, porosity 0.35)",
+           "steps": 2 + steps}
+    for prec in ("f64", "f32"):
+        with LBM(inp, ind, collision=coll, precision=prec) as lb:
+            lb.set_state(*cases.initial_state(inp, ind, blue))
+            lb.step(2)
+            lb.sync()
+            tot, tc, td = lb.profile(steps)
+            d = lb.check_state(1, steps)
+            out[prec] = {"ms_per_step": tot / steps, "mlups": lb.nl * steps / (tot * 1e-3) / 1e6,
+                         "permr": d.permr, "permb": d.permb, "sat_b": d.outm[11]}
+            out["fluid_nodes"] = lb.nl
+    out["storage"] = {"f64": "FP64 (bit-identical to the reference)", "f32": "FP32 deviation + colour fraction"}
+    return out
+
+
+def run_precision(steps=20000, every=500, window_from=10000):
+    """FP32 storage against FP64 on the headline workload over ``steps``
+    steps: permeabilities, blue saturation and colour masses of three runs:
+    FP64 and FP32 from the same start, and, as the control, FP64 from that
+    start with every population scaled by 1 + 1e-7 u (u uniform in [-1, 1],
+    seed 0): how far a perturbation of the size of FP32 storage rounding
+    alone moves the observables of this transient.  Sampled every ``every``
+    steps; reported at the last step and as the mean over the samples from
+    ``window_from`` on (the transient's permeabilities fluctuate)."""
     import numpy as np
     from porescale_amd.lbm import LBM, mrt_rates
     ind, inp, blue = lbm_workload()
@@ -159,25 +192,37 @@ def run_precision(steps, chunk=2000):
     rng = np.random.default_rng(0)
     fr1 = fr0 * (1.0 + 1e-7 * rng.uniform(-1.0, 1.0, fr0.shape))
     fb1 = fb0 * (1.0 + 1e-7 * rng.uniform(-1.0, 1.0, fb0.shape))
-    out = {"workload": "lbm-d3q19-twophase-mrt-256^3", "steps": steps}
+    keys = ("permr", "permb", "sat_b", "mass_r", "mass_b")
+    out = {"workload": "lbm-d3q19-twophase-mrt-256^3", "steps": steps, "sample_every": every,
+           "window": [window_from, steps]}
+    series = {}
     for name, prec, st in (("f64", "f64", (fr0, fb0)), ("f64_perturbed_start", "f64", (fr1, fb1)),
                            ("f32", "f32", (fr0, fb0))):
+        rows = []
         with LBM(inp, ind, collision=coll, precision=prec) as lb:
             lb.set_state(*st)
             done = 0
             while done < steps:
-                n = min(chunk, steps - done)
-                lb.step(n)
-                lb.sync()
-                done += n
-                log(f"precision run {name}: {done}/{steps} steps")
-            d = lb.check_state(1, steps)
-            out[name] = {"permr": d.permr, "permb": d.permb, "sat_b": d.outm[11], "mass_r": d.tmr,
-                         "mass_b": d.tmb}
-    for k in ("permr", "permb", "sat_b", "mass_r", "mass_b"):
-        ref = out["f64"][k]
-        out[f"rel_{k}"] = {"f32": abs(out["f32"][k] - ref) / abs(ref),
-                           "f64_perturbed_start": abs(out["f64_perturbed_start"][k] - ref) / abs(ref)}
+                lb.step(every)
+                done += every
+                d = lb.check_state(1, every)
+                rows.append((done, d.permr, d.permb, d.outm[11], d.tmr, d.tmb))
+                if done % 5000 == 0:
+                    log(f"precision run {name}: {done}/{steps} steps")
+        series[name] = np.array(rows)
+        last = series[name][-1]
+        out[name] = dict(zip(keys, (float(x) for x in last[1:])))
+    w = series["f64"][:, 0] >= window_from
+    for name in ("f64", "f64_perturbed_start", "f32"):
+        out[name]["window_mean"] = dict(zip(keys, (float(x) for x in series[name][w, 1:].mean(0))))
+    for i, k in enumerate(keys):
+        ref, ref_w = series["f64"][-1, i + 1], series["f64"][w, i + 1].mean()
+        out[f"rel_{k}"] = {n: abs(series[n][-1, i + 1] - ref) / abs(ref) for n in ("f32", "f64_perturbed_start")}
+        out[f"rel_{k}_window_mean"] = {n: abs(series[n][w, i + 1].mean() - ref_w) / abs(ref_w)
+                                       for n in ("f32", "f64_perturbed_start")}
+        out[f"max_rel_{k}_over_samples"] = {
+            n: float(np.max(np.abs(series[n][:, i + 1] - series["f64"][:, i + 1]) / np.abs(series["f64"][:, i + 1])))
+            for n in ("f32", "f64_perturbed_start")}
     return out
 
 
@@ -204,6 +249,86 @@ def cpu_baseline(ind, inp, blue, budget_s=12.0):
             "sample": f"{n} steps of the same 256^3 two-phase MRT workload, oracle in C, {threads} OpenMP threads"}
 
 
+def _time_ref(exe, d, timeout=600.0):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+    t0 = time.perf_counter()
+    ob._run(exe, d, timeout)
+    return time.perf_counter() - t0
+
+
+def cpu_reference(lsm_steps_to_converge=None):
+    """The reference programs themselves (oracle/_ref: L1 and XX_P.f
+    compiled from the reference's sources with flang, serial) and the LSM
+    oracle, each timed on this host over a bounded window.  Per-step times
+    come from two runs that differ only in their step count, so set-up
+    (tables, file I/O) cancels.
+      L1: the 64^3 TRT pack (tau 1), icon 0 start at S_b = 0.5; one L1
+          main-loop step is the LBM step plus lsm_step of the solid lattice
+          (L1 always runs both).
+      XX_P.f: the 64^3 solid pack, alpha 1, beta 0.1 (E, nu from the inverse
+          of XX:411-414): damped-Verlet node-steps per second."""
+    import dataclasses
+    import shutil
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+    from porescale_amd import cases
+    from porescale_amd import geometry as geo
+    from porescale_amd import io as pio
+    out = {}
+    if ob.ref_available("l1_ref"):
+        ind, inp, coll, S, blue = cases.twophase_trt_pack(64)
+        times = {}
+        for itw in (5, 25):
+            d = ob._prep_dir(ind)
+            try:
+                pio.write_lbm_input(os.path.join(d, "INPUT_PARAMETERS"),
+                                    dataclasses.replace(inp, itm=1, itw=itw, its=1000, sat=0.5, icon=0))
+                times[itw] = _time_ref("l1_ref", d)
+            finally:
+                shutil.rmtree(d, ignore_errors=True)
+        per = (times[25] - times[5]) / 20
+        nl = int((ind == 1).sum())
+        out["l1_ref"] = {"value": nl / per / 1e6, "unit": "MLUPS (fluid nodes, LBM + lsm_step per step)",
+                         "s_per_step": per, "cores": 1, "kind": "reference",
+                         "sample": "L1 compiled from the reference's sources (flang -O2, serial) on the 64^3 TRT "
+                                   "pack: 25 minus 5 main-loop steps, each the LBM step and lsm_step"}
+    ind = geo.sphere_pack((64, 64, 64), 0.35, 6.0, seed=0)
+    nz, ny, nx = ind.shape
+    r = 0.1  # beta / alpha; alpha = 1 (inverse of XX:411-414)
+    nu = (1 - 4 * r) / (4 * (1 + r))
+    E = 1.0 * (5 - 10 * nu)
+    if ob.ref_available("xx_ref"):
+        times = {}
+        for its in (10, 60):
+            d = ob._prep_dir(ind)
+            try:
+                pio.write_lsm_input(os.path.join(d, "INPUT_PARAMETERS"),
+                                    pio.LSMInput(ncx=nx, ncy=ny, ncz=nz, Em1=E, nu1=nu, Em2=E, nu2=nu, rs=1.0,
+                                                 dt=0.01, itm=1, its=its, gstrain=0.01, nthread=1))
+                times[its] = _time_ref("xx_ref", d)
+            finally:
+                shutil.rmtree(d, ignore_errors=True)
+        per = (times[60] - times[10]) / 50
+        o = ob.OracleLSM(ind, "xx", alf=(1.0, 0.1))
+        ns = o.ns
+        ent = {"value": ns / per, "unit": "node-steps/s", "s_per_step": per, "nodes": ns, "cores": 1,
+               "kind": "reference",
+               "sample": "XX_P.f compiled from the reference's sources (flang -O2, serial) on the 64^3 solid "
+                         "pack, alpha 1, beta 0.1: 60 minus 10 damped-Verlet steps"}
+        if lsm_steps_to_converge:
+            ent["verlet_steps_to_moduli_1e-5"] = lsm_steps_to_converge
+            ent["solves_per_s_estimate"] = 1.0 / (lsm_steps_to_converge * per)
+        out["xx_ref"] = ent
+        t0 = time.perf_counter()
+        o.verlet(20)
+        per_o = (time.perf_counter() - t0) / 20
+        out["lsm_oracle"] = {"value": ns / per_o, "unit": "node-steps/s", "s_per_step": per_o, "cores": 1,
+                             "kind": "port", "sample": "the LSM oracle (C restatement of XX_P.f), same case, "
+                                                       "20 damped-Verlet steps"}
+    return out
+
+
 def _plain_rate(ind, cases, rtol, pars=(1.0, 0.1), maxit=200000, stiff=None, end_to_end=False):
     """Plain FP64 GPU path: block-Jacobi PCG, one load case per solve.
     Returns the seconds of the solves alone (after a warm-up solve that
@@ -225,7 +350,60 @@ def _plain_rate(ind, cases, rtol, pars=(1.0, 0.1), maxit=200000, stiff=None, end
     return (t, its, t_e2e) if end_to_end else (t, its)
 
 
-def run_lsm(full=False):
+def lsm_roofline():
+    """The LSM operator (k_apply, one right-hand side) against the FP32 and
+    FP64 vector peaks.  Algorithmic flops per application are the
+    reference's force loops over the same volume (XX:1406-1500): 26 per
+    bond and 57 per angle, over its bond and angle tables (nbo, nna of load
+    case xx); the kernel time is the mean of 20 launches with HIP events on
+    the handle's stream."""
+    from porescale_amd import geometry as geo
+    from porescale_amd.lsm import LSM, table_counts
+    out = {}
+    for n, ind in ((64, geo.sphere_pack((64, 64, 64), 0.35, 6.0, seed=0)),
+                   (128, geo.two_component(geo.sphere_pack((128, 128, 128), 0.35, 6.0, seed=5), 0.3, seed=6,
+                                           radius=4.0))):
+        tc = table_counts(ind, "xx")
+        flops = 26.0 * tc["nbo"] + 57.0 * tc["nna"]
+        with LSM(ind, 1.0, 0.1, 0.3, 0.02, precision="f32") as s:
+            s.profile(5)
+            t64, t32 = s.profile(20)
+            nodes = s.nsolid_nodes
+        out[f"{n}^3"] = {
+            "bound": "valu", "kernel": "k_apply<float> (one right-hand side)", "unit": "TFLOP/s",
+            "algorithmic_flops_per_launch": flops, "ref_nodes": tc["ns"], "bonds": tc["nbo"], "angles": tc["nna"],
+            "flops_per_node": flops / tc["ns"], "solid_nodes": nodes,
+            "fp32": {"mean_launch_ms": t32, "achieved": flops / (t32 * 1e-3) / 1e12, "peak": 157.3,
+                     "frac": flops / (t32 * 1e-3) / 1e12 / 157.3},
+            "fp64": {"mean_launch_ms": t64, "achieved": flops / (t64 * 1e-3) / 1e12, "peak": 78.6,
+                     "frac": flops / (t64 * 1e-3) / 1e12 / 78.6}}
+    return out
+
+
+def verlet_convergence(ind, chunk=10000, max_steps=300000, tol=1e-5):
+    """Damped-Verlet steps (the reference's solver, XX:1377-1545, on the
+    GPU in FP64) until C11 of load case xx is within ``tol`` of its
+    equilibrium value (PCG to 1e-12): the step count one reference solve
+    needs."""
+    from porescale_amd.lsm import LSM
+    with LSM(ind, 1.0, 0.1, precision="f64") as s:
+        s.solve("xx", 1e-12, 200000)
+        c_eq = s.calc_stress("xx")[1][0, 0]
+        done, err = 0, None
+        t0 = time.perf_counter()
+        while done < max_steps:
+            s.verlet("xx", chunk, restart=(done == 0))
+            done += chunk
+            c = s.calc_stress("xx")[1][0, 0]
+            err = abs(c - c_eq) / abs(c_eq)
+            if err <= tol:
+                break
+        t = time.perf_counter() - t0
+    return {"steps": done, "converged": bool(err is not None and err <= tol), "C11_rel_err": err,
+            "C11_equilibrium": c_eq, "gpu_seconds": t, "tol": tol}
+
+
+def run_lsm(calib_steps=50):
     """LSM rows: six-case moduli at 64^3 (plain FP64 vs batched FP32+MG, at
     rtol 1e-10 and 1e-5), uneven sub-volumes in one batch, adjoint gradients
     at 128^3 with a lognormal stiffness field and Gauss-Newton calibration."""
@@ -254,6 +432,8 @@ def run_lsm(full=False):
         t64, t32 = s.profile(20)
         cfg2["matvec_ms"] = {"fp64": t64, "fp32": t32, "solid_nodes": s.nsolid_nodes}
     res["moduli_64"] = cfg2
+    res["roofline"] = lsm_roofline()
+    res["verlet_64"] = verlet_convergence(ind)
     # uneven sub-volumes, two components, one batch
     rng = np.random.default_rng(0)
     sizes = rng.integers(32, 97, size=64)
@@ -303,6 +483,19 @@ def run_lsm(full=False):
             t_a = time.perf_counter() - t0
         adj[prec] = {"forward_s": t_f, "adjoint_s": t_a, "grads_per_s": 1.0 / (t_f + t_a), "K": K5, "G": G5}
     adj["speedup"] = adj["f32"]["grads_per_s"] / adj["f64"]["grads_per_s"]
+    # the same gradient at the metric's tolerance 1e-5 (forward and adjoint)
+    a5 = {}
+    for prec in ("f64", "f32"):
+        with LSM(ind5, *pars, precision=prec, stiff=stiff) as s:
+            s.moduli(1e-5, 200000)  # warm-up: hierarchy and graphs
+            t0 = time.perf_counter()
+            _, K5, G5 = s.moduli(1e-5, 200000)
+            dab, _ = s.adjoint(1.0, 0.0, 1e-5, 200000, want_stiff=False)
+            t_g = time.perf_counter() - t0
+        a5[prec] = {"seconds": t_g, "grads_per_s": 1.0 / t_g, "K": K5, "G": G5, "dK_dab": dab.tolist()}
+    a5["speedup"] = a5["f32"]["grads_per_s"] / a5["f64"]["grads_per_s"]
+    a5["K_rel_err_vs_fp64"] = abs(a5["f32"]["K"] - a5["f64"]["K"]) / a5["f64"]["K"]
+    adj["rtol_1e-05"] = a5
     # plain FP64 (one load case per block-Jacobi PCG solve) on the forward
     # moduli.  The gradient of K weights only the three normal load cases, so
     # its adjoint has three nonzero systems of the same matrix; the batched
@@ -315,12 +508,13 @@ def run_lsm(full=False):
     adj["plain_fp64_forward"] = {"seconds": t_plain, "solves_per_s": 6 / t_plain, "iters": its5,
                                  "speedup_f32_forward": t_plain / adj["f32"]["forward_s"]}
     adj["plain_fp64_gradient_lower_bound"] = {"seconds": t_grad, "speedup_f32": t_grad / f32_grad}
-    steps = 50 if full else 5
+    steps = calib_steps
     tgt = np.array([adj["f32"]["K"], adj["f32"]["G"]]) * np.array([1.05, 0.97])
     t0 = time.perf_counter()
     hist, losses = calibrate(ind5, np.array(pars), tgt, steps=steps, rtol=1e-10, precision="f32", stiff=stiff)
     t_c = time.perf_counter() - t0
-    adj["calibration"] = {"steps": steps, "seconds": t_c, "s_per_step": t_c / steps,
+    adj["calibration"] = {"steps": steps, "handle": "one LSM handle, springs replaced in place each step",
+                          "seconds": t_c, "s_per_step": t_c / steps,
                           "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
                           "target_KG": tgt.tolist(), "fitted_ab": hist[-1].tolist()}
     res["adjoint_128"] = adj
@@ -361,9 +555,11 @@ def main():
     ap.add_argument("--no-lsm", dest="lsm", action="store_false")
     ap.add_argument("--no-cpu", dest="cpu", action="store_false", default=True)
     ap.add_argument("--no-coupled", dest="coupled", action="store_false", default=True)
+    ap.add_argument("--no-precision", dest="precision", action="store_false", default=True)
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB")
-    ap.add_argument("--full", action="store_true", help="50-step calibration in the LSM rows")
+    ap.add_argument("--calib-steps", type=int, default=50, help="Gauss-Newton calibration steps in the LSM rows")
     ap.add_argument("--lsm-only", action="store_true", help="only the LSM rows (for profiling them)")
+    ap.add_argument("--lsm-matvec", action="store_true", help="only the LSM operator roofline (profiling)")
     ap.add_argument("--precision-steps", type=int, default=0, metavar="N",
                     help="FP32 storage against FP64 on the headline workload over N steps (own JSON line)")
     args = ap.parse_args()
@@ -376,8 +572,11 @@ def main():
     if args.calib_copy:
         calib_copy(args.calib_copy)
         return
+    if args.lsm_matvec:
+        print(json.dumps({"lsm_roofline": lsm_roofline()}), flush=True)
+        return
     if args.lsm_only:
-        print(json.dumps({"lsm": run_lsm(args.full)}), flush=True)
+        print(json.dumps({"lsm": run_lsm(args.calib_steps)}), flush=True)
         return
     if args.precision_steps:
         print(json.dumps({"precision": run_precision(args.precision_steps)}), flush=True)
@@ -385,7 +584,7 @@ def main():
     from porescale_amd import _lib
     dev, ncu, gb = _lib.device_info()
     log(f"device {dev}, {ncu} CUs, {gb:.0f} GB")
-    lbm, ind, inp, blue = run_lbm(args.steps, args.warmup)
+    lbm, ind, inp, blue = run_lbm(args.steps, args.warmup, before=run_lbm_f64)
     ms_step = lbm["tot_ms"] / args.steps
     mlups = lbm["nl"] * args.steps / (lbm["tot_ms"] * 1e-3) / 1e6
     traffic, traffic_src = pmc_traffic("k_collide_f32")
@@ -406,7 +605,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "storage": "f32 (total deviation + red fraction)",
+        "storage": "f32 (total deviation + minority-colour fraction)",
         "data": "synthetic (seeded sphere pack and blob phase distribution)",
         "config": {"workload": "lbm-d3q19-twophase-mrt-256^3", "grid": [256, 256, 256],
                    "fluid_nodes": lbm["nl"], "porosity": 0.30, "sphere_radii": [4, 10],
@@ -419,16 +618,23 @@ def main():
                      "mean_launch_ms": lbm["coll_ms"]},
         "lbm_diag": {"permr": lbm["permr"], "permb": lbm["permb"], "sat_b": lbm["sat_b"]},
     }
-    out["lbm_f64"] = run_lbm_f64(ind, inp, blue)
+    out["lbm_f64"] = lbm["before"]
     log(f"LBM f64 storage {json.dumps(out['lbm_f64'])}")
+    out["trt_pack_64"] = run_trt_pack()
+    log(f"TRT pack 64^3 {json.dumps(out['trt_pack_64'])}")
+    if args.precision:
+        out["precision"] = run_precision()
+        log(f"precision {json.dumps(out['precision'])[:600]}")
     if args.coupled:
         out["coupled"] = run_coupled(ind, inp, blue)
         log(f"coupled {json.dumps(out['coupled'])}")
     if args.lsm:
-        out["lsm"] = run_lsm(args.full)
+        out["lsm"] = run_lsm(args.calib_steps)
         log(f"LSM {json.dumps(out['lsm'])[:400]}")
     if args.cpu:
         out["cpu_baseline"] = cpu_baseline(ind, inp, blue)
+        conv = out.get("lsm", {}).get("verlet_64", {})
+        out["cpu_baseline"]["reference"] = cpu_reference(conv.get("steps") if conv.get("converged") else None)
         log(f"CPU baseline {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
 

@@ -157,6 +157,13 @@ int psx_lsm_create(const psx_lsm_params *p, int32_t nx, int32_t ny, int32_t nz,
 void psx_lsm_destroy(psx_lsm *h);
 /* counts[0..3] = solid nodes (periodic grid), solid cubes, nx*ny*nz, reserved */
 int psx_lsm_counts(psx_lsm *h, int64_t *counts);
+/* Replaces the spring constants of both components (the alf/bet pairs the
+   reference derives once from Em, nu at XX:411-414) and refreshes what
+   depends on them in place: per-voxel constants, preconditioners and the
+   multigrid operators, keeping the handle's buffers, node grid and solver
+   hierarchy.  Previous solutions and moduli are invalidated.  Used by
+   calibration loops that re-solve one geometry for many (alf, bet). */
+int psx_lsm_set_springs(psx_lsm *h, double alf1, double bet1, double alf2, double bet2);
 /* Host only, no handle or device: the sizes of the reference's bond and angle
    tables for load case lc (counts[0] = nbo, counts[1] = nna, counts[2] = ns),
    and its partition of the padded node grid into 2*nthread groups of z

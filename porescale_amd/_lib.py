@@ -67,7 +67,7 @@ EXPORTS = (
     "psx_lbm_create", "psx_lbm_destroy", "psx_lbm_counts", "psx_lbm_tables", "psx_lbm_rates",
     "psx_lbm_init_random", "psx_lbm_set_state", "psx_lbm_get_state", "psx_lbm_get_fields", "psx_lbm_fields",
     "psx_lbm_step", "psx_lbm_check_state", "psx_lbm_sync", "psx_lbm_profile", "psx_lbm_footprint",
-    "psx_lsm_create", "psx_lsm_destroy", "psx_lsm_counts", "psx_lsm_table_counts", "psx_lsm_set_u", "psx_lsm_get_u",
+    "psx_lsm_create", "psx_lsm_destroy", "psx_lsm_counts", "psx_lsm_set_springs", "psx_lsm_table_counts", "psx_lsm_set_u", "psx_lsm_get_u",
     "psx_lsm_forces", "psx_lsm_verlet", "psx_lsm_reference_nodes", "psx_lsm_get_verlet", "psx_lsm_set_verlet",
     "psx_lsm_solve", "psx_lsm_stress", "psx_lsm_moduli",
     "psx_lsm_adjoint", "psx_lsm_profile", "psx_lsm_batch_moduli",
@@ -104,6 +104,7 @@ def _declare(L):
     L.psx_lsm_destroy.argtypes = [_vp]
     L.psx_lsm_destroy.restype = None
     L.psx_lsm_counts.argtypes = [_vp, _vp]
+    L.psx_lsm_set_springs.argtypes = [_vp, _dbl, _dbl, _dbl, _dbl]
     L.psx_lsm_table_counts.argtypes = [_i32, _i32, _i32, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp]
     L.psx_lsm_set_u.argtypes = [_vp, _i32, _vp]
     L.psx_lsm_get_u.argtypes = [_vp, _i32, _vp]

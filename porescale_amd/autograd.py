@@ -70,19 +70,23 @@ def calibrate(ind, ab0, target_KG, steps=50, lr=0.2, gstrain=0.01, rtol=1e-10, p
         return np.array(hist), np.array(losses)
     if method != "gauss-newton":
         raise ValueError(f"unknown method {method!r}")
-    for _ in range(steps):
-        ab = np.exp(logab)
-        with LSM(ind, *ab, gstrain=gstrain, precision=precision, stiff=stiff) as s:
+    # one handle for the whole fit: each step replaces its spring constants
+    # (psx_lsm_set_springs), which refreshes the multigrid operators in place
+    with LSM(ind, *np.exp(logab), gstrain=gstrain, precision=precision, stiff=stiff) as s:
+        for k in range(steps):
+            ab = np.exp(logab)
+            if k:
+                s.set_springs(*ab)
             _, K, G = s.moduli(rtol=rtol)
             dK, _ = s.adjoint(1.0, 0.0, rtol=rtol, want_stiff=False)
             dG, _ = s.adjoint(0.0, 1.0, rtol=rtol, want_stiff=False)
-        r = np.array([K, G]) / t - 1.0
-        losses.append(float(r @ r))
-        J = np.stack([dK / t[0], dG / t[1]]) * ab[None, :]
-        delta = -np.linalg.lstsq(J, r, rcond=None)[0]
-        n = np.linalg.norm(delta)
-        if n > 0.5:
-            delta *= 0.5 / n
-        logab = logab + delta
-        hist.append(np.exp(logab))
+            r = np.array([K, G]) / t - 1.0
+            losses.append(float(r @ r))
+            J = np.stack([dK / t[0], dG / t[1]]) * ab[None, :]
+            delta = -np.linalg.lstsq(J, r, rcond=None)[0]
+            n = np.linalg.norm(delta)
+            if n > 0.5:
+                delta *= 0.5 / n
+            logab = logab + delta
+            hist.append(np.exp(logab))
     return np.array(hist), np.array(losses)

@@ -833,6 +833,35 @@ int autotune_tiles(Batch &B, hipStream_t st) {
     return PSX_OK;
 }
 
+// Per-voxel spring constants k_a = alpha_c s_v, k_b = beta_c s_v of the
+// batch's components (c = 1: code 0, c = 2: code 3; pore voxels 0) from
+// (alf1, bet1, alf2, bet2), uploaded in FP64 and FP32 with the inverse
+// diagonal blocks of both block-Jacobi preconditioners.
+int batch_springs(Batch &B, hipStream_t st, const psx_lsm_params &p) {
+    const int64_t off = B.Ntot;
+    std::vector<float> ka32(off), kb32(off);
+    for (int64_t i = 0; i < off; i++) {
+        const double s = B.stiff_h[i];
+        const int c = B.comp_h[i];
+        B.ka_h[i] = c == 1 ? s * p.alf1 : c == 2 ? s * p.alf2 : 0.0;
+        B.kb_h[i] = c == 1 ? s * p.bet1 : c == 2 ? s * p.bet2 : 0.0;
+        ka32[i] = (float)B.ka_h[i];
+        kb32[i] = (float)B.kb_h[i];
+    }
+    PSX_HIP(hipMemcpyAsync(B.d_ka, B.ka_h.data(), sizeof(double) * off, hipMemcpyHostToDevice, st));
+    PSX_HIP(hipMemcpyAsync(B.d_kb, B.kb_h.data(), sizeof(double) * off, hipMemcpyHostToDevice, st));
+    PSX_HIP(hipMemcpyAsync(B.d_ka32, ka32.data(), sizeof(float) * off, hipMemcpyHostToDevice, st));
+    PSX_HIP(hipMemcpyAsync(B.d_kb32, kb32.data(), sizeof(float) * off, hipMemcpyHostToDevice, st));
+    int g = grid_for(B.ntiles);
+    hipLaunchKernelGGL(k_diag_inv<double>, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles,
+                       B.d_ka, B.d_kb, B.d_minv, B.Ntot);
+    hipLaunchKernelGGL(k_diag_inv<float>, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles,
+                       B.d_ka, B.d_kb, B.d_minv32, B.Ntot);
+    PSX_HIP(hipGetLastError());
+    PSX_HIP(hipStreamSynchronize(st));  // the host copies above are the caller's stack
+    return PSX_OK;
+}
+
 // Builds the batch description and uploads voxel scales.
 int batch_build(Batch &B, hipStream_t st, const psx_lsm_params &p, int nvol, const int32_t *dims,
                 const int32_t *ind, const double *stiff) {
@@ -862,14 +891,10 @@ int batch_build(Batch &B, hipStream_t st, const psx_lsm_params &p, int nvol, con
     B.comp_h.assign(off, 0);
     B.stiff_h.assign(off, 1.0);
     for (int64_t i = 0; i < off; i++) {
-        int32_t c = ind[i];
-        double s = stiff ? stiff[i] : 1.0;
-        B.stiff_h[i] = s;
-        if (c == 0) { B.comp_h[i] = 1; B.ka_h[i] = s * p.alf1; B.kb_h[i] = s * p.bet1; }
-        else if (c == 3) { B.comp_h[i] = 2; B.ka_h[i] = s * p.alf2; B.kb_h[i] = s * p.bet2; }
+        const int32_t c = ind[i];
+        B.stiff_h[i] = stiff ? stiff[i] : 1.0;
+        B.comp_h[i] = c == 0 ? 1 : c == 3 ? 2 : 0;
     }
-    std::vector<float> ka32(off), kb32(off);
-    for (int64_t i = 0; i < off; i++) { ka32[i] = (float)B.ka_h[i]; kb32[i] = (float)B.kb_h[i]; }
     RC(dalloc(&B.d_vols, B.vols.size()));
     RC(dalloc(&B.d_tvol, B.tvol.size()));
     RC(dalloc(&B.d_tstart, B.tstart.size()));
@@ -879,18 +904,8 @@ int batch_build(Batch &B, hipStream_t st, const psx_lsm_params &p, int nvol, con
     PSX_HIP(hipMemcpyAsync(B.d_vols, B.vols.data(), sizeof(VolDesc) * B.vols.size(), hipMemcpyHostToDevice, st));
     PSX_HIP(hipMemcpyAsync(B.d_tvol, B.tvol.data(), sizeof(int32_t) * B.tvol.size(), hipMemcpyHostToDevice, st));
     PSX_HIP(hipMemcpyAsync(B.d_tstart, B.tstart.data(), sizeof(int32_t) * B.tstart.size(), hipMemcpyHostToDevice, st));
-    PSX_HIP(hipMemcpyAsync(B.d_ka, B.ka_h.data(), sizeof(double) * off, hipMemcpyHostToDevice, st));
-    PSX_HIP(hipMemcpyAsync(B.d_kb, B.kb_h.data(), sizeof(double) * off, hipMemcpyHostToDevice, st));
-    PSX_HIP(hipMemcpyAsync(B.d_ka32, ka32.data(), sizeof(float) * off, hipMemcpyHostToDevice, st));
-    PSX_HIP(hipMemcpyAsync(B.d_kb32, kb32.data(), sizeof(float) * off, hipMemcpyHostToDevice, st));
+    RC(batch_springs(B, st, p));
     RC(autotune_tiles(B, st));
-    int g = grid_for(B.ntiles);
-    hipLaunchKernelGGL(k_diag_inv<double>, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles,
-                       B.d_ka, B.d_kb, B.d_minv, B.Ntot);
-    hipLaunchKernelGGL(k_diag_inv<float>, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles,
-                       B.d_ka, B.d_kb, B.d_minv32, B.Ntot);
-    PSX_HIP(hipGetLastError());
-    PSX_HIP(hipStreamSynchronize(st));
     return PSX_OK;
 }
 
@@ -1265,6 +1280,20 @@ extern "C" int psx_lsm_counts(psx_lsm *h, int64_t *c) {
     PSX_CHECK_ARG(h && c, "psx_lsm_counts: NULL argument");
     c[0] = h->nsolid_nodes; c[1] = h->ncubes; c[2] = h->B.Ntot; c[3] = 0;
     return PSX_OK;
+}
+
+extern "C" int psx_lsm_set_springs(psx_lsm *h, double alf1, double bet1, double alf2, double bet2) {
+    PSX_CHECK_ARG(h, "psx_lsm_set_springs: NULL handle");
+    PSX_CHECK_ARG(std::isfinite(alf1) && std::isfinite(bet1) && std::isfinite(alf2) && std::isfinite(bet2),
+                  "psx_lsm_set_springs: spring constants must be finite");
+    PSX_GUARD_BEGIN
+    h->p.alf1 = alf1; h->p.bet1 = bet1; h->p.alf2 = alf2; h->p.bet2 = bet2;
+    RC(batch_springs(h->B, h->stream, h->p));
+    if (h->sv.mg.built) RC(mg_operators(h->stream, h->B, h->sv.mg));
+    for (bool &b : h->have_case) b = false;
+    h->have_moduli = false;
+    return PSX_OK;
+    PSX_GUARD_END
 }
 
 namespace {

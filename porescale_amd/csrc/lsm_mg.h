@@ -52,6 +52,10 @@ struct MG {
 // Builds the hierarchy for the batch (R right-hand sides per volume).
 int mg_build(hipStream_t st, const Batch &B, int R, const MGParams &prm, MG &mg);
 void mg_free(MG &mg);
+// Recomputes the operators of a built hierarchy from the batch's current
+// spring constants (B.d_ka, B.d_kb, B.d_minv): coarse stencils, inverse
+// diagonal blocks and spectral bounds.  Levels, tiles and work vectors stay.
+int mg_operators(hipStream_t st, const Batch &B, MG &mg);
 // The PCG update that mg_apply can fold into its first kernel:
 // X += alpha P, r -= alpha Q (alpha per system), partials of r.r into prr.
 struct CGUpdate {

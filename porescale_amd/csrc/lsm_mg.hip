@@ -962,6 +962,17 @@ int mg_build(hipStream_t st, const Batch &B, int R, const MGParams &prm, MG &mg)
         c.ntiles = tile;
     }
     for (int l = 0; l < L; l++) RC(upload_level(st, mg.lv[l]));
+    for (int l = 0; l < L; l++) RC(alloc_level_vectors(mg.lv[l], R, l > 0));
+    RC(mg_operators(st, B, mg));
+    mg.built = true;
+    return PSX_OK;
+}
+
+int mg_operators(hipStream_t st, const Batch &B, MG &mg) {
+    const MGParams &prm = mg.prm;
+    const int L = (int)mg.lv.size();
+    const int nvol = (int)B.vols.size();
+    MGLevel &l0 = mg.lv[0];
     // operators: A0 assembled in FP64, Galerkin products down the levels
     // Af: the FP64 operator of the level being processed (freed as soon as
     // the next level's Galerkin product exists); freed on every way out
@@ -978,15 +989,17 @@ int mg_build(hipStream_t st, const Batch &B, int R, const MGParams &prm, MG &mg)
                        l0.d_tstart, l0.ntiles, B.d_ka, B.d_kb, Af.get(), l0.N);
     for (int l = 0; l < L; l++) {
         MGLevel &lv = mg.lv[l];
-        RC(dalloc(&lv.lam, nvol));
+        if (!lv.lam) RC(dalloc(&lv.lam, nvol));
         if (l == 0) {
             hipLaunchKernelGGL(k_level_prep, dim3(grid_for(lv.ntiles)), dim3(256), 0, st, lv.d_vols, lv.d_tvol,
                                lv.d_tstart, lv.ntiles, (const double *)Af.get(), lv.N, (const double *)B.d_minv,
                                (float *)nullptr, (float *)nullptr, pmax);
         } else {
-            RC(dalloc(&lv.A, (size_t)NSTEN * lv.N));
-            RC(dalloc(&lv.minv, (size_t)6 * lv.N));
-            lv.own_minv = true;
+            if (!lv.A) RC(dalloc(&lv.A, (size_t)NSTEN * lv.N));
+            if (!lv.minv) {
+                RC(dalloc(&lv.minv, (size_t)6 * lv.N));
+                lv.own_minv = true;
+            }
             hipLaunchKernelGGL(k_level_prep, dim3(grid_for(lv.ntiles)), dim3(256), 0, st, lv.d_vols, lv.d_tvol,
                                lv.d_tstart, lv.ntiles, (const double *)Af.get(), lv.N, (const double *)nullptr, lv.A,
                                lv.minv, pmax);
@@ -1015,7 +1028,6 @@ int mg_build(hipStream_t st, const Batch &B, int R, const MGParams &prm, MG &mg)
                          mg.lv[l].vols[0].nx, mg.lv[l].vols[0].ny, mg.lv[l].vols[0].nz, lam[0]);
         }
     }
-    for (int l = 0; l < L; l++) RC(alloc_level_vectors(mg.lv[l], R, l > 0));
     // sharpen the bounds: power iteration on D^-1 A per volume and level
     if (prm.power_steps > 0) {
         psx::CallScope scope;
@@ -1056,7 +1068,6 @@ int mg_build(hipStream_t st, const Batch &B, int R, const MGParams &prm, MG &mg)
             std::fprintf(stderr, "mg level %d: lam0 after power iteration %g\n", l, lam[0]);
         }
     }
-    mg.built = true;
     return PSX_OK;
 }
 

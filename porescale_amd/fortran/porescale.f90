@@ -182,6 +182,14 @@ module porescale
        integer(c_int64_t) :: counts(4)
      end function psx_lsm_counts
 
+     ! new spring constants for the same geometry (calibration loops)
+     function psx_lsm_set_springs(h, alf1, bet1, alf2, bet2) bind(C, name='psx_lsm_set_springs')
+       import :: c_int, c_double, c_ptr
+       integer(c_int) :: psx_lsm_set_springs
+       type(c_ptr), value :: h
+       real(c_double), value :: alf1, bet1, alf2, bet2
+     end function psx_lsm_set_springs
+
      ! host only: bond/angle table sizes and the z-layer partition (lsm_settings)
      function psx_lsm_table_counts(nx, ny, nz, ind, loadcase, nthread, counts, layers, layb, lib, lia) &
           bind(C, name='psx_lsm_table_counts')

@@ -122,6 +122,14 @@ class LSM:
         _lib.check(L.psx_lsm_counts(h, _lib.ptr(c)))
         self.nsolid_nodes, self.ncubes, self.N = int(c[0]), int(c[1]), int(c[2])
 
+    def set_springs(self, alf1: float, bet1: float, alf2: Optional[float] = None, bet2: Optional[float] = None):
+        """New spring constants for the same geometry; the handle, its node
+        grid and multigrid hierarchy are kept and refreshed in place."""
+        a2 = alf1 if alf2 is None else alf2
+        b2 = bet1 if bet2 is None else bet2
+        _lib.check(self._L.psx_lsm_set_springs(self._h, float(alf1), float(bet1), float(a2), float(b2)))
+        self.params.alf1, self.params.bet1, self.params.alf2, self.params.bet2 = alf1, bet1, a2, b2
+
     @classmethod
     def from_input(cls, inp: LSMInput, ind: np.ndarray, **kw):
         a1, b1 = spring_constants(inp.Em1, inp.nu1, inp.dl)

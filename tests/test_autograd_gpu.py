@@ -68,3 +68,26 @@ def test_adam_calibration_decreases_loss():
         tgt = LSMModuli.apply(torch.tensor(true), None, ind, 0.01, 1e-10, "f64").numpy()
     hist, losses = calibrate(ind, true * np.array([1.5, 0.6, 1.3, 1.4]), tgt, steps=15, lr=0.05, method="adam")
     assert losses[-1] < 0.25 * losses[0]
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_set_springs_equals_fresh_handle(precision):
+    """psx_lsm_set_springs refreshes the per-voxel constants, the
+    preconditioners and the multigrid operators in place: moduli and the
+    adjoint after a swap equal those of a handle built with the new
+    constants (16^3, three multigrid levels in FP32)."""
+    from porescale_amd.lsm import LSM
+    ind = geo.two_component(geo.sphere_pack((16, 16, 16), 0.35, 3.0, seed=41), 0.4, seed=42, radius=2.0)
+    a0, a1 = (1.0, 0.1, 0.3, 0.02), (1.4, 0.05, 0.2, 0.03)
+    print(f"case set_springs {precision}", flush=True)
+    with LSM(ind, *a1, precision=precision) as s:
+        C1, K1, G1 = s.moduli(1e-11, 20000)
+        d1, _ = s.adjoint(1.0, 0.5, 1e-11, 20000, want_stiff=False)
+    with LSM(ind, *a0, precision=precision) as s:
+        s.moduli(1e-11, 20000)
+        s.set_springs(*a1)
+        C2, K2, G2 = s.moduli(1e-11, 20000)
+        d2, _ = s.adjoint(1.0, 0.5, 1e-11, 20000, want_stiff=False)
+    np.testing.assert_allclose(C2, C1, rtol=1e-9, atol=1e-12)
+    assert K2 == pytest.approx(K1, rel=1e-10) and G2 == pytest.approx(G1, rel=1e-10)
+    np.testing.assert_allclose(d2, d1, rtol=1e-8, atol=1e-12)
