@@ -768,8 +768,13 @@ extern "C" int psx_lbm_sync(psx_lbm *h) {
 
 extern "C" int psx_lbm_profile(psx_lbm *h, int64_t nsteps, double *ms) {
     PSX_CHECK_ARG(h && ms && nsteps > 0, "psx_lbm_profile: bad argument");
-    std::vector<hipEvent_t> ev(2 * nsteps + 1);
-    for (auto &e : ev) PSX_HIP(hipEventCreate(&e));
+    PSX_GUARD_BEGIN
+    psx::CallScope scope;  // destroys every event on every way out
+    std::vector<hipEvent_t> ev(2 * nsteps + 1, nullptr);
+    for (auto &e : ev) {
+        PSX_HIP(hipEventCreate(&e));
+        scope.event(e);
+    }
     PSX_HIP(hipEventRecord(ev[0], h->stream));
     for (int64_t s = 0; s < nsteps; s++) {
         int rc = launch_collide(h);
@@ -789,11 +794,11 @@ extern "C" int psx_lbm_profile(psx_lbm *h, int64_t nsteps, double *ms) {
     }
     float tot = 0.f;
     PSX_HIP(hipEventElapsedTime(&tot, ev[0], ev[2 * nsteps]));
-    for (auto &e : ev) (void)hipEventDestroy(e);
     ms[0] = tot;
     ms[1] = tc / (double)nsteps;
     ms[2] = td / (double)nsteps;
     return PSX_OK;
+    PSX_GUARD_END
 }
 
 extern "C" int psx_lbm_footprint(psx_lbm *h, int64_t *bytes) {
