@@ -216,6 +216,21 @@ int psx_lsm_moduli(psx_lsm *h, double rtol, int32_t maxit, double *C, double *K,
    d/d(per-voxel stiffness multiplier).  No counterpart in the reference. */
 int psx_lsm_adjoint(psx_lsm *h, double wK, double wG, double rtol, int32_t maxit,
                     double *dab, double *dstiff);
+/* calc_stress with no device allocation, for callers that cannot allocate
+   (code under HIP graph capture, tight serving loops).  No counterpart in
+   the reference: an allocation-free form of psx_lsm_stress.  All pointers
+   are device memory the caller owns: ka_dev, kb_dev [nx*ny*nz] the
+   per-voxel spring constants (alpha or beta of the voxel's component times
+   its stiffness, 0 for pore voxels), u_dev [3][nx*ny*nz] the displacement
+   on the periodic node grid (x fastest), asig_dev [9] the summed cube
+   stress, column-major (force component, bond direction), as
+   psx_lsm_stress returns it.  Queued on `stream` (a hipStream_t, NULL =
+   the default stream); asig_dev is ready when the stream reaches it.  The
+   partial sums live in a library-owned device array of about 5 MiB per GPU
+   that every call on that GPU shares: volumes up to 65536*256 voxels (larger
+   ones return PSX_EINVAL), and calls that overlap on one GPU race. */
+int psx_lsm_stress_noalloc(int32_t nx, int32_t ny, int32_t nz, const double *ka_dev, const double *kb_dev,
+                           const double *u_dev, int32_t loadcase, double gstrain, double *asig_dev, void *stream);
 /* Matvec timing: mean ms of the FP64 and FP32 operator kernels over n calls. */
 int psx_lsm_profile(psx_lsm *h, int32_t n, double *ms);
 

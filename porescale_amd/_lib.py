@@ -70,7 +70,7 @@ EXPORTS = (
     "psx_lsm_create", "psx_lsm_destroy", "psx_lsm_counts", "psx_lsm_set_springs", "psx_lsm_table_counts", "psx_lsm_set_u", "psx_lsm_get_u",
     "psx_lsm_forces", "psx_lsm_verlet", "psx_lsm_reference_nodes", "psx_lsm_get_verlet", "psx_lsm_set_verlet",
     "psx_lsm_solve", "psx_lsm_stress", "psx_lsm_moduli",
-    "psx_lsm_adjoint", "psx_lsm_profile", "psx_lsm_batch_moduli",
+    "psx_lsm_adjoint", "psx_lsm_stress_noalloc", "psx_lsm_profile", "psx_lsm_batch_moduli",
     "psx_cpl_create", "psx_cpl_destroy", "psx_cpl_counts", "psx_cpl_nodes", "psx_cpl_step", "psx_cpl_get_state",
     "psx_cpl_set_state", "psx_cpl_calc_ab",
 )
@@ -114,6 +114,7 @@ def _declare(L):
     L.psx_lsm_stress.argtypes = [_vp, _i32, _vp, _vp]
     L.psx_lsm_moduli.argtypes = [_vp, _dbl, _i32, _vp, C.POINTER(_dbl), C.POINTER(_dbl)]
     L.psx_lsm_adjoint.argtypes = [_vp, _dbl, _dbl, _dbl, _i32, _vp, _vp]
+    L.psx_lsm_stress_noalloc.argtypes = [_i32, _i32, _i32, _vp, _vp, _vp, _i32, _dbl, _vp, _vp]
     L.psx_lsm_profile.argtypes = [_vp, _i32, _vp]
     L.psx_lsm_batch_moduli.argtypes = [C.POINTER(LSMParams), _i32, _vp, _vp, _dbl, _i32, _vp, _vp, _vp]
     L.psx_lsm_reference_nodes.argtypes = [_vp, _i32, _vp, _vp, _vp, _vp]

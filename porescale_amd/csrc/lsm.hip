@@ -1992,3 +1992,99 @@ extern "C" int psx_lsm_batch_moduli(const psx_lsm_params *p, int32_t nvol, const
     return PSX_OK;
     PSX_GUARD_END
 }
+
+// ===================================================== allocation-free ====
+// psx_lsm_stress_noalloc: calc_stress of a caller-held displacement with no
+// device allocation.  Its scratch (volume descriptor, jump, tile lists and
+// the per-tile partial sums of k_stress) is the file-scope device array
+// `scratch`; each GPU holds its own copy for the life of the process, and
+// every call on a GPU shares it, so calls that overlap on one GPU race.
+namespace {
+
+constexpr int NOALLOC_MAX_TILES = 65536;  // volumes of at most 65536 * 256 voxels
+constexpr size_t NOALLOC_OFF_JUMP = 256;  // bytes: VolDesc at 0, Jump at 256
+constexpr size_t NOALLOC_OFF_TVOL = NOALLOC_OFF_JUMP + ((sizeof(Jump) + 255) / 256) * 256;
+constexpr size_t NOALLOC_OFF_TSTART = NOALLOC_OFF_TVOL + sizeof(int32_t) * NOALLOC_MAX_TILES;
+constexpr size_t NOALLOC_OFF_PART = NOALLOC_OFF_TSTART + sizeof(int32_t) * NOALLOC_MAX_TILES;
+constexpr size_t NOALLOC_BYTES = NOALLOC_OFF_PART + sizeof(double) * 9 * NOALLOC_MAX_TILES;
+static_assert(sizeof(VolDesc) <= NOALLOC_OFF_JUMP, "VolDesc does not fit its scratch slot");
+
+static __device__ __align__(256) double scratch[NOALLOC_BYTES / sizeof(double)];
+
+// writes the descriptor of one volume with linear 256-node tiles and the
+// jump of one load case into the scratch
+__global__ void k_noalloc_setup(VolDesc vd, Jump jmp, int ntiles, VolDesc *__restrict__ dvol,
+                                Jump *__restrict__ djump, int32_t *__restrict__ tvol, int32_t *__restrict__ tstart) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) {
+        *dvol = vd;
+        *djump = jmp;
+    }
+    if (t < ntiles) {
+        tvol[t] = 0;
+        tstart[t] = t * TILE;
+    }
+}
+
+// the device address of `scratch` on each device, looked up once per device
+std::mutex noalloc_mu;
+void *noalloc_ptr[64] = {};
+
+int noalloc_scratch(void **out) {
+    int dev = 0;
+    PSX_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) {
+        psx::set_error("psx_lsm_stress_noalloc: device %d out of range", dev);
+        return PSX_EHIP;
+    }
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    if (!noalloc_ptr[dev]) {
+        void *p = nullptr;
+        PSX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        noalloc_ptr[dev] = p;
+    }
+    *out = noalloc_ptr[dev];
+    return PSX_OK;
+}
+
+}  // namespace
+
+extern "C" int psx_lsm_stress_noalloc(int32_t nx, int32_t ny, int32_t nz, const double *ka_dev,
+                                      const double *kb_dev, const double *u_dev, int32_t lc, double gstrain,
+                                      double *asig_dev, void *stream) {
+    PSX_CHECK_ARG(ka_dev && kb_dev && u_dev && asig_dev, "psx_lsm_stress_noalloc: NULL argument");
+    PSX_CHECK_ARG(nx >= 2 && ny >= 2 && nz >= 2, "psx_lsm_stress_noalloc: every dimension must be >= 2");
+    RC(check_case(lc));
+    const int64_t N = (int64_t)nx * ny * nz;
+    const int64_t ntiles = psx::ceil_div(N, TILE);
+    if (ntiles > NOALLOC_MAX_TILES) {
+        psx::set_error("psx_lsm_stress_noalloc: %lld voxels exceed the scratch limit of %lld",
+                       (long long)N, (long long)NOALLOC_MAX_TILES * TILE);
+        return PSX_EINVAL;
+    }
+    PSX_GUARD_BEGIN
+    void *base = nullptr;
+    RC(noalloc_scratch(&base));
+    char *b = static_cast<char *>(base);
+    VolDesc *dvol = reinterpret_cast<VolDesc *>(b);
+    Jump *djump = reinterpret_cast<Jump *>(b + NOALLOC_OFF_JUMP);
+    int32_t *tvol = reinterpret_cast<int32_t *>(b + NOALLOC_OFF_TVOL);
+    int32_t *tstart = reinterpret_cast<int32_t *>(b + NOALLOC_OFF_TSTART);
+    double *partial = reinterpret_cast<double *>(b + NOALLOC_OFF_PART);
+    VolDesc vd{};
+    vd.nx = nx; vd.ny = ny; vd.nz = nz; vd.pad_ = 0;  // linear tiles
+    vd.off = 0; vd.tile0 = 0; vd.ntile = (int)ntiles;
+    Jump jmp{};
+    jmp.J[0][CASE_PD[lc][0]][CASE_PD[lc][1]] = 2.0 * gstrain;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_noalloc_setup, dim3((unsigned)psx::ceil_div(ntiles, 256)), dim3(256), 0, st, vd, jmp,
+                       (int)ntiles, dvol, djump, tvol, tstart);
+    hipLaunchKernelGGL(k_stress, dim3(grid_for((int)ntiles)), dim3(TILE), 0, st, (const VolDesc *)dvol,
+                       (const int32_t *)tvol, (const int32_t *)tstart, (int)ntiles, 1, ka_dev, kb_dev, u_dev, N,
+                       (const Jump *)djump, partial);
+    hipLaunchKernelGGL(k_stress_reduce, dim3(9), dim3(256), 0, st, (const VolDesc *)dvol, 1,
+                       (const double *)partial, asig_dev);
+    PSX_HIP(hipGetLastError());
+    return PSX_OK;
+    PSX_GUARD_END
+}

@@ -79,3 +79,15 @@ def test_fortran_drivers_built_and_report_missing_input(tmp_path):
         assert os.path.exists(exe), f"{name} not built"
         r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=60)
         assert r.returncode != 0 and "INPUT_PARAMETERS" in r.stdout
+
+
+def test_noalloc_limits_checked_before_device_use():
+    """psx_lsm_stress_noalloc refuses volumes beyond its fixed scratch and
+    bad load cases before any device call (the pointers are not touched)."""
+    L = _lib.lib()
+    buf = np.zeros(16)
+    p = _lib.ptr(buf)
+    rc = L.psx_lsm_stress_noalloc(300, 300, 300, p, p, p, 0, 0.01, p, None)
+    assert rc == _lib.PSX_EINVAL and b"scratch" in L.psx_last_error()
+    assert L.psx_lsm_stress_noalloc(4, 4, 4, p, p, p, 6, 0.01, p, None) == _lib.PSX_EINVAL
+    assert L.psx_lsm_stress_noalloc(4, 4, 4, None, p, p, 0, 0.01, p, None) == _lib.PSX_EINVAL
