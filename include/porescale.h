@@ -172,7 +172,9 @@ int psx_lsm_set_springs(psx_lsm *h, double alf1, double bet1, double alf2, doubl
    table sizes XX_P.f / YZ_P.f write to lsm_settings (XX:1314-1348: nbo, nna,
    layers, layb, lib, lia; tables built at XX:522-548, 826-1018, 1140-1262);
    the element operator itself uses no tables.  ind is nx*ny*nz voxel codes,
-   x fastest (0 and 3 solid). */
+   x fastest (0 and 3 solid).  lc = -1 gives the same for the solid lattice
+   of L1 (L1:501-1017: no padding, periodic on every axis, code 0 solid), the
+   entries of its settings_19 (L1:1500-1510). */
 int psx_lsm_table_counts(int32_t nx, int32_t ny, int32_t nz, const int32_t *ind, int32_t loadcase,
                          int32_t nthread, int64_t *counts, int32_t *layers, int64_t *layb,
                          int64_t *lib, int64_t *lia);

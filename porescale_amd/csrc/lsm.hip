@@ -705,14 +705,19 @@ struct Solver {
     Jump *d_jump = nullptr;
     Jump jump_h{};
     MG mg;  // multigrid preconditioner of the FP32 inner solves (built on first use)
-    // instantiated iteration graphs, reused while the captured arguments match
+    // instantiated iteration graphs, reused while everything a capture
+    // records matches: the buffers, the scalars, the multigrid hierarchy
+    // (its generation) and the launch variants read from the environment
     struct GraphKey {
-        const void *X = nullptr;
+        const void *ptr[16] = {};
         double tol2 = 0.0;
-        int maxit = 0, chunk = 0;
+        int maxit = 0, chunk = 0, R = 0, mg_gen = 0, rall = 0, uyz = 0;
         bool mg = false;
         bool operator==(const GraphKey &o) const {
-            return X == o.X && tol2 == o.tol2 && maxit == o.maxit && chunk == o.chunk && mg == o.mg;
+            for (int i = 0; i < 16; i++)
+                if (ptr[i] != o.ptr[i]) return false;
+            return tol2 == o.tol2 && maxit == o.maxit && chunk == o.chunk && R == o.R && mg_gen == o.mg_gen &&
+                   rall == o.rall && uyz == o.uyz && mg == o.mg;
         }
     };
     hipGraphExec_t exec[2] = {nullptr, nullptr};  // [FP64, FP32]
@@ -934,13 +939,14 @@ int solver_alloc(Solver &S, const Batch &B, int R, bool f32) {
 }
 
 void solver_free(Solver &S) {
+    // the graphs first: they record the buffers freed below
+    for (auto &e : S.exec)
+        if (e) (void)hipGraphExecDestroy(e);
     void *bufs[] = {S.X, S.Rr, S.Z, S.P, S.Qv, S.B, S.X32, S.R32, S.Z32, S.P32, S.Q32,
                     S.sys, S.outer, S.prz, S.prr, S.ppq, S.d_jump};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     mg_free(S.mg);
-    for (auto &e : S.exec)
-        if (e) (void)hipGraphExecDestroy(e);
     S = Solver();
 }
 
@@ -1033,7 +1039,13 @@ int cg_run(hipStream_t st, const Batch &B, Solver &S, SysState *sys, T *X, T *Rv
         // the chunk of iterations is captured once per solver and argument set
         const int slot = std::is_same<T, float>::value ? 1 : 0;
         Solver::GraphKey key;
-        key.X = X; key.tol2 = tol2; key.maxit = maxit; key.chunk = chunk; key.mg = mg;
+        const void *p[16] = {X, Rv, Z, P, Qv, ka, kb, minv, sys, B.d_vols, B.d_tvol, B.d_tstart,
+                             S.prz, S.prr, S.ppq, S.d_jump};
+        for (int i = 0; i < 16; i++) key.ptr[i] = p[i];
+        key.tol2 = tol2; key.maxit = maxit; key.chunk = chunk; key.mg = mg; key.R = S.R;
+        key.mg_gen = mg ? S.mg.gen : 0;
+        key.rall = mg ? mg_launch_rall_tiles() : 0;
+        key.uyz = mg ? mg_launch_uyz() : 0;
         if (!S.exec[slot] || !(S.exec_key[slot] == key)) {
             if (S.exec[slot]) (void)hipGraphExecDestroy(S.exec[slot]);
             S.exec[slot] = nullptr;
@@ -1321,7 +1333,8 @@ const int TAB_RING4[6][4] = {{12, 16, 13, 17}, {14, 18, 15, 19}, {8, 12, 10, 14}
 struct PaddedGeom {
     int n[3], G[3], pa;
     const int32_t *ind;
-    bool code_solid(int64_t v) const { return ind[v] == 0 || ind[v] == 3; }
+    bool code3 = true;  // code 3 (the second solid) counts as solid; L1's lattice has code 0 only
+    bool code_solid(int64_t v) const { return ind[v] == 0 || (code3 && ind[v] == 3); }
     bool node_solid(const int (&c)[3]) const {
         for (int corner = 0; corner < 8; corner++) {
             int v[3];
@@ -1353,7 +1366,9 @@ struct PaddedGeom {
 }  // namespace
 
 // Sizes of the reference's bond and angle tables and its z-layer partition
-// for load case lc, from the voxel codes alone (host only).  The rules are
+// for load case lc, from the voxel codes alone (host only).  lc = -1 is the
+// solid lattice of L1 (L1:501-1017): no padding, periodic on every axis, and
+// only code 0 solid.  The rules are
 // the reference's: a bond from solid node i along bond vector j exists when
 // the node it reaches is solid and one of the cubes sharing the bond is
 // (XX:826-1018); an angle of type t at i exists when both arm nodes are solid
@@ -1367,11 +1382,12 @@ extern "C" int psx_lsm_table_counts(int32_t nx, int32_t ny, int32_t nz, const in
     PSX_CHECK_ARG(ind && counts, "psx_lsm_table_counts: NULL argument");
     PSX_CHECK_ARG(nx >= 2 && ny >= 2 && nz >= 2, "psx_lsm_table_counts: every dimension must be >= 2");
     PSX_CHECK_ARG(nthread >= 1, "psx_lsm_table_counts: nthread must be >= 1");
-    PSX_CHECK_ARG(lc >= 0 && lc <= 5, "psx_lsm_table_counts: load case must be 0..5");
+    PSX_CHECK_ARG(lc >= -1 && lc <= 5, "psx_lsm_table_counts: load case must be -1..5");
     PSX_GUARD_BEGIN
     PaddedGeom g;
     g.n[0] = nx; g.n[1] = ny; g.n[2] = nz;
-    g.pa = CASE_PD[lc][0];
+    g.pa = lc < 0 ? -1 : CASE_PD[lc][0];  // -1: L1's fully periodic lattice
+    g.code3 = lc >= 0;
     for (int a = 0; a < 3; a++) g.G[a] = g.n[a] + (a == g.pa ? 2 : 0);
     g.ind = ind;
     int nta[48][2], t = 0;

@@ -47,6 +47,7 @@ struct MG {
     MGParams prm;
     int R = 0;
     bool built = false;
+    int gen = 0;  // changes with every build (new buffers)
 };
 
 // Builds the hierarchy for the batch (R right-hand sides per volume).
@@ -69,6 +70,10 @@ struct CGUpdate {
 // With upd, r is first updated in place as described above.
 void mg_apply(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, const float *r, float *z, double *prz,
               const CGUpdate *upd = nullptr);
+// The launch variants mg_apply selects by environment (PSX_MG_RALL_TILES,
+// PSX_MG_UYZ), read at each call: a captured graph keeps those of its capture.
+int mg_launch_rall_tiles();
+int mg_launch_uyz();
 // Parameters from the environment (PSX_MG_LEVELS, PSX_MG_NU, PSX_MG_NU0, PSX_MG_CDEG,
 // PSX_MG_RATIO, PSX_MG_CRATIO, PSX_MG_POWER); defaults otherwise.
 MGParams mg_params_from_env();

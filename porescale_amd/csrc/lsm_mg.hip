@@ -2,6 +2,7 @@
 // lsm_mg.h for the construction).  All levels of all volumes of a batch are
 // processed by the same launches: work is dealt as 256-node tiles of every
 // volume, as in the solver.
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -779,10 +780,10 @@ void launch_step_t(hipStream_t st, const LevelArgs &L, bool elem, int R, const S
     // All right-hand sides per thread only where the level alone fills the
     // chip: with fewer tiles the R grid rows are the parallelism (64^3:
     // 2x slower with it on every level, 128^3: 15% faster on the 64^3 level).
-    // read at every launch (cheap next to one; graphs are captured once), so
-    // a process can compare the variants
-    const int rall_tiles = env_int("PSX_MG_RALL_TILES", 1024);
-    const int uyz = env_int("PSX_MG_UYZ", 3);
+    // read at every launch, so a process can compare the variants; a graph
+    // records the variant of its capture (the solver's graph key holds both)
+    const int rall_tiles = mg_launch_rall_tiles();
+    const int uyz = mg_launch_uyz();
     if (elem)
         hipLaunchKernelGGL((k_cheb_step<true, NEWD, 1, X0>), dim3(g, R), dim3(TILE), 0, st, L, R, sys, k, x, r, d,
                            dout, rin);
@@ -920,8 +921,13 @@ void mg_free(MG &mg) {
     mg = MG();
 }
 
+int mg_launch_rall_tiles() { return env_int("PSX_MG_RALL_TILES", 1024); }
+int mg_launch_uyz() { return env_int("PSX_MG_UYZ", 3); }
+
 int mg_build(hipStream_t st, const Batch &B, int R, const MGParams &prm, MG &mg) {
+    static std::atomic<int> generation{0};
     mg_free(mg);
+    mg.gen = ++generation;
     mg.prm = prm;
     mg.R = R;
     const int nvol = (int)B.vols.size();

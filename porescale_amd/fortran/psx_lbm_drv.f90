@@ -9,8 +9,10 @@
 ! itw steps (:3951-3961) and appends one output.m row per check_state
 ! (:3274-3282).  Every its-th iteration it writes the fluid part of
 ! save_results (:2067-2155): velocity_field, color_field, pressure<it1> and
-! a block appended to ind_red_blue.  settings_19 receives the node and table
-! counts.
+! a block appended to ind_red_blue.  settings_19 is the reference's
+! (:1442-1514): the run's parameters, node and table counts of the fluid
+! and of the solid lattice (its bond/angle table sizes and z-layer groups,
+! nthread = 1 as the reference fixes it, :87).
 !
 ! When the cube_bon/cube_ang/cube_sbond/cube_abond files are present (the
 ! reference's run directory has them), the driver runs the reference's
@@ -45,6 +47,8 @@ program psx_lbm_drv
   logical :: coupled
   integer(c_int32_t), allocatable :: cbon(:), cang(:)
   real(c_double), allocatable :: csb(:), cab(:)
+  integer(c_int64_t) :: lcounts(3), layb(3), lib(3), lia(3)
+  integer(c_int32_t) :: layers(2)
 
   open(newunit=u, file='INPUT_PARAMETERS', status='old', iostat=ios)
   if (ios /= 0) then
@@ -166,20 +170,7 @@ program psx_lbm_drv
      call psx_check(psx_lbm_set_state(h, fr, fb), 'psx_lbm_set_state')
   end if
 
-  open(newunit=u, file='settings_19', status='replace')
-  write(u, *) 'LBM settings (GPU library)'
-  write(u, *) 'iipore = ', iipore
-  write(u, *) 'npores = ', npores
-  write(u, *) 'surface tension parameter sig = ', sig
-  write(u, *) 'wettability parameter wet = ', wet
-  write(u, *) 'red fluid collision vector Sr = ', Sr
-  write(u, *) 'blue fluid collision vector Sb = ', Sb
-  write(u, *) 'number of fluid nodes nl = ', counts(1)
-  write(u, *) 'streaming links nps = ', counts(2)
-  write(u, *) 'edge bounce-back links npb2 = ', counts(3)
-  write(u, *) 'face bounce-back links npb4 = ', counts(4)
-  write(u, *) 'precision = ', merge('FP32', 'FP64', prec == 1)
-  close(u)
+  call write_settings_19()
 
   coupled = read_cube_tables()
   if (coupled) call psx_check(psx_cpl_create(h, Em, nu, dl, rs, cbon, cang, csb, cab, cpl), 'psx_cpl_create')
@@ -221,6 +212,86 @@ program psx_lbm_drv
   call psx_lbm_destroy(h)
 
 contains
+
+  ! settings_19 in the reference's layout and list-directed formats
+  ! (lsm_lbm_static_alpha_beta_multipore.f:1442-1514): the integers it writes
+  ! are default integers (nbo, nna integer(8)) and the reals real(8), as
+  ! there.
+  subroutine write_settings_19()
+    integer :: uo, k
+    real(8) :: tr(19), tb(19), cb2, alf, bet
+    cb2 = rr0 * cr2 / rb0
+    tr(1) = 1.d0 - 2.d0 * cr2 * dt * dt
+    tr(2:7) = cr2 * dt * dt * (1.d0 / 6.d0)
+    tr(8:19) = cr2 * dt * dt * (1.d0 / 12.d0)
+    tb(1) = 1.d0 - 2.d0 * cb2 * dt * dt
+    tb(2:7) = cb2 * dt * dt * (1.d0 / 6.d0)
+    tb(8:19) = cb2 * dt * dt * (1.d0 / 12.d0)
+    alf = dl * Em / (5.d0 - 10.d0 * nu)
+    bet = dl * dl * dl * Em * (4 * nu - 1) / (20.d0 * (2 * nu * nu + nu - 1))
+    call psx_check(psx_lsm_table_counts(int(ncx, c_int32_t), int(ncy, c_int32_t), int(ncz, c_int32_t), ind, &
+         -1_c_int32_t, 1_c_int32_t, lcounts, layers, layb, lib, lia), 'psx_lsm_table_counts')
+    open(newunit=uo, file='settings_19', status='replace')
+    write(uo, *) 'LBM settings'
+    write(uo, *) 'Physical parameters'
+    write(uo, *) 'iipore = ', iipore
+    write(uo, *) 'npores = ', npores
+    write(uo, *) 'surface tension parameter sig = ', sig
+    write(uo, *) 'wettability parameter wet = ', wet
+    write(uo, *) 'initial red fluid density rr0 = ', rr0
+    write(uo, *) 'red fluid cinematic viscosity vir = ', vir
+    write(uo, *) 'red fluid sound speed square cr2 = ', cr2
+    write(uo, *) 'red fluid lattice weigts tr = ', tr
+    write(uo, *) 'red fluid body force bfr = ', bfr
+    write(uo, *) 'red fluid collision vector Sr = ', Sr
+    write(uo, *) 'initial blue fluid density rb0 = ', rb0
+    write(uo, *) 'blue fluid cinematic viscosity vib = ', vib
+    write(uo, *) 'blue fluid sound speed square cb2 = ', cb2
+    write(uo, *) 'blue fluid lattice weigts tb = ', tb
+    write(uo, *) 'blue fluid body force bfb = ', bfb
+    write(uo, *) 'blue fluid collision vector Sb = ', Sb
+    write(uo, *) 'Computational domain properties'
+    write(uo, *) 'Unit cell size xyz = ', ncx, 'x', ncy, 'x', ncz
+    write(uo, *) 'number of liquid points nl = ', int(counts(1))
+    write(uo, *) 'simple propagation entries nps = ', int(counts(2))
+    write(uo, *) 'bounce-back boundary condition entries npb2 = ', int(counts(3))
+    write(uo, *) 'bounce-back boundary condition entries npb4 = ', int(counts(4))
+    write(uo, *) 'multirefletion boundary condition entries npm = ', 0
+    write(uo, *) 'linear size for Reynolds number lsize = ', lsize
+    write(uo, *) 'Lattice Boltzmann algorithm parameters'
+    write(uo, *) 'interface width parameter B = ', B
+    write(uo, *) 'direction of the body force idir = ', idir
+    write(uo, *) 'number of steps in the main loop itm = ', itm
+    write(uo, *) 'number of steps in the inner loop itw = ', itw
+    write(uo, *) 'save parameter its = ', its
+    write(uo, *) 'permeability convergence test parameter eps = ', eps
+    write(uo, *) 'LBM type (1 - interpolation) mtype = ', mtype
+    write(uo, *) 'read initial state data from archiv? icon = ', icon
+    write(uo, *) 'small number to supress small distributions ee = ', ee
+    write(uo, *) 'LSM settings'
+    write(uo, *) 'Physical parameters'
+    write(uo, *) 'Physical parameters'
+    write(uo, *) 'Young modulus Em = ', Em
+    write(uo, *) 'Poissons ratio nu = ', nu
+    write(uo, *) 'linear spring parameter alf = ', alf
+    write(uo, *) 'angular spring parameter bet = ', bet
+    write(uo, *) 'solid medium density rs = ', rs
+    write(uo, *) 'integration time step dt = ', dt
+    write(uo, *) 'lattice unit step dl = ', dl
+    write(uo, *) 'Computational domain properties'
+    write(uo, *) 'x unit cell size ncx = ', ncx
+    write(uo, *) 'y unit cell size ncy = ', ncy
+    write(uo, *) 'z unit cell size ncz = ', ncz
+    write(uo, *) 'number of solid points ns = ', int(lcounts(3))
+    write(uo, *) 'LSM algorithm parameters'
+    write(uo, *) 'nbo = ', lcounts(1)
+    write(uo, *) 'nna = ', lcounts(2)
+    write(uo, *) 'layers = ', (int(layers(k)), k = 1, 2)
+    write(uo, *) 'layb = ', (int(layb(k)), k = 1, 3)
+    write(uo, *) 'lib = ', (int(lib(k)), k = 1, 3)
+    write(uo, *) 'lia = ', (int(lia(k)), k = 1, 3)
+    close(uo)
+  end subroutine write_settings_19
 
   logical function read_cube_tables()
     integer :: v, st

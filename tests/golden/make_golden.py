@@ -82,6 +82,28 @@ LSM_INP = dict(Em1=18.639, nu1=0.35065, Em2=94.529, nu2=0.0742, rs=1.0, dt=0.01,
                gstrain=0.01)
 
 
+def settings19_golden():
+    """settings_19 as the reference L1 writes it (L1:1442-1514) for the blob
+    case (icon = 1) and the coupled case (icon = 0)."""
+    import dataclasses
+    import shutil
+    import oracle_bind as ob
+    (_, _), (ind1, inp1, blue) = lbm_cases()
+    ind2, inp2 = coupled_case()
+    for name, ind, inp, restart in (("blobs", ind1, inp1, blue), ("coupled", ind2, inp2, None)):
+        d = ob._prep_dir(ind)
+        try:
+            if restart is not None:
+                inp = dataclasses.replace(inp, icon=1)
+                pio.write_lbm_restart(os.path.join(d, "iteration2"), 0,
+                                      *ob.equilibrium_state(inp, ob.fluid_order_mask(ind, restart)))
+            pio.write_lbm_input(os.path.join(d, "INPUT_PARAMETERS"), dataclasses.replace(inp, itm=1, itw=1))
+            ob._run("l1_ref", d, 600.0)
+            shutil.copy(os.path.join(d, "settings_19"), os.path.join(HERE, f"settings_19_{name}.txt"))
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+
+
 def lsm_settings_golden():
     import oracle_bind as ob
     from porescale_amd import io as pio
@@ -128,11 +150,14 @@ def main():
         os.path.join(ref, "velocities19"), 19), velocities9=pio.read_velocities(os.path.join(ref, "velocities9"), 9),
         cube_bon=b, cube_ang=an, cube_sbond=sb, cube_abond=ab)
     lsm_settings_golden()
+    settings19_golden()
     print("golden fixtures written to", HERE)
 
 
 if __name__ == "__main__":
     if sys.argv[1:] == ["settings"]:
         lsm_settings_golden()
+    elif sys.argv[1:] == ["settings19"]:
+        settings19_golden()
     else:
         main()

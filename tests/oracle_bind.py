@@ -220,7 +220,8 @@ class OracleLSM:
                  rs=1.0, dt=0.01, gstrain=0.01, nthread=1, cube=None):
         self.L = lib()
         nz, ny, nx = ind.shape
-        pad, comp = CASES[case]
+        # "periodic": L1's solid lattice (no padded axis, L1:501-1017)
+        pad, comp = (-1, 0) if case == "periodic" else CASES[case]
         p = LSMParams()
         p.nx, p.ny, p.nz, p.pad_axis, p.jump_comp, p.nthread = nx, ny, nz, pad, comp, nthread
         a1, b1 = alf if bet is None else (alf, bet)

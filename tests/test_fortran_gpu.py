@@ -218,3 +218,33 @@ def test_lsm_driver_all_cases_closed_form():
     assert C[1, 0] == pytest.approx(a - 4 * b, rel=1e-8)
     assert C[3, 3] == pytest.approx(a + 6 * b, rel=1e-9)
     assert K == pytest.approx(5 * a / 3, rel=1e-9) and G == pytest.approx(a + 6 * b, rel=1e-9)
+
+
+@pytest.mark.parametrize("case", ["blobs", "coupled"])
+def test_lbm_driver_settings_19_is_the_references(case):
+    """settings_19 written by psx_lbm_drv equals, character for character,
+    the file the reference L1 writes for the same inputs (L1:1442-1514):
+    parameters, fluid table sizes, and the solid lattice's node, bond and
+    angle counts with its z-layer groups (golden settings_19_*.txt, written
+    by tests/golden/make_golden.py from the flang-built L1)."""
+    import dataclasses
+    if case == "blobs":
+        _, (ind, inp, blue) = lbm_cases()
+        inp = dataclasses.replace(inp, icon=1, itm=1, itw=1)
+    else:
+        ind, inp = coupled_case()
+        inp = dataclasses.replace(inp, itm=1, itw=1)
+    d = _prep(ind)
+    try:
+        if case == "blobs":
+            pio.write_lbm_restart(os.path.join(d, "iteration2"), 0,
+                                  *ob.equilibrium_state(inp, ob.fluid_order_mask(ind, blue)))
+        pio.write_lbm_input(os.path.join(d, "INPUT_PARAMETERS"), inp)
+        _run([_drv("psx_lbm_drv")], d)
+        with open(os.path.join(d, "settings_19")) as fh:
+            got = fh.read()
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    with open(os.path.join(GOLD, f"settings_19_{case}.txt")) as fh:
+        want = fh.read()
+    assert got == want

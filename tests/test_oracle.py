@@ -383,3 +383,30 @@ def test_oracle_lbm_threads_change_no_bit():
     finally:
         L.or_set_threads(os.cpu_count() or 1)
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("case", ["blobs", "coupled"])
+def test_l1_lattice_counts_equal_reference_settings_19(case):
+    """The solid lattice of L1 (load case -1 of psx_lsm_table_counts: no
+    padding, periodic, code 0 solid) has the node, bond and angle counts and
+    z-layer groups the reference L1 wrote to settings_19, and the oracle's
+    periodic lattice agrees."""
+    from golden.make_golden import coupled_case
+    from porescale_amd.lsm import table_counts
+    if case == "blobs":
+        ind = lbm_cases()[1][0]
+    else:
+        ind = coupled_case()[0]
+    with open(os.path.join(GOLD, f"settings_19_{case}.txt")) as fh:
+        text = fh.read()
+    vals = {}
+    for line in text.splitlines():
+        key, sep, rest = line.partition("=")
+        if sep:
+            vals[key.strip().split()[-1]] = [int(float(x)) for x in rest.split() if x.replace(".", "").isdigit()]
+    t = table_counts(ind, -1, 1)
+    assert [t["ns"]] == vals["ns"] and [t["nbo"]] == vals["nbo"] and [t["nna"]] == vals["nna"]
+    assert list(t["layers"]) == vals["layers"] and list(t["layb"]) == vals["layb"]
+    assert list(t["lib"]) == vals["lib"] and list(t["lia"]) == vals["lia"]
+    c = ob.OracleLSM(np.where(ind == 0, 0, 1).astype(np.int32), case="periodic", nthread=1).counts()
+    assert (c["ns"], c["nbo"], c["nna"]) == (t["ns"], t["nbo"], t["nna"])
