@@ -1,7 +1,7 @@
 // FP32-storage variant of the two-phase LBM step.  Populations are stored
 // per (direction, node) as
 //   T = (fr + fb) - tr_q*rr0   and   s = the minority colour's fraction of
-//                                        fr + fb, signed by colour (frac_encode)
+//                                        fr + fb with a colour bit (frac_encode)
 // in FP32 (arithmetic in FP64 registers), post-collision and not yet
 // streamed, in AoSoA blocks of 64 nodes (lbm_device.h: aos()).  Streaming is
 // a pull: each node gathers direction q from its source node (src table) or,
@@ -312,10 +312,10 @@ __device__ __forceinline__ double colour_pull(const LbmConst &c, const float *__
         pull_from<M>(P, pb, c.nl, i, q, j, k);
         int64_t a = aos(k, j);
         double t = c.w0[q] + (double)in[a];
-        double pr, pb;
-        frac_split(t, in[a + Q * AOS], pr, pb);
-        rr += pr;
-        rb += pb;
+        double fr, fb;
+        frac_split(t, in[a + Q * AOS], fr, fb);
+        rr += fr;
+        rb += fb;
     }
     return (rr - rb) / (rr + rb);
 }
