@@ -104,6 +104,34 @@ def settings19_golden():
             shutil.rmtree(d, ignore_errors=True)
 
 
+def lsm_converged_geometry():
+    """10^3 two-component pack (solid 0.66, 21 % code 3) for the converged
+    reference moduli."""
+    return geo.two_component(geo.sphere_pack((10, 10, 10), 0.35, 2.5, seed=11), 0.3, seed=12, radius=2.0)
+
+
+def lsm_converged_golden(its=20000, itm=2):
+    """The reference's own solver run to equilibrium: XX_P.f and YZ_P.f for
+    itm * its damped-Verlet steps on lsm_converged_geometry() with the
+    LSM_INP springs (the moduli of the last stress_tensor block agree with
+    the block before to ~1e-14 at 20000 steps a block)."""
+    import oracle_bind as ob
+    ind = lsm_converged_geometry()
+    nz, ny, nx = ind.shape
+    inp = pio.LSMInput(ncx=nx, ncy=ny, ncz=nz, nthread=1, **dict(LSM_INP, itm=itm, its=its))
+    out = {"steps": itm * its}
+    for case in ("xx", "yz"):
+        st = ob.run_ref_lsm(case, ind, inp, timeout=3000)[3]
+        vals = {}
+        for line in st.splitlines():
+            for key in ("Exxxx", "Eyyxx", "Eyyzz", "Eyzyz"):
+                if line.strip().startswith(key):
+                    vals.setdefault(key, []).append(pio.parse_fortran_numbers(line.split("=", 1)[1])[0])
+        for key, v in vals.items():
+            out[f"{case}_{key}"] = np.array(v)
+    np.savez_compressed(os.path.join(HERE, "lsm_converged.npz"), **out)
+
+
 def lsm_settings_golden():
     import oracle_bind as ob
     from porescale_amd import io as pio
@@ -151,6 +179,7 @@ def main():
         cube_bon=b, cube_ang=an, cube_sbond=sb, cube_abond=ab)
     lsm_settings_golden()
     settings19_golden()
+    lsm_converged_golden()
     print("golden fixtures written to", HERE)
 
 
@@ -159,5 +188,7 @@ if __name__ == "__main__":
         lsm_settings_golden()
     elif sys.argv[1:] == ["settings19"]:
         settings19_golden()
+    elif sys.argv[1:] == ["converged"]:
+        lsm_converged_golden()
     else:
         main()

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import oracle_bind as ob
-from golden.make_golden import LSM_INP, lsm_geometry
+from golden.make_golden import LSM_INP, lsm_converged_geometry, lsm_geometry
 from porescale_amd import geometry as geo
 from porescale_amd.lsm import CASES, LSM, batch_moduli, moduli_from_C, reference_node_map, spring_constants
 
@@ -324,3 +324,30 @@ def test_edge_volumes_single_and_batched():
         assert np.all(Cb[0] == 0.0) and np.all(KG[0] == 0.0)
         np.testing.assert_allclose(Cb[1], closed, atol=1e-9)
         np.testing.assert_allclose(Cb[2], singles[4][0], atol=1e-8 * np.abs(singles[4][0]).max())
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_moduli_equal_reference_run_to_equilibrium(precision):
+    """The reference's own solver run to equilibrium (XX_P.f and YZ_P.f, 2 x
+    20000 damped-Verlet steps on a 10^3 two-component pack; golden
+    lsm_converged.npz): the six-case moduli of the GPU solvers (FP64 PCG, and
+    FP32 multigrid PCG with FP64 refinement) and of the batched solve of that
+    volume beside a second one equal its Exxxx, Eyyxx, Eyyzz and Eyzyz to
+    1e-9 relative."""
+    g = np.load(os.path.join(GOLD, "lsm_converged.npz"))
+    # the reference itself is converged: its last two blocks agree
+    for k in ("xx_Exxxx", "xx_Eyyxx", "xx_Eyyzz", "yz_Eyzyz"):
+        assert abs(g[k][-1] - g[k][-2]) <= 1e-12 * abs(g[k][-1])
+    ind = lsm_converged_geometry()
+    a1, b1 = spring_constants(LSM_INP["Em1"], LSM_INP["nu1"])
+    a2, b2 = spring_constants(LSM_INP["Em2"], LSM_INP["nu2"])
+    want = {(0, 0): g["xx_Exxxx"][-1], (1, 0): g["xx_Eyyxx"][-1], (2, 0): g["xx_Eyyzz"][-1],
+            (3, 3): g["yz_Eyzyz"][-1]}
+    print(f"case converged reference moduli 10^3 {precision}", flush=True)
+    with LSM(ind, a1, b1, a2, b2, precision=precision, gstrain=LSM_INP["gstrain"]) as s:
+        C6, K, G = s.moduli(1e-11, 200000)
+    other = geo.two_component(geo.sphere_pack((12, 9, 11), 0.4, 2.5, seed=13), 0.5, seed=14, radius=2.0)
+    Cb, KG, _, _ = batch_moduli([other, ind], a1, b1, a2, b2, precision=precision, rtol=1e-11)
+    for (i, j), v in want.items():
+        assert C6[i, j] == pytest.approx(v, rel=1e-9), (i, j)
+        assert Cb[1][i, j] == pytest.approx(v, rel=1e-9), (i, j)
