@@ -68,7 +68,8 @@ typedef struct {
     double Sr[19], Sb[19];
     int32_t precision;    /* PSX_PREC_F64: FP64 populations, bit-identical to L1
                              PSX_PREC_F32: FP32 storage (total deviation from
-                             tr*rr0 and red fraction), FP64 arithmetic         */
+                             tr*rr0 and minority-colour fraction), FP64
+                             arithmetic                                        */
     int32_t reserved[7];
 } psx_lbm_params;
 
