@@ -4,20 +4,24 @@
 Headline (the JSON line's metric/value): the D3Q19 two-phase colour-gradient
 LBM step with MRT collision at 256^3 (sphere pack, radii 4..10, porosity
 0.30, viscosity ratio 1:10, wetting 0.7071, blob saturation 0.5), FP32
-population storage with FP64 arithmetic: MLUPS over fluid nodes.  The step
-is one pull+collide launch and one pull+density launch; the roofline entry is
-the collide kernel, timed with HIP events on the library's own stream.
+population storage with the colour fraction to 33 bits (f32x) and FP64
+arithmetic: MLUPS over fluid nodes.  The step is one pull+collide launch and
+one pull+density launch; the roofline entry is the collide kernel, timed
+with HIP events on the library's own stream.
 
 Also reported (objects in the same line):
+  lbm_f32      the same LBM workload with the 25-bit colour fraction (f32):
+               ms per step and MLUPS.
   lbm_f64      the same LBM workload with FP64 storage (states bit-identical
                to the reference): ms per step and MLUPS.
   trt_pack_64  the reference method as configured: 64^3 radius-6 pack, FP64,
                TRT tau = 1 with magic parameter 1/4, 2 + 2000 steps: MLUPS,
                k_r, k_b and the blue saturation.
-  precision    FP32 storage against FP64 on the headline workload over 20000
-               steps, with the FP64 control started from a 1e-7 perturbation:
-               permeabilities, saturation, colour masses at the end and
-               averaged over the second half.
+  precision    f32 and f32x storage against FP64 on the headline workload
+               over 20000 steps, with the FP64 control started from a 1e-7
+               perturbation: permeabilities, saturation, colour masses at the
+               end, averaged over the second half and at worst over 40
+               samples.
   coupled      L1's coupled model on the same 256^3 workload: ms per main-loop
                step (LBM + momentum exchange + lsm_step) and calc_ab.
   lsm          64^3 single-solid sphere pack (porosity 0.35), all six load
@@ -35,7 +39,7 @@ Also reported (objects in the same line):
                64^3 cases, and the LSM oracle.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-lsm] [--no-cpu] [--no-coupled]
-                    [--no-precision] [--calib-steps N]
+                    [--no-precision] [--calib-steps N] [--lbm-storage f32x|f32]
     python bench.py --lsm-only         # the LSM rows alone (profiling)
     python bench.py --lsm-matvec       # the LSM operator alone at 64^3 and 128^3 (profiling)
     python bench.py --precision-steps N  # FP32 storage against FP64 over N steps
@@ -72,18 +76,26 @@ def initial_state(inp, ind, blue):
     return cases.initial_state(inp, ind, blue)
 
 
-def run_lbm(steps, warmup, before=None):
+# algorithmic HBM bytes per fluid node and step of the collide kernel: the
+# 19 pulled populations read and the node's own 19 written, each as T and
+# colour word (4 + 4 B) plus the extension byte (f32x), or as fr and fb
+# (8 + 8 B, f64); plus the colour field, 2 x 4 B (f32, f32x) or 2 x 8 B (f64)
+ALG_BYTES = {"f32": 2 * 19 * 8 + 8.0, "f32x": 2 * 19 * 9 + 8.0, "f64": 2 * 19 * 16 + 16.0}
+
+
+def run_lbm(steps, warmup, before=None, prec="f32x"):
     """The headline: ``warmup`` untimed steps, then ``steps`` steps with HIP
     events around every kernel.  ``before(ind, inp, blue)`` runs once the
-    handle holds its start state, just ahead of the warm-up (the FP64
-    storage leg: its GPU work keeps the clocks up while the headline's host
-    set-up would otherwise leave the device idle); its result is returned."""
+    handle holds its start state, just ahead of the warm-up (the other
+    storage legs: their GPU work keeps the clocks up while the headline's
+    host set-up would otherwise leave the device idle); its result is
+    returned."""
     from porescale_amd.lbm import LBM, mrt_rates
     ind, inp, blue = lbm_workload()
     fr0, fb0 = initial_state(inp, ind, blue)
     coll = ("mrt", mrt_rates(inp.vir), mrt_rates(inp.vib))
     out = {}
-    with LBM(inp, ind, collision=coll, precision="f32") as lb:
+    with LBM(inp, ind, collision=coll, precision=prec) as lb:
         lb.set_state(fr0, fb0)
         if before is not None:
             out["before"] = before(ind, inp, blue)
@@ -98,20 +110,34 @@ def run_lbm(steps, warmup, before=None):
     return out, ind, inp, blue
 
 
-def run_lbm_f64(ind, inp, blue, steps=50, warmup=5):
-    """The same workload on the FP64 storage path, whose states are
-    bit-identical to the reference's: what exact parity costs."""
+STORAGE = {"f64": "f64 (bit-identical to the reference)",
+           "f32": "f32 (FP32 total deviation + minority-colour fraction to 25 bits)",
+           "f32x": "f32x (FP32 total deviation + minority-colour fraction to 33 bits: a 32-bit word and a byte)"}
+
+
+def run_lbm_storage(ind, inp, blue, prec, steps=50, warmup=5):
+    """The same workload in another storage: FP64, whose states are
+    bit-identical to the reference's (what exact parity costs), or FP32
+    storage with the other colour-fraction width."""
     from porescale_amd.lbm import LBM, mrt_rates
     coll = ("mrt", mrt_rates(inp.vir), mrt_rates(inp.vib))
-    with LBM(inp, ind, collision=coll, precision="f64") as lb:
+    with LBM(inp, ind, collision=coll, precision=prec) as lb:
         lb.set_state(*initial_state(inp, ind, blue))
         lb.step(warmup)
         lb.sync()
         tot, tc, td = lb.profile(steps)
         nl = lb.nl
-    return {"storage": "f64 (bit-identical to the reference)", "steps": steps, "ms_per_step": tot / steps,
+    alg = ALG_BYTES[prec]
+    return {"storage": STORAGE[prec], "steps": steps, "ms_per_step": tot / steps,
             "collide_ms": tc, "density_ms": td, "mlups": nl * steps / (tot * 1e-3) / 1e6,
-            "algorithmic_bytes_per_node": 624.0, "collide_gbs": 624.0 * nl / (tc * 1e-3) / 1e9}
+            "algorithmic_bytes_per_node": alg, "collide_gbs": alg * nl / (tc * 1e-3) / 1e9}
+
+
+def run_other_storage(headline):
+    """The storage legs other than the headline's, for run_lbm's ``before``."""
+    def before(ind, inp, blue):
+        return {p: run_lbm_storage(ind, inp, blue, p) for p in ("f64", "f32", "f32x") if p != headline}
+    return before
 
 
 def run_coupled(ind, inp, blue, steps=20):
@@ -124,7 +150,7 @@ def run_coupled(ind, inp, blue, steps=20):
     from porescale_amd.lbm import LBM, Coupled, mrt_rates
     inp2 = dataclasses.replace(inp, Em=0.2, nu=0.2, rs=2.0)
     coll = ("mrt", mrt_rates(inp.vir), mrt_rates(inp.vib))
-    with LBM(inp2, ind, collision=coll, precision="f32") as lb:
+    with LBM(inp2, ind, collision=coll, precision="f32x") as lb:
         lb.set_state(*initial_state(inp2, ind, blue))
         lb.step(2)
         with Coupled(lb) as cp:
@@ -161,7 +187,7 @@ def run_trt_pack(n=64, steps=2000):
     ind, inp, coll, S, blue = cases.twophase_trt_pack(n)
     out = {"workload": f"lbm-d3q19-twophase-trt-{n}^3 (tau 1, magic 1/4, radius-6 pack, porosity 0.35)",
            "steps": 2 + steps}
-    for prec in ("f64", "f32"):
+    for prec in ("f64", "f32", "f32x"):
         with LBM(inp, ind, collision=coll, precision=prec) as lb:
             lb.set_state(*cases.initial_state(inp, ind, blue))
             lb.step(2)
@@ -171,18 +197,19 @@ def run_trt_pack(n=64, steps=2000):
             out[prec] = {"ms_per_step": tot / steps, "mlups": lb.nl * steps / (tot * 1e-3) / 1e6,
                          "permr": d.permr, "permb": d.permb, "sat_b": d.outm[11]}
             out["fluid_nodes"] = lb.nl
-    out["storage"] = {"f64": "FP64 (bit-identical to the reference)", "f32": "FP32 deviation + colour fraction"}
+    out["storage"] = {p: STORAGE[p] for p in ("f64", "f32", "f32x")}
     return out
 
 
 def run_precision(steps=20000, every=500, window_from=10000):
     """FP32 storage against FP64 on the headline workload over ``steps``
-    steps: permeabilities, blue saturation and colour masses of three runs:
-    FP64 and FP32 from the same start, and, as the control, FP64 from that
-    start with every population scaled by 1 + 1e-7 u (u uniform in [-1, 1],
-    seed 0): how far a perturbation of the size of FP32 storage rounding
-    alone moves the observables of this transient.  Sampled every ``every``
-    steps; reported at the last step and as the mean over the samples from
+    steps: permeabilities, blue saturation and colour masses of four runs:
+    FP64, FP32 (25-bit colour fraction) and FP32X (33-bit colour fraction)
+    from the same start, and, as the control, FP64 from that start with
+    every population scaled by 1 + 1e-7 u (u uniform in [-1, 1], seed 0):
+    how far a perturbation of the size of FP32 storage rounding alone moves
+    the observables of this transient.  Sampled every ``every`` steps;
+    reported at the last step and as the mean over the samples from
     ``window_from`` on (the transient's permeabilities fluctuate)."""
     import numpy as np
     from porescale_amd.lbm import LBM, mrt_rates
@@ -196,8 +223,9 @@ def run_precision(steps=20000, every=500, window_from=10000):
     out = {"workload": "lbm-d3q19-twophase-mrt-256^3", "steps": steps, "sample_every": every,
            "window": [window_from, steps]}
     series = {}
+    legs = ("f32", "f32x", "f64_perturbed_start")
     for name, prec, st in (("f64", "f64", (fr0, fb0)), ("f64_perturbed_start", "f64", (fr1, fb1)),
-                           ("f32", "f32", (fr0, fb0))):
+                           ("f32", "f32", (fr0, fb0)), ("f32x", "f32x", (fr0, fb0))):
         rows = []
         with LBM(inp, ind, collision=coll, precision=prec) as lb:
             lb.set_state(*st)
@@ -213,16 +241,15 @@ def run_precision(steps=20000, every=500, window_from=10000):
         last = series[name][-1]
         out[name] = dict(zip(keys, (float(x) for x in last[1:])))
     w = series["f64"][:, 0] >= window_from
-    for name in ("f64", "f64_perturbed_start", "f32"):
+    for name in ("f64",) + legs:
         out[name]["window_mean"] = dict(zip(keys, (float(x) for x in series[name][w, 1:].mean(0))))
     for i, k in enumerate(keys):
         ref, ref_w = series["f64"][-1, i + 1], series["f64"][w, i + 1].mean()
-        out[f"rel_{k}"] = {n: abs(series[n][-1, i + 1] - ref) / abs(ref) for n in ("f32", "f64_perturbed_start")}
-        out[f"rel_{k}_window_mean"] = {n: abs(series[n][w, i + 1].mean() - ref_w) / abs(ref_w)
-                                       for n in ("f32", "f64_perturbed_start")}
+        out[f"rel_{k}"] = {n: abs(series[n][-1, i + 1] - ref) / abs(ref) for n in legs}
+        out[f"rel_{k}_window_mean"] = {n: abs(series[n][w, i + 1].mean() - ref_w) / abs(ref_w) for n in legs}
         out[f"max_rel_{k}_over_samples"] = {
             n: float(np.max(np.abs(series[n][:, i + 1] - series["f64"][:, i + 1]) / np.abs(series["f64"][:, i + 1])))
-            for n in ("f32", "f64_perturbed_start")}
+            for n in legs}
     return out
 
 
@@ -526,9 +553,10 @@ def run_lsm(calib_steps=50):
     return res
 
 
-def pmc_traffic(kernel, profile="profiles/k_collide_f32_lbm256_mrt.md"):
+def pmc_traffic(kernel, profile=None):
     """HBM bytes per launch of ``kernel`` from the committed rocprofv3 --pmc
     summary (FETCH_SIZE and WRITE_SIZE, each scaled by a copy of known size)."""
+    profile = profile or f"profiles/{kernel}_lbm256_mrt.md"
     try:
         with open(os.path.join(ROOT, profile)) as f:
             for line in f:
@@ -563,6 +591,8 @@ def main():
     ap.add_argument("--no-precision", dest="precision", action="store_false", default=True)
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB")
     ap.add_argument("--calib-steps", type=int, default=50, help="Gauss-Newton calibration steps in the LSM rows")
+    ap.add_argument("--lbm-storage", choices=("f32x", "f32"), default="f32x",
+                    help="storage of the headline LBM workload (profiling the other one)")
     ap.add_argument("--lsm-only", action="store_true", help="only the LSM rows (for profiling them)")
     ap.add_argument("--lsm-matvec", action="store_true", help="only the LSM operator roofline (profiling)")
     ap.add_argument("--precision-steps", type=int, default=0, metavar="N",
@@ -589,11 +619,13 @@ def main():
     from porescale_amd import _lib
     dev, ncu, gb = _lib.device_info()
     log(f"device {dev}, {ncu} CUs, {gb:.0f} GB")
-    lbm, ind, inp, blue = run_lbm(args.steps, args.warmup, before=run_lbm_f64)
+    prec = args.lbm_storage
+    lbm, ind, inp, blue = run_lbm(args.steps, args.warmup, before=run_other_storage(prec), prec=prec)
     ms_step = lbm["tot_ms"] / args.steps
     mlups = lbm["nl"] * args.steps / (lbm["tot_ms"] * 1e-3) / 1e6
-    traffic, traffic_src = pmc_traffic("k_collide_f32")
-    alg_b = 312.0  # per fluid node and step: 2 colours x 19 pops x (read+write) x 4 B + phi r/w 2 x 4 B
+    kname = "k_collide_f32x" if prec == "f32x" else "k_collide_f32"
+    traffic, traffic_src = pmc_traffic(kname)
+    alg_b = ALG_BYTES[prec]
     coll_gbs = alg_b * lbm["nl"] / (lbm["coll_ms"] * 1e-3) / 1e9
     step_gbs = alg_b * lbm["nl"] / (ms_step * 1e-3) / 1e9
     log(f"LBM nl={lbm['nl']} {ms_step:.3f} ms/step ({lbm['coll_ms']:.3f} collide + {lbm['dens_ms']:.3f} density)"
@@ -610,21 +642,22 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "storage": "f32 (total deviation + minority-colour fraction)",
+        "storage": STORAGE[prec],
         "data": "synthetic (seeded sphere pack and blob phase distribution)",
         "config": {"workload": "lbm-d3q19-twophase-mrt-256^3", "grid": [256, 256, 256],
                    "fluid_nodes": lbm["nl"], "porosity": 0.30, "sphere_radii": [4, 10],
                    "viscosity_ratio": "1:10", "wet": 0.7071, "saturation_blue": 0.5},
         "hbm_frac_step": step_gbs / HBM_PEAK_GBS,
-        "roofline": {"bound": "hbm", "kernel": "k_collide_f32", "achieved": coll_gbs, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": coll_gbs, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": coll_gbs / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": traffic_src,
                      "algorithmic_bytes_per_node": alg_b, "nodes_per_launch": lbm["nl"],
                      "mean_launch_ms": lbm["coll_ms"]},
         "lbm_diag": {"permr": lbm["permr"], "permb": lbm["permb"], "sat_b": lbm["sat_b"]},
     }
-    out["lbm_f64"] = lbm["before"]
-    log(f"LBM f64 storage {json.dumps(out['lbm_f64'])}")
+    for p, leg in lbm["before"].items():
+        out[f"lbm_{p}"] = leg
+        log(f"LBM {p} storage {json.dumps(leg)}")
     out["trt_pack_64"] = run_trt_pack()
     log(f"TRT pack 64^3 {json.dumps(out['trt_pack_64'])}")
     if args.precision:

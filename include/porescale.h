@@ -50,7 +50,7 @@ int psx_device_info(char *name, int32_t name_len, int32_t *num_cu, double *hbm_g
 
 /* =============================================================== LBM ===== */
 enum { PSX_COLL_REF_TRT = 0, PSX_COLL_TRT = 1, PSX_COLL_MRT = 2 };
-enum { PSX_PREC_F64 = 0, PSX_PREC_F32 = 1 };
+enum { PSX_PREC_F64 = 0, PSX_PREC_F32 = 1, PSX_PREC_F32X = 2 };
 
 /* Run parameters: lines 1-8, 9, 17, 19, 21, 22 of the LBM INPUT_PARAMETERS
    (L1:435-458) plus dt (L1:465).  sig is the value as read (the library
@@ -68,8 +68,10 @@ typedef struct {
     double Sr[19], Sb[19];
     int32_t precision;    /* PSX_PREC_F64: FP64 populations, bit-identical to L1
                              PSX_PREC_F32: FP32 storage (total deviation from
-                             tr*rr0 and minority-colour fraction), FP64
-                             arithmetic                                        */
+                             tr*rr0, and the minority colour's fraction to a
+                             25-bit significand), FP64 arithmetic
+                             PSX_PREC_F32X: the same with the fraction to 33
+                             bits (one more byte per population)         */
     int32_t reserved[7];
 } psx_lbm_params;
 
@@ -128,7 +130,8 @@ int psx_lbm_sync(psx_lbm *h);
    ms[0] = total wall ms (event to event), ms[1] = mean collide kernel ms,
    ms[2] = mean density kernel ms. */
 int psx_lbm_profile(psx_lbm *h, int64_t nsteps, double *ms);
-/* Device bytes per fluid node held by the state (for reporting). */
+/* Device bytes per fluid node of one population buffer (the state; the step
+   keeps two), for reporting: 304 (FP64), 152 (FP32), 171 (FP32X). */
 int psx_lbm_footprint(psx_lbm *h, int64_t *bytes);
 
 /* =============================================================== LSM ===== */

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libporescale.so")
 
 PSX_OK, PSX_EINVAL, PSX_EHIP, PSX_ENOMEM, PSX_ESTATE, PSX_ECONV = 0, -1, -2, -3, -4, -5
 COLL_REF_TRT, COLL_TRT, COLL_MRT = 0, 1, 2
-PREC_F64, PREC_F32 = 0, 1
+PREC_F64, PREC_F32, PREC_F32X = 0, 1, 2
 LOAD_CASES = ("xx", "yy", "zz", "yz", "zx", "xy")
 
 

@@ -22,11 +22,11 @@
 //                per-plane bases (36 B per node instead of 72) when every
 //                difference fits, as the int32 table otherwise.  T keeps the
 //                small deviation of the total population (momentum is exact to
-//                FP32 ulps of the deviation); the colour fraction keeps the
-//                minority colour to FP32 relative precision and is exact (-0
-//                or +0) for single-colour populations, so pure-phase nodes
-//                keep exactly zero minority density as the reference's branch
-//                test rb<=ee requires.
+//                FP32 ulps of the deviation); the colour word (lbm_frac.h)
+//                keeps the minority colour's fraction to a 25-bit significand
+//                and is exact for single-colour populations, so pure-phase
+//                nodes keep exactly zero minority density as the reference's
+//                branch test rb<=ee requires.
 //   vox[nl]      voxel index of each node in the phi grid.
 //   phi grid     dense nx*ny*nz FP64, z fastest (index z + nz*(y + ny*x)):
 //                colour field cf at fluid voxels, `wet` at solid voxels, so
@@ -67,6 +67,7 @@ struct psx_lbm {
     hipStream_t stream = nullptr;
     int32_t *d_dst = nullptr, *d_vox = nullptr;
     int16_t *d_src16 = nullptr;  // FP32 path: compressed pull table (null: use d_dst)
+    int32_t *d_srca = nullptr;   // FP32 path: node-major int32 pull table [nl][20] for collide
     int32_t *d_pbase = nullptr;  // [3][nx] per-plane bases of d_src16
     void *d_pop[2] = {nullptr, nullptr};
     void *d_phi = nullptr;
@@ -74,6 +75,7 @@ struct psx_lbm {
     int cur = 0;
     int xcd = 1;
     bool f32 = false;
+    bool f32x = false;  // FP32 storage with the colour words' extension bytes
     // FP32 fused step (k_step_f32): nodes numbered brick by brick; perm maps
     // that numbering to the reference's (empty: the reference's own order)
     bool fused = false;
@@ -195,12 +197,19 @@ void fill_const(psx_lbm *h) {
     c.nx = h->nx; c.ny = h->ny; c.nz = h->nz;
 }
 
+// extension bytes of population buffer buf (FP32 storage with extended
+// colour words), else null: they follow the 2*19*aos_pad(nl) floats
+uint8_t *popx(psx_lbm *h, int buf) {
+    if (!h->f32x) return nullptr;
+    return (uint8_t *)((float *)h->d_pop[buf] + (size_t)2 * Q * aos_pad(h->nl > 0 ? h->nl : 1));
+}
+
 int launch_density(psx_lbm *h, int buf) {
     int64_t nb = psx::ceil_div(h->nl, 256);
     if (nb == 0) return PSX_OK;
     if (h->f32) {
-        PSX_HIP(psx_lbm_fast::density(h->c, (const float *)h->d_pop[buf], h->d_dst, h->d_src16, h->d_pbase,
-                                      h->d_vox, (double *)h->d_phi, h->xcd, h->stream));
+        PSX_HIP(psx_lbm_fast::density(h->c, (const float *)h->d_pop[buf], popx(h, buf), h->d_dst, h->d_src16,
+                                      h->d_pbase, h->d_vox, (double *)h->d_phi, h->xcd, h->stream));
     } else {
         StoreF64 s{(double *)h->d_pop[buf], h->nl};
         hipLaunchKernelGGL(k_density<StoreF64>, dim3((unsigned)nb), dim3(256), 0, h->stream, h->c, s,
@@ -218,8 +227,9 @@ int launch_collide(psx_lbm *h) {
         PSX_HIP(psx_lbm_fast::step(h->c, (const float *)h->d_pop[a], (float *)h->d_pop[b], h->d_dst, h->d_vox,
                                    h->d_bricks, h->nbricks, h->d_hnode, h->d_hcell, h->box, h->xcd, h->stream));
     } else if (h->f32) {
-        PSX_HIP(psx_lbm_fast::collide(h->c, (const float *)h->d_pop[a], (float *)h->d_pop[b], h->d_dst, h->d_src16,
-                                      h->d_pbase, h->d_vox, (const double *)h->d_phi, h->xcd, h->stream));
+        PSX_HIP(psx_lbm_fast::collide(h->c, (const float *)h->d_pop[a], popx(h, a), (float *)h->d_pop[b], popx(h, b),
+                                      h->d_dst, h->d_src16, h->d_pbase, h->d_srca, h->d_vox, (const double *)h->d_phi,
+                                      h->xcd, h->stream));
     } else {
         StoreF64 si{(double *)h->d_pop[a], h->nl}, so{(double *)h->d_pop[b], h->nl};
         hipLaunchKernelGGL(k_collide<StoreF64>, dim3((unsigned)nb), dim3(256), 0, h->stream, h->c, si, so,
@@ -304,9 +314,12 @@ void build_bricks(psx_lbm *h, const int (&bd)[3], std::vector<int32_t> &vox,
 }
 
 // host-side conversion of one population pair into FP32 storage: T = f - w0
-// and the signed minority-colour fraction s (frac_encode)
-inline void f32_encode(double w0, double fr, double fb, float &T, float &s) {
-    s = frac_encode(fr, fb);
+// and the colour word s of the minority colour's fraction, with its
+// extension byte x when ext (frac_encode_t)
+inline void f32_encode(double w0, double fr, double fb, bool ext, float &T, float &s, uint8_t &x) {
+    uint32_t xb = 0;
+    s = ext ? frac_encode_t<8>(fr, fb, xb) : frac_encode(fr, fb);
+    x = (uint8_t)xb;
     T = (float)((fr + fb) - w0);
 }
 
@@ -320,7 +333,7 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
     PSX_CHECK_ARG(p->dt > 0.0 && p->rr0 > 0.0 && p->rb0 > 0.0, "psx_lbm_create: dt, rr0, rb0 must be > 0");
     PSX_CHECK_ARG(p->idir >= 1 && p->idir <= 3, "psx_lbm_create: idir must be 1, 2 or 3");
     PSX_CHECK_ARG(p->collision >= 0 && p->collision <= 2, "psx_lbm_create: unknown collision model");
-    PSX_CHECK_ARG(p->precision == PSX_PREC_F64 || p->precision == PSX_PREC_F32,
+    PSX_CHECK_ARG(p->precision == PSX_PREC_F64 || p->precision == PSX_PREC_F32 || p->precision == PSX_PREC_F32X,
                   "psx_lbm_create: unknown precision");
     *out = nullptr;
     int rc = psx::require_device();
@@ -330,7 +343,8 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
     psx::CreateGuard<psx_lbm> guard(h, psx_lbm_destroy);
     h->p = *p;
     h->nx = nx; h->ny = ny; h->nz = nz;
-    h->f32 = p->precision == PSX_PREC_F32;
+    h->f32 = p->precision == PSX_PREC_F32 || p->precision == PSX_PREC_F32X;
+    h->f32x = p->precision == PSX_PREC_F32X;
     if (const char *e = std::getenv("PSX_LBM_XCD")) h->xcd = std::atoi(e);  // block->XCD remap on/off
     const int64_t nv = (int64_t)nx * ny * nz;
     h->ind_raw.assign(ind, ind + nv);
@@ -373,7 +387,7 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
     std::vector<psx_lbm_fast::Brick> bricks;
     std::vector<int32_t> hnode;
     std::vector<int16_t> hcell;
-    if (h->f32 && nl > 0) {  // opt-in: measured slower than the two-kernel step (occupancy 2)
+    if (h->f32 && !h->f32x && nl > 0) {  // opt-in: measured slower than the two-kernel step (occupancy 2)
         const char *ef = std::getenv("PSX_LBM_FUSED");
         h->fused = ef && std::atoi(ef) != 0;
     }
@@ -451,16 +465,35 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
     }
     fill_const(h);
     PSX_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-    size_t popb = h->f32 ? (size_t)aos_pad(nl > 0 ? nl : 1) * 2 * Q * sizeof(float)
+    size_t popb = h->f32 ? (size_t)aos_pad(nl > 0 ? nl : 1) * Q * (2 * sizeof(float) + (h->f32x ? 1 : 0))
                          : (size_t)(nl > 0 ? nl : 1) * 2 * Q * sizeof(double);
     PSX_HIP(hipMalloc(&h->d_pop[0], popb));
     PSX_HIP(hipMalloc(&h->d_pop[1], popb));
     PSX_HIP(hipMemsetAsync(h->d_pop[0], 0, popb, h->stream));
     PSX_HIP(hipMalloc(&h->d_dst, sizeof(int32_t) * 18 * (size_t)(nl > 0 ? nl : 1)));
     PSX_HIP(hipMalloc(&h->d_vox, sizeof(int32_t) * (size_t)(nl > 0 ? nl : 1)));
+    // With the extended colour words (three loads per population) a collide
+    // wavefront would hold more loads than it can keep in flight (63) with
+    // 18 table loads per node: its int32 table, taken when the direct int16
+    // form does not fit, goes node-major, five 16-byte loads a node
+    // (measured at 256^3: collide 0.61 ms instead of 0.92; without the
+    // extension it gains nothing, 0.567 vs 0.562 ms; PSX_LBM_SRCAOS=0/1
+    // forces it off/on):
+    const char *ea = std::getenv("PSX_LBM_SRCAOS");
+    const bool node_major = ea ? std::atoi(ea) != 0 : h->f32x;
+    if (h->f32 && !h->fused && nl > 0 && !(use16 && direct)) {
+        if (node_major) {
+            std::vector<int32_t> sa(20 * (size_t)nl, -1);
+            for (int q = 1; q < Q; q++)
+                for (int64_t n = 0; n < nl; n++) sa[20 * (size_t)n + (q - 1)] = h->dst_h[(size_t)(q - 1) * nl + n];
+            PSX_HIP(hipMalloc(&h->d_srca, sizeof(int32_t) * sa.size()));
+            PSX_HIP(hipMemcpyAsync(h->d_srca, sa.data(), sizeof(int32_t) * sa.size(), hipMemcpyHostToDevice, h->stream));
+            PSX_HIP(hipStreamSynchronize(h->stream));
+        }
+    }
     if (use16) {
-        PSX_HIP(hipMalloc(&h->d_src16, sizeof(int16_t) * 18 * (size_t)nl));
-        PSX_HIP(hipMemcpyAsync(h->d_src16, s16.data(), sizeof(int16_t) * 18 * nl, hipMemcpyHostToDevice, h->stream));
+        PSX_HIP(hipMalloc(&h->d_src16, sizeof(int16_t) * s16.size()));
+        PSX_HIP(hipMemcpyAsync(h->d_src16, s16.data(), sizeof(int16_t) * s16.size(), hipMemcpyHostToDevice, h->stream));
         if (!direct) {  // the plane-based form needs the bases
             PSX_HIP(hipMalloc(&h->d_pbase, sizeof(int32_t) * 3 * (size_t)nx));
             PSX_HIP(hipMemcpyAsync(h->d_pbase, pbase.data(), sizeof(int32_t) * 3 * nx, hipMemcpyHostToDevice, h->stream));
@@ -500,7 +533,7 @@ extern "C" void psx_lbm_destroy(psx_lbm *h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void *bufs[] = {h->d_pop[0], h->d_pop[1], h->d_dst, h->d_vox, h->d_phi, h->d_partial, h->d_red, h->d_src16,
-                    h->d_pbase, h->d_bricks, h->d_hnode, h->d_hcell};
+                    h->d_pbase, h->d_bricks, h->d_hnode, h->d_hcell, h->d_srca};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -591,6 +624,7 @@ extern "C" int psx_lbm_set_state(psx_lbm *h, const double *fr, const double *fb)
         // stored state is post-collision and not yet streamed: place f_q(i)
         // where the pull of node i reads it (inverse of the streaming step)
         std::vector<float> st((size_t)2 * Q * aos_pad(nl), 0.0f);
+        std::vector<uint8_t> sx(h->f32x ? (size_t)Q * aos_pad(nl) : 1, 0);
         for (int64_t i = 0; i < nl; i++)
             for (int q = 0; q < Q; q++) {
                 int64_t j = i;
@@ -601,9 +635,12 @@ extern "C" int psx_lbm_set_state(psx_lbm *h, const double *fr, const double *fb)
                     else k = opp(q);
                 }
                 const int64_t r = h->ref_node(i);
-                f32_encode(h->c.w0[q], fr[Q * r + q], fb[Q * r + q], st[aos(k, j)], st[aos(Q + k, j)]);
+                f32_encode(h->c.w0[q], fr[Q * r + q], fb[Q * r + q], h->f32x, st[aos(k, j)], st[aos(Q + k, j)],
+                           sx[h->f32x ? aosx(k, j) : 0]);
             }
         PSX_HIP(hipMemcpyAsync(h->d_pop[0], st.data(), sizeof(float) * st.size(), hipMemcpyHostToDevice, h->stream));
+        if (h->f32x)
+            PSX_HIP(hipMemcpyAsync(popx(h, 0), sx.data(), sx.size(), hipMemcpyHostToDevice, h->stream));
         PSX_HIP(hipStreamSynchronize(h->stream));
     } else {
         std::vector<double> st((size_t)2 * Q * nl);
@@ -629,7 +666,10 @@ extern "C" int psx_lbm_get_state(psx_lbm *h, double *fr, double *fb) {
     const int64_t nl = h->nl;
     if (h->f32) {
         std::vector<float> st((size_t)2 * Q * aos_pad(nl));
+        std::vector<uint8_t> sx(h->f32x ? (size_t)Q * aos_pad(nl) : 0);
         PSX_HIP(hipMemcpyAsync(st.data(), h->d_pop[h->cur], sizeof(float) * st.size(), hipMemcpyDeviceToHost, h->stream));
+        if (h->f32x)
+            PSX_HIP(hipMemcpyAsync(sx.data(), popx(h, h->cur), sx.size(), hipMemcpyDeviceToHost, h->stream));
         PSX_HIP(hipStreamSynchronize(h->stream));
         for (int64_t i = 0; i < nl; i++)
             for (int q = 0; q < Q; q++) {
@@ -642,7 +682,7 @@ extern "C" int psx_lbm_get_state(psx_lbm *h, double *fr, double *fb) {
                 }
                 double f = h->c.w0[q] + (double)st[aos(k, j)];
                 const int64_t o = h->ref_node(i);
-                frac_split(f, st[aos(Q + k, j)], fr[Q * o + q], fb[Q * o + q]);
+                frac_split(f, st[aos(Q + k, j)], fr[Q * o + q], fb[Q * o + q], h->f32x ? sx[aosx(k, j)] : 0u);
             }
     } else {
         std::vector<double> st((size_t)2 * Q * nl);
@@ -700,7 +740,7 @@ extern "C" int psx_lbm_fields(psx_lbm *h, double *u, double *color, double *pres
     if (e == hipSuccess) {
         const unsigned nb = (unsigned)psx::ceil_div(nl, 256);
         if (h->f32) {
-            e = psx_lbm_fast::fields(h->c, (const float *)h->d_pop[h->cur], h->d_dst, h->d_vox,
+            e = psx_lbm_fast::fields(h->c, (const float *)h->d_pop[h->cur], popx(h, h->cur), h->d_dst, h->d_vox,
                                      (const double *)h->d_phi, d_out, d_ph, h->stream);
         } else {
             StoreF64 st{(double *)h->d_pop[h->cur], nl};
@@ -803,7 +843,7 @@ extern "C" int psx_lbm_profile(psx_lbm *h, int64_t nsteps, double *ms) {
 
 extern "C" int psx_lbm_footprint(psx_lbm *h, int64_t *bytes) {
     PSX_CHECK_ARG(h && bytes, "psx_lbm_footprint: NULL argument");
-    *bytes = (int64_t)2 * Q * (h->f32 ? 4 : 8);
+    *bytes = (int64_t)Q * (h->f32 ? 2 * 4 + (h->f32x ? 1 : 0) : 2 * 8);
     return PSX_OK;
 }
 
@@ -818,7 +858,7 @@ extern "C" int psx_lbm_check_state(psx_lbm *h, int64_t it1, int64_t itw, psx_lbm
             if (rc) return rc;
         }
         if (h->f32) {
-            PSX_HIP(psx_lbm_fast::diag(h->c, (const float *)h->d_pop[h->cur], h->d_dst, h->d_vox,
+            PSX_HIP(psx_lbm_fast::diag(h->c, (const float *)h->d_pop[h->cur], popx(h, h->cur), h->d_dst, h->d_vox,
                                        (const double *)h->d_phi, h->d_partial, h->stream));
         } else {
             StoreF64 st{(double *)h->d_pop[h->cur], h->nl};

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <type_traits>
 
+#include "lbm_frac.h"
 #include "psx_common.h"
 
 namespace psx_lbm_dev {
@@ -48,61 +49,19 @@ struct StoreF64 {
     }
 };
 
-// FP32 colour storage of one population f = fr + fb: the fraction of its
-// minority colour (the one of smaller magnitude) rounded to the upper 31 bits
-// of a float (a 23-bit mantissa), and in its lowest mantissa bit which colour
-// that is (1: blue).
-//   bit 0 = 0:  fr = s f        bit 0 = 1:  fb = s f   (s with bit 0 cleared)
-// The minority colour keeps a relative precision of 2^-23 however small it
-// is (a red fraction R = fr/f stored as such rounds a blue share below 3e-8
-// to zero, which flips such nodes into the pure-red branch of the exact
-// rb <= ee test), and single-colour populations are exact: the bit patterns
-// 0x00000001 (all red) and 0x00000000 (all blue).  Decoding is branch-free
-// for any real fraction, so colours of opposite sign need no special case.
-__host__ __device__ inline float frac_encode(double fr, double fb) {
-    const double f = fr + fb;
-    const bool blue = std::fabs(fb) < std::fabs(fr);
-    // correctly rounded division: an approximate reciprocal biases the split
-    const double r = f == 0.0 ? 0.5 : (blue ? fb : fr) / f;
-    const float s = (float)r;
-    uint32_t b;
-    std::memcpy(&b, &s, 4);
-    if (b & 1u) {  // round to the nearer of the two neighbours with bit 0 clear
-        const uint32_t lo = b & ~1u, hi = lo + 2u;
-        float slo, shi;
-        std::memcpy(&slo, &lo, 4);
-        std::memcpy(&shi, &hi, 4);
-        b = std::fabs(r - (double)slo) <= std::fabs((double)shi - r) ? lo : hi;
-    }
-    b |= blue ? 1u : 0u;
-    float out;
-    std::memcpy(&out, &b, 4);
-    return out;
-}
-// red and blue parts of the population f whose stored fraction is s
-__host__ __device__ inline void frac_split(double f, float s, double &fr, double &fb) {
-    uint32_t b;
-    std::memcpy(&b, &s, 4);
-    const uint32_t c = b & ~1u;
-    float a;
-    std::memcpy(&a, &c, 4);
-    const double m = (double)a * f, o = f - m;
-    const bool blue = b & 1u;
-    fr = blue ? o : m;
-    fb = blue ? m : o;
-}
-// the population is all red / all blue
-__device__ __forceinline__ bool frac_all_red(float s) { return __float_as_uint(s) == 1u; }
-__device__ __forceinline__ bool frac_all_blue(float s) { return __float_as_uint(s) == 0u; }
-
 // FP32 pull layout: post-collision populations in AoSoA blocks of 64 nodes,
 // block-major: element (slot k, node n) at (n/64)*38*64 + k*64 + n%64, slots
-// 0..18 hold T, 19..37 the colour fraction s (frac_encode).  A node pulls
+// 0..18 hold T, 19..37 the colour word s (frac_encode).  A node pulls
 // direction q from the node src streams from, or from its own opposite slot
-// for a bounce-back link.
+// for a bounce-back link.  With the extended colour word the extension
+// bytes follow the float blocks, in byte blocks of 19 x 64: element
+// (direction k, node n) at aosx(k, n) bytes past the 2*19*aos_pad(nl) floats.
 constexpr int AOS = 64;
 __host__ __device__ __forceinline__ int64_t aos(int k, int64_t n) {
     return (n >> 6) * (2 * Q * AOS) + (int64_t)k * AOS + (n & (AOS - 1));
+}
+__host__ __device__ __forceinline__ int64_t aosx(int k, int64_t n) {
+    return (n >> 6) * (Q * AOS) + (int64_t)k * AOS + (n & (AOS - 1));
 }
 __host__ __device__ __forceinline__ int64_t aos_pad(int64_t nl) { return (nl + AOS - 1) / AOS * AOS; }
 
@@ -113,6 +72,7 @@ struct PullF32 {
     const float *__restrict__ p;
     const int32_t *__restrict__ src;  // [18][nl], -1 = bounce-back
     int64_t nl;
+    const uint8_t *__restrict__ x;  // extension bytes of the colour words, or null
     __device__ __forceinline__ void load_c(const LbmConst &c, int q, int64_t i, double &fr, double &fb) const {
         int64_t j = i;
         int k = q;
@@ -122,7 +82,7 @@ struct PullF32 {
             else k = opp(q);
         }
         double f = c.w0[q] + (double)p[aos(k, j)];  // w0 is the same for q and opp(q)
-        frac_split(f, p[aos(Q + k, j)], fr, fb);
+        frac_split(f, p[aos(Q + k, j)], fr, fb, x ? (uint32_t)x[aosx(k, j)] : 0u);
     }
 };
 

@@ -13,19 +13,22 @@ struct Brick {
     int32_t x0, y0, z0, pad_;
 };
 
-// Pull + collide over all nodes; `in`/`out` are AoSoA (T, R) blocks.  The
-// pull table is src16 (int16 residuals, -32768 = bounce-back) with the
-// per-plane bases pbase[3][nx] when src16 is not null, else src [18][nl]
-// (-1 = bounce-back).
+// Pull + collide over all nodes; `in`/`out` are AoSoA (T, s) blocks and
+// `inx`/`outx` the extension bytes of the colour words (null: the plain
+// 25-bit words).  The pull table is src16 (int16 residuals, -32768 =
+// bounce-back) with the per-plane bases pbase[3][nx] when src16 is not null,
+// else srca [nl][20] (node-major, two pad entries) when not null, else src
+// [18][nl] (-1 = bounce-back).
 // true when the collide kernel needs 64-bit pull offsets (the population
 // buffer of nl nodes holds 2^31 floats or more; PSX_LBM_WIDE=1 forces it)
 bool wide_index(int64_t nl);
-hipError_t collide(const psx_lbm_dev::LbmConst &c, const float *in, float *out, const int32_t *src,
-                   const int16_t *src16, const int32_t *pbase, const int32_t *vox, const double *phi, int xcd,
-                   hipStream_t st);
+hipError_t collide(const psx_lbm_dev::LbmConst &c, const float *in, const uint8_t *inx, float *out, uint8_t *outx,
+                   const int32_t *src, const int16_t *src16, const int32_t *pbase, const int32_t *srca,
+                   const int32_t *vox, const double *phi, int xcd, hipStream_t st);
 // Pulled densities -> colour field phi at every node's voxel.
-hipError_t density(const psx_lbm_dev::LbmConst &c, const float *in, const int32_t *src, const int16_t *src16,
-                   const int32_t *pbase, const int32_t *vox, double *phi, int xcd, hipStream_t st);
+hipError_t density(const psx_lbm_dev::LbmConst &c, const float *in, const uint8_t *inx, const int32_t *src,
+                   const int16_t *src16, const int32_t *pbase, const int32_t *vox, double *phi, int xcd,
+                   hipStream_t st);
 // Fused pull + colour + collide step over bricks (k_step_f32): one launch
 // per time step.  box = the LDS box (brick + halo) dimensions (x, y, z).
 hipError_t step(const psx_lbm_dev::LbmConst &c, const float *in, float *out, const int32_t *src,
@@ -34,11 +37,11 @@ hipError_t step(const psx_lbm_dev::LbmConst &c, const float *in, float *out, con
 // Per-link coupling term of the FP32 state (k_link_force).
 hipError_t link_force(const psx_lbm_dev::LbmConst &c, const float *pop, const psx_lbm_dev::CplLinks &L, hipStream_t st);
 // check_state partial sums (k_diag) of the pulled state.
-hipError_t diag(const psx_lbm_dev::LbmConst &c, const float *in, const int32_t *src, const int32_t *vox,
-                const double *phi, double *partial, hipStream_t st);
+hipError_t diag(const psx_lbm_dev::LbmConst &c, const float *in, const uint8_t *inx, const int32_t *src,
+                const int32_t *vox, const double *phi, double *partial, hipStream_t st);
 
 // save_results fields (k_fields) of the pulled state: out [5][nl], phase [nl].
-hipError_t fields(const psx_lbm_dev::LbmConst &c, const float *in, const int32_t *src, const int32_t *vox,
-                  const double *phi, double *out, int32_t *phase, hipStream_t st);
+hipError_t fields(const psx_lbm_dev::LbmConst &c, const float *in, const uint8_t *inx, const int32_t *src,
+                  const int32_t *vox, const double *phi, double *out, int32_t *phase, hipStream_t st);
 
 }  // namespace psx_lbm_fast

@@ -1,8 +1,8 @@
 // FP32-storage variant of the two-phase LBM step.  Populations are stored
 // per (direction, node) as
-//   T = (fr + fb) - tr_q*rr0   and   s = the minority colour's fraction of
-//                                        fr + fb with a colour bit (frac_encode)
-// in FP32 (arithmetic in FP64 registers), post-collision and not yet
+//   T = (fr + fb) - tr_q*rr0   (FP32)   and   s = the colour word of the
+//   minority colour's fraction of fr + fb (25-bit significand, lbm_frac.h)
+// in 32 bits each (arithmetic in FP64 registers), post-collision and not yet
 // streamed, in AoSoA blocks of 64 nodes (lbm_device.h: aos()).  Streaming is
 // a pull: each node gathers direction q from its source node (src table) or,
 // for a bounce-back link, from its own opposite slot, and writes its own
@@ -26,18 +26,22 @@ __host__ __device__ constexpr int cxq(int q) {
     return q == 1 || q == 7 || q == 9 || q == 11 || q == 13 ? 1 : q == 2 || q == 8 || q == 10 || q == 12 || q == 14 ? -1 : 0;
 }
 
-// Pull table in one of three forms (template parameter M):
+// Pull table in one of four forms (template parameter M):
 //   M = 2: src = i + d, d the int16 offset (fits when an x-plane holds fewer
 //          than ~32k fluid nodes);
 //   M = 1: src = i + pb[cx_q + 1] + d, d the int16 rank difference of source
 //          and node within their x-planes and pb the per-plane bases of the
 //          node's plane;
 //   in both d = -32768 marks a bounce-back link;
-//   M = 0: the int32 source index (-1: bounce-back).
+//   M = 0: the int32 source index (-1: bounce-back), direction-major [18][nl];
+//   M = 3: the same node-major, srca[20 i + q - 1] (two pad entries): five
+//          16-byte loads per node instead of 18, which leaves a wavefront
+//          room for more population loads in flight.
 struct PullIdx {
     const int32_t *__restrict__ src;
     const int16_t *__restrict__ src16;
     const int32_t *__restrict__ pbase;  // [3][nx]
+    const int32_t *__restrict__ srca = nullptr;  // [nl][20] (M = 3)
 };
 
 template <int M>
@@ -70,10 +74,15 @@ __device__ __forceinline__ void pull_from(const PullIdx &P, const int32_t (&pb)[
     }
 }
 
-__device__ __forceinline__ void put(const LbmConst &c, float *__restrict__ out, int q, int64_t j, double fr,
-                                    double fb) {
+// X: the colour word's extension byte goes to outx (33-bit fraction)
+template <bool X>
+__device__ __forceinline__ void put(const LbmConst &c, float *__restrict__ out, uint8_t *__restrict__ outx, int q,
+                                    int64_t j, double fr, double fb) {
     out[aos(q, j)] = (float)((fr + fb) - c.w0[q]);
-    out[aos(Q + q, j)] = frac_encode(fr, fb);
+    uint32_t x;
+    const float s = frac_encode_t<X ? 8 : 0>(fr, fb, x);
+    out[aos(Q + q, j)] = s;
+    if (X) outx[aosx(q, j)] = (uint8_t)x;
 }
 
 // The collision of L1:2407-2620 with the relaxation vector selected per lane
@@ -187,36 +196,68 @@ __device__ __forceinline__ void collide_fast(const LbmConst &c, double t1, doubl
 }
 
 // Pull + collide + recolour of node i (voxel v) into its own slots of out.
-// fs(x, y, z, Cg, Fs) evaluates calcFs at the node's voxel.
-template <int M, class CalcFs, bool WIDE = false>
+// fs(x, y, z, Cg, Fs) evaluates calcFs at the node's voxel.  X: extended
+// colour words, their bytes in inx / outx.
+template <int M, class CalcFs, bool WIDE = false, bool X = false>
 __device__ __forceinline__ void collide_pull(const LbmConst &c, const float *__restrict__ in,
-                                             float *__restrict__ out, const PullIdx &P, int64_t i, int32_t v,
+                                             const uint8_t *__restrict__ inx, float *__restrict__ out,
+                                             uint8_t *__restrict__ outx, const PullIdx &P, int64_t i, int32_t v,
                                              const CalcFs &fs) {
     const int64_t nl = c.nl;
     int32_t pb[3];
     load_bases<M>(c, P, v, pb);
+    int32_t sa[20];  // M = 3: the node's source indices
+    if (M == 3) {
+        const int4 *p4 = reinterpret_cast<const int4 *>(P.srca + 20 * i);
+#pragma unroll
+        for (int g = 0; g < 5; g++) {
+            const int4 w = p4[g];
+            sa[4 * g] = w.x; sa[4 * g + 1] = w.y; sa[4 * g + 2] = w.z; sa[4 * g + 3] = w.w;
+        }
+    }
     double f[Q];
     // where direction q is pulled from: an AoSoA element index, 32-bit while
     // the population buffer holds fewer than 2^31 floats (nl < 5.6e7), 64-bit
     // beyond (WIDE: 19 more registers)
     typename std::conditional<WIDE, int64_t, int32_t>::type at[Q];
+    uint32_t xs[Q];  // extension bytes (X)
     double rr = 0.0, rb = 0.0;
-    bool allred = true, allblue = true;
+    bool allred = true, allblue = true, esc = false;
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         int64_t j;
         int k;
-        pull_from<M>(P, pb, nl, i, q, j, k);
+        if (M == 3) {
+            j = i;
+            k = q;
+            if (q > 0) {
+                if (sa[q - 1] >= 0) j = sa[q - 1];
+                else k = opp(q);
+            }
+        } else {
+            pull_from<M>(P, pb, nl, i, q, j, k);
+        }
         at[q] = (decltype(at[0] + 0))aos(k, j);
         double t = c.w0[q] + (double)in[at[q]];
         float s = in[at[q] + Q * AOS];
+        xs[q] = X ? (uint32_t)inx[aosx(k, j)] : 0u;
         double a, b;
-        frac_split(t, s, a, b);
+        frac_split_fast(t, s, xs[q], a, b, esc);
         f[q] = t;
         rr += a;
         rb += b;
-        allred &= frac_all_red(s);
-        allblue &= frac_all_blue(s);
+        allred &= frac_all_red(s) && xs[q] == 0u;
+        allblue &= frac_all_blue(s) && xs[q] == 0u;
+    }
+    if (esc) {  // a fraction outside [0, 1) (parts of opposite sign): rare
+        rr = 0.0;
+        rb = 0.0;
+        for (int q = 0; q < Q; q++) {
+            double a, b;
+            frac_split(f[q], in[at[q] + Q * AOS], a, b, xs[q]);
+            rr += a;
+            rb += b;
+        }
     }
     const double dt2 = c.dt2;
     double bf[3];
@@ -231,7 +272,7 @@ __device__ __forceinline__ void collide_pull(const LbmConst &c, const float *__r
 #pragma unroll
             for (int q = 0; q < Q; q++) {
                 double a, b;
-                frac_split(f[q], in[at[q] + Q * AOS], a, b);
+                frac_split(f[q], in[at[q] + Q * AOS], a, b, xs[q]);
                 f[q] = a;
             }
         }
@@ -243,7 +284,7 @@ __device__ __forceinline__ void collide_pull(const LbmConst &c, const float *__r
 #pragma unroll
             for (int q = 0; q < Q; q++) {
                 double a, b;
-                frac_split(f[q], in[at[q] + Q * AOS], a, b);
+                frac_split(f[q], in[at[q] + Q * AOS], a, b, xs[q]);
                 f[q] = b;
             }
         }
@@ -274,7 +315,7 @@ __device__ __forceinline__ void collide_pull(const LbmConst &c, const float *__r
     if (mode == 0) { o_r = f[0]; o_b = c.tb[0] * rb; }
     else if (mode == 1) { o_b = f[0]; o_r = c.tr[0] * rr; }
     else { o_r = f[0] * ar; o_b = f[0] * ab; }
-    put(c, out, 0, i, o_r, o_b);
+    put<X>(c, out, outx, 0, i, o_r, o_b);
     const double cn = 0.70710678118654752440;
 #pragma unroll
     for (int q = 1; q < Q; q++) {
@@ -294,14 +335,15 @@ __device__ __forceinline__ void collide_pull(const LbmConst &c, const float *__r
             o_r = (f[q] + tw * rb) * ar;
             o_b = (f[q] - tw * rr) * ab;
         }
-        put(c, out, q, i, o_r, o_b);
+        put<X>(c, out, outx, q, i, o_r, o_b);
     }
 }
 
 // Colour field of node i from its pulled populations.
-template <int M>
-__device__ __forceinline__ double colour_pull(const LbmConst &c, const float *__restrict__ in, const PullIdx &P,
-                                              int64_t i, int32_t v) {
+template <int M, bool X = false>
+__device__ __forceinline__ double colour_pull(const LbmConst &c, const float *__restrict__ in,
+                                              const uint8_t *__restrict__ inx, const PullIdx &P, int64_t i,
+                                              int32_t v) {
     int32_t pb[3];
     load_bases<M>(c, P, v, pb);
     double rr = 0.0, rb = 0.0;
@@ -313,34 +355,36 @@ __device__ __forceinline__ double colour_pull(const LbmConst &c, const float *__
         int64_t a = aos(k, j);
         double t = c.w0[q] + (double)in[a];
         double fr, fb;
-        frac_split(t, in[a + Q * AOS], fr, fb);
+        frac_split(t, in[a + Q * AOS], fr, fb, X ? (uint32_t)inx[aosx(k, j)] : 0u);
         rr += fr;
         rb += fb;
     }
     return (rr - rb) / (rr + rb);
 }
 
-template <int M, bool WIDE>
+template <int M, bool WIDE, bool X>
 __global__ __launch_bounds__(BLOCK) void k_collide_f32(const LbmConst c, const float *__restrict__ in,
-                                                       float *__restrict__ out, const PullIdx P,
+                                                       const uint8_t *__restrict__ inx, float *__restrict__ out,
+                                                       uint8_t *__restrict__ outx, const PullIdx P,
                                                        const int32_t *__restrict__ vox,
                                                        const double *__restrict__ phi, int xcd) {
     int64_t b = xcd ? psx::xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
     int64_t i = b * blockDim.x + threadIdx.x;
     if (i >= c.nl) return;
     auto fs = [&](int x, int y, int z, double *Cg, double *Fs) { calc_fs<double, false>(c, phi, x, y, z, Cg, Fs); };
-    collide_pull<M, decltype(fs), WIDE>(c, in, out, P, i, vox[i], fs);
+    collide_pull<M, decltype(fs), WIDE, X>(c, in, inx, out, outx, P, i, vox[i], fs);
 }
 
-template <int M>
+template <int M, bool X>
 __global__ __launch_bounds__(BLOCK) void k_density_f32(const LbmConst c, const float *__restrict__ in,
-                                                       const PullIdx P, const int32_t *__restrict__ vox,
-                                                       double *__restrict__ phi, int xcd) {
+                                                       const uint8_t *__restrict__ inx, const PullIdx P,
+                                                       const int32_t *__restrict__ vox, double *__restrict__ phi,
+                                                       int xcd) {
     int64_t b = xcd ? psx::xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
     int64_t i = b * blockDim.x + threadIdx.x;
     if (i >= c.nl) return;
     const int32_t v = vox[i];
-    phi[v] = colour_pull<M>(c, in, P, i, v);
+    phi[v] = colour_pull<M, X>(c, in, inx, P, i, v);
 }
 
 // One whole step over one brick of voxels per workgroup: the colour field of
@@ -364,7 +408,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_f32(const LbmConst c, const floa
     __syncthreads();
     for (int k = threadIdx.x; k < B.hcount; k += BLOCK) {
         const int64_t i = hnode[B.hstart + k];
-        phs[hcell[B.hstart + k]] = colour_pull<0>(c, in, P, i, vox[i]);
+        phs[hcell[B.hstart + k]] = colour_pull<0>(c, in, nullptr, P, i, vox[i]);
     }
     __syncthreads();
     auto fs = [&](int x, int y, int z, double *Cg, double *Fs) {
@@ -374,7 +418,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_f32(const LbmConst c, const floa
     };
     for (int k = threadIdx.x; k < B.ncount; k += BLOCK) {
         const int64_t i = B.nstart + k;
-        collide_pull<0>(c, in, out, P, i, vox[i], fs);
+        collide_pull<0>(c, in, nullptr, out, nullptr, P, i, vox[i], fs);
     }
 }
 
@@ -387,44 +431,63 @@ bool wide_index(int64_t nl) {
     return (e && std::atoi(e) != 0) || aos_pad(nl > 0 ? nl : 1) * 2 * Q >= (int64_t)INT32_MAX;
 }
 
-hipError_t collide(const LbmConst &c, const float *in, float *out, const int32_t *src, const int16_t *src16,
-                   const int32_t *pbase, const int32_t *vox, const double *phi, int xcd, hipStream_t st) {
+namespace {
+template <int M, bool X>
+void launch_collide(const LbmConst &c, const float *in, const uint8_t *inx, float *out, uint8_t *outx,
+                    const PullIdx &P, const int32_t *vox, const double *phi, int xcd, hipStream_t st, int64_t nb) {
+    if (wide_index(c.nl))
+        hipLaunchKernelGGL((k_collide_f32<M, true, X>), dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, inx, out, outx,
+                           P, vox, phi, xcd);
+    else
+        hipLaunchKernelGGL((k_collide_f32<M, false, X>), dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, inx, out,
+                           outx, P, vox, phi, xcd);
+}
+template <int M>
+void launch_density(const LbmConst &c, const float *in, const uint8_t *inx, const PullIdx &P, const int32_t *vox,
+                    double *phi, int xcd, hipStream_t st, int64_t nb) {
+    if (inx)
+        hipLaunchKernelGGL((k_density_f32<M, true>), dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, inx, P, vox, phi,
+                           xcd);
+    else
+        hipLaunchKernelGGL((k_density_f32<M, false>), dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, inx, P, vox,
+                           phi, xcd);
+}
+}  // namespace
+
+hipError_t collide(const LbmConst &c, const float *in, const uint8_t *inx, float *out, uint8_t *outx,
+                   const int32_t *src, const int16_t *src16, const int32_t *pbase, const int32_t *srca,
+                   const int32_t *vox, const double *phi, int xcd, hipStream_t st) {
     int64_t nb = psx::ceil_div(c.nl, BLOCK);
     if (nb == 0) return hipSuccess;
     // the collide kernel is latency-bound (occupancy 3): it takes the direct
-    // int16 offsets when they exist, else the int32 table, whose source
-    // indices need no dependent load of a plane base
+    // int16 offsets when they exist, else the node-major int32 table (five
+    // loads a node, no dependent load of a plane base), else the
+    // direction-major one
     if (src16 && !pbase) {
         const PullIdx P{src, src16, nullptr};
-        if (wide_index(c.nl))
-            hipLaunchKernelGGL((k_collide_f32<2, true>), dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, out, P, vox,
-                               phi, xcd);
-        else
-            hipLaunchKernelGGL((k_collide_f32<2, false>), dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, out, P, vox,
-                               phi, xcd);
+        if (inx) launch_collide<2, true>(c, in, inx, out, outx, P, vox, phi, xcd, st, nb);
+        else launch_collide<2, false>(c, in, inx, out, outx, P, vox, phi, xcd, st, nb);
+    } else if (srca) {
+        const PullIdx P{src, nullptr, nullptr, srca};
+        if (inx) launch_collide<3, true>(c, in, inx, out, outx, P, vox, phi, xcd, st, nb);
+        else launch_collide<3, false>(c, in, inx, out, outx, P, vox, phi, xcd, st, nb);
     } else {
         const PullIdx P{src, nullptr, nullptr};
-        if (wide_index(c.nl))
-            hipLaunchKernelGGL((k_collide_f32<0, true>), dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, out, P, vox,
-                               phi, xcd);
-        else
-            hipLaunchKernelGGL((k_collide_f32<0, false>), dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, out, P, vox,
-                               phi, xcd);
+        if (inx) launch_collide<0, true>(c, in, inx, out, outx, P, vox, phi, xcd, st, nb);
+        else launch_collide<0, false>(c, in, inx, out, outx, P, vox, phi, xcd, st, nb);
     }
     return hipGetLastError();
 }
 
-hipError_t density(const LbmConst &c, const float *in, const int32_t *src, const int16_t *src16,
-                   const int32_t *pbase, const int32_t *vox, double *phi, int xcd, hipStream_t st) {
+hipError_t density(const LbmConst &c, const float *in, const uint8_t *inx, const int32_t *src,
+                   const int16_t *src16, const int32_t *pbase, const int32_t *vox, double *phi, int xcd,
+                   hipStream_t st) {
     int64_t nb = psx::ceil_div(c.nl, BLOCK);
     if (nb == 0) return hipSuccess;
     const PullIdx P{src, src16, pbase};
-    if (src16 && pbase)
-        hipLaunchKernelGGL(k_density_f32<1>, dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, P, vox, phi, xcd);
-    else if (src16)
-        hipLaunchKernelGGL(k_density_f32<2>, dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, P, vox, phi, xcd);
-    else
-        hipLaunchKernelGGL(k_density_f32<0>, dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, P, vox, phi, xcd);
+    if (src16 && pbase) launch_density<1>(c, in, inx, P, vox, phi, xcd, st, nb);
+    else if (src16) launch_density<2>(c, in, inx, P, vox, phi, xcd, st, nb);
+    else launch_density<0>(c, in, inx, P, vox, phi, xcd, st, nb);
     return hipGetLastError();
 }
 
@@ -459,20 +522,20 @@ hipError_t link_force(const LbmConst &c, const float *pop, const CplLinks &L, hi
     return hipGetLastError();
 }
 
-hipError_t diag(const LbmConst &c, const float *in, const int32_t *src, const int32_t *vox, const double *phi,
-                double *partial, hipStream_t st) {
+hipError_t diag(const LbmConst &c, const float *in, const uint8_t *inx, const int32_t *src, const int32_t *vox,
+                const double *phi, double *partial, hipStream_t st) {
     int64_t nb = psx::ceil_div(c.nl, BLOCK);
     if (nb == 0) return hipSuccess;
-    PullF32 ld{in, src, c.nl};
+    PullF32 ld{in, src, c.nl, inx};
     hipLaunchKernelGGL(k_diag<PullF32>, dim3((unsigned)nb), dim3(BLOCK), 0, st, c, ld, vox, phi, partial);
     return hipGetLastError();
 }
 
-hipError_t fields(const LbmConst &c, const float *in, const int32_t *src, const int32_t *vox, const double *phi,
-                  double *out, int32_t *phase, hipStream_t st) {
+hipError_t fields(const LbmConst &c, const float *in, const uint8_t *inx, const int32_t *src, const int32_t *vox,
+                  const double *phi, double *out, int32_t *phase, hipStream_t st) {
     int64_t nb = psx::ceil_div(c.nl, BLOCK);
     if (nb == 0) return hipSuccess;
-    PullF32 ld{in, src, c.nl};
+    PullF32 ld{in, src, c.nl, inx};
     hipLaunchKernelGGL(k_fields<PullF32>, dim3((unsigned)nb), dim3(BLOCK), 0, st, c, ld, vox, phi, out, phase);
     return hipGetLastError();
 }

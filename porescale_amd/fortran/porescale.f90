@@ -11,7 +11,7 @@ module porescale
   integer(c_int), parameter :: PSX_OK = 0, PSX_EINVAL = -1, PSX_EHIP = -2, &
        PSX_ENOMEM = -3, PSX_ESTATE = -4, PSX_ECONV = -5
   integer(c_int32_t), parameter :: PSX_COLL_REF_TRT = 0, PSX_COLL_TRT = 1, PSX_COLL_MRT = 2
-  integer(c_int32_t), parameter :: PSX_PREC_F64 = 0, PSX_PREC_F32 = 1
+  integer(c_int32_t), parameter :: PSX_PREC_F64 = 0, PSX_PREC_F32 = 1, PSX_PREC_F32X = 2
   integer(c_int32_t), parameter :: PSX_XX = 0, PSX_YY = 1, PSX_ZZ = 2, PSX_YZ = 3, &
        PSX_ZX = 4, PSX_XY = 5
 

@@ -24,7 +24,8 @@
 ! Optional lines after line 30 (ignored by the reference, which stops reading
 ! at line 30) select the GPU options:
 !   31  precision  0 = FP64 populations (bit-identical to the reference),
-!                  1 = FP32 storage with FP64 arithmetic
+!                  1 = FP32 storage with FP64 arithmetic, 2 = the same with
+!                  the colour fraction to 33 bits
 !   32  collision  0 = the reference's TRT (collmatrTRT), 1 = TRT with the
 !                  magic parameter of line 33, 2 = MRT (viscous rates from
 !                  vir/vib, free rates 1.19, 1.4, 1.2, 1.4, 1.98)
@@ -115,7 +116,14 @@ program psx_lbm_drv
   p%cr2 = cr2; p%rr0 = rr0; p%rb0 = rb0; p%B = B; p%sig = sig
   p%vir = vir; p%vib = vib; p%wet = wet; p%idir = idir; p%iipore = iipore
   p%ee = ee; p%lsize = lsize; p%bfr = bfr; p%bfb = bfb; p%dt = dt
-  p%precision = merge(PSX_PREC_F32, PSX_PREC_F64, prec == 1)
+  select case (prec)
+  case (1)
+     p%precision = PSX_PREC_F32
+  case (2)
+     p%precision = PSX_PREC_F32X
+  case default
+     p%precision = PSX_PREC_F64
+  end select
   select case (coll)
   case (0)
      p%collision = PSX_COLL_REF_TRT

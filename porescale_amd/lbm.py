@@ -43,7 +43,8 @@ class LBM:
     Parameters follow the reference's INPUT_PARAMETERS (``inp``).  ``ind`` is
     the voxel code array indexed ``[k, j, i]`` (what ``io.read_ind`` returns).
     ``collision``: "ref" (collmatrTRT), ("trt", lam) or ("mrt", Sr, Sb).
-    ``precision``: "f64" (bit-identical to the reference) or "f32".
+    ``precision``: "f64" (bit-identical to the reference), "f32" (FP32
+    deviation + 25-bit colour fraction) or "f32x" (the fraction to 33 bits).
     """
 
     def __init__(self, inp: LBMInput, ind: np.ndarray, collision="ref", precision: str = "f64"):
@@ -67,7 +68,7 @@ class LBM:
             p.Sb[:] = [float(x) for x in collision[2]]
         else:
             raise ValueError(f"unknown collision {collision!r}")
-        p.precision = {"f64": _lib.PREC_F64, "f32": _lib.PREC_F32}[precision]
+        p.precision = {"f64": _lib.PREC_F64, "f32": _lib.PREC_F32, "f32x": _lib.PREC_F32X}[precision]
         self.precision = precision
         self._ind = _lib.i32(ind)
         h = C.c_void_p()

@@ -90,12 +90,13 @@ def test_mrt_pack_fp64_bitexact_vs_oracle():
         assert d[k] == pytest.approx(od[k], rel=1e-11, abs=1e-300), k
 
 
-def test_mrt_pack_fp32_storage_tracks_fp64():
-    """FP32 storage (total deviation + signed minority-colour fraction)
-    against FP64 storage on the 48^3 MRT workload over 2000 steps, and the
-    control: FP64 from the same start with every population scaled by
-    1 + 1e-7 u (u uniform in [-1, 1]), i.e. how far a perturbation of FP32
-    storage size moves this transient by itself.  Bounds: total mass to
+@pytest.mark.parametrize("prec32", ["f32", "f32x"])
+def test_mrt_pack_fp32_storage_tracks_fp64(prec32):
+    """FP32 storage (total deviation + minority-colour fraction to 25 bits,
+    or 33 with f32x) against FP64 storage on the 48^3 MRT workload over 2000
+    steps, and the control: FP64 from the same start with every population
+    scaled by 1 + 1e-7 u (u uniform in [-1, 1]), i.e. how far a perturbation
+    of FP32 storage size moves this transient by itself.  Bounds: total mass to
     1e-10, each colour's mass, the blue saturation and the blue permeability
     to 1e-5 relative; the red permeability no further from FP64 than 1e-5 or
     three times the control's own spread, whichever is larger."""
@@ -105,7 +106,7 @@ def test_mrt_pack_fp32_storage_tracks_fp64():
     pert = (fr0 * (1.0 + 1e-7 * rng.uniform(-1.0, 1.0, fr0.shape)),
             fb0 * (1.0 + 1e-7 * rng.uniform(-1.0, 1.0, fb0.shape)))
     out = {}
-    for name, prec, st in (("f64", "f64", (fr0, fb0)), ("ctl", "f64", pert), ("f32", "f32", (fr0, fb0))):
+    for name, prec, st in (("f64", "f64", (fr0, fb0)), ("ctl", "f64", pert), ("f32", prec32, (fr0, fb0))):
         _say(f"case twophase MRT pack 48^3 2000 steps {name}")
         with LBM(inp, ind, collision=coll, precision=prec) as lb:
             lb.set_state(*st)

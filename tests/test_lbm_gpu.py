@@ -139,10 +139,12 @@ def test_profile_reports_kernel_times():
     assert tot > 0 and tc > 0 and td > 0 and tc + td <= tot * 1.01
 
 
-def test_fp32_state_round_trip_and_first_step():
+@pytest.mark.parametrize("prec32", ["f32", "f32x"])
+def test_fp32_state_round_trip_and_first_step(prec32):
     """FP32 storage holds post-collision populations ahead of the pull; set/get
-    must invert each other to storage precision, and one step must match the
-    FP64 path to storage precision."""
+    must invert each other to storage precision (each population's colour
+    fraction to its word's 25 significand bits, or 33 with the extension
+    byte), and one step must match the FP64 path to storage precision."""
     ind = geo.sphere_pack((14, 12, 10), 0.45, 2.5, seed=13)
     nz, ny, nx = ind.shape
     inp = pio.LBMInput(B=0.7, sig=0.02, vir=0.1, vib=0.2, wet=0.3, ncx=nx, ncy=ny, ncz=nz, dt=1.0,
@@ -151,23 +153,43 @@ def test_fp32_state_round_trip_and_first_step():
     rng = np.random.default_rng(0)
     state = (state[0] * (1 + 1e-3 * rng.standard_normal(state[0].shape)),
              state[1] * (1 + 1e-3 * rng.standard_normal(state[1].shape)))
+    # colours mixed in every population, minority shares from 1/2 down to
+    # 1e-12 of it, a fifth left single-coloured
+    f = state[0] + state[1]
+    r = 0.5 * 10.0 ** (-12.0 * rng.uniform(size=f.shape))
+    r[rng.uniform(size=f.shape) < 0.2] = 0.0
+    blue_minor = rng.uniform(size=f.shape) < 0.5
+    state = (np.where(blue_minor, f - r * f, r * f), np.where(blue_minor, r * f, f - r * f))
     out = {}
-    for prec in ("f32", "f64"):
+    for prec in (prec32, "f64"):
         with LBM(inp, ind, precision=prec) as lb:
             lb.set_state(*state)
             back = lb.get_state()
             lb.step(1)
             out[prec] = (back, lb.get_state())
-    for a, b in zip(out["f32"][0], state):
+    for a, b in zip(out[prec32][0], state):
         np.testing.assert_allclose(a, b, rtol=0, atol=2e-8)
-    for a, b in zip(out["f32"][1], out["f64"][1]):
+    # the minority colour's fraction of every population, whatever T's rounding
+    fr, fb = state
+    br, bb = out[prec32][0]
+    minor_blue = np.abs(fb) < np.abs(fr)
+    frac = np.where(minor_blue, fb, fr) / (fr + fb)
+    frac_back = np.where(minor_blue, bb, br) / (br + bb)
+    nz = frac != 0.0
+    assert np.all(frac_back[~nz] == 0.0)  # single-colour populations stay exact
+    bits = 33 if prec32 == "f32x" else 25
+    assert np.max(np.abs(frac_back[nz] - frac[nz]) / np.abs(frac[nz])) <= 2.0 ** -bits * (1 + 1e-6)
+    for a, b in zip(out[prec32][1], out["f64"][1]):
         np.testing.assert_allclose(a, b, rtol=0, atol=5e-8)
 
 
-def test_fp32_compressed_pull_table_is_exact(monkeypatch):
+@pytest.mark.parametrize("prec32", ["f32", "f32x"])
+def test_fp32_compressed_pull_table_is_exact(monkeypatch, prec32):
     """The int16 pull tables (direct offsets, flag 1; rank differences within
-    x-planes, flag 2) address exactly the populations the int32 table does
-    (flag 0): states bit-identical."""
+    x-planes, flag 2) and the collide kernel's node-major int32 table (taken
+    with flags 0 and 2) address exactly the populations the direction-major
+    int32 table does (flag 0 without the node-major table): states
+    bit-identical."""
     ind = geo.sphere_pack((20, 14, 16), 0.4, 3.0, seed=17)
     nz, ny, nx = ind.shape
     inp = pio.LBMInput(B=0.7, sig=0.02, vir=0.1, vib=0.2, wet=0.3, ncx=nx, ncy=ny, ncz=nz, dt=1.0,
@@ -175,18 +197,20 @@ def test_fp32_compressed_pull_table_is_exact(monkeypatch):
     state = ob.equilibrium_state(inp, ob.fluid_order_mask(ind, geo.phase_blobs(ind, 0.5, 5, radius=2.5)))
     out = {}
     monkeypatch.setenv("PSX_LBM_FUSED", "0")  # the int16 table serves the two-kernel step
-    for flag in ("0", "1", "2"):
+    for flag, aos in (("0", "0"), ("0", "1"), ("1", "1"), ("2", "1")):
         monkeypatch.setenv("PSX_LBM_IDX16", flag)
-        _say(f"case fp32 pull table int16={flag}")
-        with LBM(inp, ind, precision="f32") as lb:
+        monkeypatch.setenv("PSX_LBM_SRCAOS", aos)
+        _say(f"case {prec32} pull table int16={flag} node-major={aos}")
+        with LBM(inp, ind, precision=prec32) as lb:
             lb.set_state(*state)
             lb.step(40)
-            out[flag] = lb.get_state()
-    for flag in ("1", "2"):
-        assert np.array_equal(out["0"][0], out[flag][0]) and np.array_equal(out["0"][1], out[flag][1])
+            out[flag + aos] = lb.get_state()
+    for key in ("01", "11", "21"):
+        assert np.array_equal(out["00"][0], out[key][0]) and np.array_equal(out["00"][1], out[key][1])
 
 
-def test_fp32_wide_pull_offsets_are_exact(monkeypatch):
+@pytest.mark.parametrize("prec32", ["f32", "f32x"])
+def test_fp32_wide_pull_offsets_are_exact(monkeypatch, prec32):
     """The collide kernel's 64-bit pull offsets (taken automatically once the
     population buffer reaches 2^31 floats, about 56 M fluid nodes; forced
     here with PSX_LBM_WIDE=1) give the same bits as the 32-bit ones, with the
@@ -202,8 +226,8 @@ def test_fp32_wide_pull_offsets_are_exact(monkeypatch):
         out = {}
         for wide in ("0", "1"):
             monkeypatch.setenv("PSX_LBM_WIDE", wide)
-            _say(f"case fp32 wide={wide} int16={idx16}")
-            with LBM(inp, ind, precision="f32") as lb:
+            _say(f"case {prec32} wide={wide} int16={idx16}")
+            with LBM(inp, ind, precision=prec32) as lb:
                 lb.set_state(*state)
                 lb.step(25)
                 out[wide] = lb.get_state()
