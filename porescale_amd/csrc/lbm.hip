@@ -69,6 +69,7 @@ struct psx_lbm {
     int16_t *d_src16 = nullptr;  // FP32 path: compressed pull table (null: use d_dst)
     int32_t *d_srca = nullptr;   // FP32 path: node-major int32 pull table [nl][20] for collide
     int32_t *d_pbase = nullptr;  // [3][nx] per-plane bases of d_src16
+    uint32_t *d_etab = nullptr;  // FP32 path: the collide kernel's element table [nl][20] (null: not used)
     void *d_pop[2] = {nullptr, nullptr};
     void *d_phi = nullptr;
     double *d_partial = nullptr, *d_red = nullptr;
@@ -228,8 +229,8 @@ int launch_collide(psx_lbm *h) {
                                    h->d_bricks, h->nbricks, h->d_hnode, h->d_hcell, h->box, h->xcd, h->stream));
     } else if (h->f32) {
         PSX_HIP(psx_lbm_fast::collide(h->c, (const float *)h->d_pop[a], popx(h, a), (float *)h->d_pop[b], popx(h, b),
-                                      h->d_dst, h->d_src16, h->d_pbase, h->d_srca, h->d_vox, (const double *)h->d_phi,
-                                      h->xcd, h->stream));
+                                      h->d_dst, h->d_src16, h->d_pbase, h->d_srca, h->d_etab, h->d_vox,
+                                      (const double *)h->d_phi, h->xcd, h->stream));
     } else {
         StoreF64 si{(double *)h->d_pop[a], h->nl}, so{(double *)h->d_pop[b], h->nl};
         hipLaunchKernelGGL(k_collide<StoreF64>, dim3((unsigned)nb), dim3(256), 0, h->stream, h->c, si, so,
@@ -465,7 +466,8 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
     }
     fill_const(h);
     PSX_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-    size_t popb = h->f32 ? (size_t)aos_pad(nl > 0 ? nl : 1) * Q * (2 * sizeof(float) + (h->f32x ? 1 : 0))
+    size_t popb = h->f32 ? (size_t)aos_pad(nl > 0 ? nl : 1) * 2 * Q * sizeof(float) +
+                               (h->f32x ? (size_t)xbytes(nl > 0 ? nl : 1) : 0)
                          : (size_t)(nl > 0 ? nl : 1) * 2 * Q * sizeof(double);
     PSX_HIP(hipMalloc(&h->d_pop[0], popb));
     PSX_HIP(hipMalloc(&h->d_pop[1], popb));
@@ -481,7 +483,34 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
     // forces it off/on):
     const char *ea = std::getenv("PSX_LBM_SRCAOS");
     const bool node_major = ea ? std::atoi(ea) != 0 : h->f32x;
-    if (h->f32 && !h->fused && nl > 0 && !(use16 && direct)) {
+    // The collide kernel's element table (collide_pull_e): per node the 18
+    // pulled AoSoA element indices with bounce-back resolved, the voxel and
+    // the packed coordinates, so the kernel forms no address and divides
+    // nothing (collide 2-4 % faster at 256^3 than with the int32 source
+    // table).  The density kernel keeps its int16 table: reading this one
+    // instead (80 B a node for 40) made it 29 % slower.  It needs 32-bit
+    // element indices and dimensions below 1025; PSX_LBM_ETAB=0 keeps the
+    // index tables above.
+    const char *ee = std::getenv("PSX_LBM_ETAB");
+    const bool etab = h->f32 && !h->fused && nl > 0 && (!ee || std::atoi(ee) != 0) && !psx_lbm_fast::wide_index(nl) &&
+                      nx <= 1024 && ny <= 1024 && nz <= 1024;
+    if (etab) {
+        std::vector<uint32_t> et(20 * (size_t)nl);
+        for (int64_t n = 0; n < nl; n++) {
+            uint32_t *row = &et[20 * (size_t)n];
+            for (int q = 1; q < Q; q++) {
+                const int32_t sidx = h->dst_h[(size_t)(q - 1) * nl + n];
+                row[q - 1] = (uint32_t)(sidx >= 0 ? aos(q, sidx) : aos(opp(q), n));
+            }
+            const int32_t *ic = &h->icor[3 * n];
+            row[18] = (uint32_t)vox[n];
+            row[19] = (uint32_t)(ic[0] - 1) | (uint32_t)(ic[1] - 1) << 10 | (uint32_t)(ic[2] - 1) << 20;
+        }
+        PSX_HIP(hipMalloc(&h->d_etab, sizeof(uint32_t) * et.size()));
+        PSX_HIP(hipMemcpyAsync(h->d_etab, et.data(), sizeof(uint32_t) * et.size(), hipMemcpyHostToDevice, h->stream));
+        PSX_HIP(hipStreamSynchronize(h->stream));
+    }
+    if (h->f32 && !h->fused && nl > 0 && !(use16 && direct) && !etab) {
         if (node_major) {
             std::vector<int32_t> sa(20 * (size_t)nl, -1);
             for (int q = 1; q < Q; q++)
@@ -533,7 +562,7 @@ extern "C" void psx_lbm_destroy(psx_lbm *h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void *bufs[] = {h->d_pop[0], h->d_pop[1], h->d_dst, h->d_vox, h->d_phi, h->d_partial, h->d_red, h->d_src16,
-                    h->d_pbase, h->d_bricks, h->d_hnode, h->d_hcell, h->d_srca};
+                    h->d_pbase, h->d_bricks, h->d_hnode, h->d_hcell, h->d_srca, h->d_etab};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -624,7 +653,7 @@ extern "C" int psx_lbm_set_state(psx_lbm *h, const double *fr, const double *fb)
         // stored state is post-collision and not yet streamed: place f_q(i)
         // where the pull of node i reads it (inverse of the streaming step)
         std::vector<float> st((size_t)2 * Q * aos_pad(nl), 0.0f);
-        std::vector<uint8_t> sx(h->f32x ? (size_t)Q * aos_pad(nl) : 1, 0);
+        std::vector<uint8_t> sx(h->f32x ? (size_t)xbytes(nl) : 1, 0);
         for (int64_t i = 0; i < nl; i++)
             for (int q = 0; q < Q; q++) {
                 int64_t j = i;
@@ -666,7 +695,7 @@ extern "C" int psx_lbm_get_state(psx_lbm *h, double *fr, double *fb) {
     const int64_t nl = h->nl;
     if (h->f32) {
         std::vector<float> st((size_t)2 * Q * aos_pad(nl));
-        std::vector<uint8_t> sx(h->f32x ? (size_t)Q * aos_pad(nl) : 0);
+        std::vector<uint8_t> sx(h->f32x ? (size_t)xbytes(nl) : 0);
         PSX_HIP(hipMemcpyAsync(st.data(), h->d_pop[h->cur], sizeof(float) * st.size(), hipMemcpyDeviceToHost, h->stream));
         if (h->f32x)
             PSX_HIP(hipMemcpyAsync(sx.data(), popx(h, h->cur), sx.size(), hipMemcpyDeviceToHost, h->stream));

@@ -54,16 +54,18 @@ struct StoreF64 {
 // 0..18 hold T, 19..37 the colour word s (frac_encode).  A node pulls
 // direction q from the node src streams from, or from its own opposite slot
 // for a bounce-back link.  With the extended colour word the extension
-// bytes follow the float blocks, in byte blocks of 19 x 64: element
-// (direction k, node n) at aosx(k, n) bytes past the 2*19*aos_pad(nl) floats.
+// bytes follow the float blocks, indexed like the floats: the byte of
+// (direction k, node n) is byte aosx(k, n) = aos(k, n) past the
+// 2*19*aos_pad(nl) floats (xbytes(nl) bytes, the bytes at colour-word
+// positions unused), so a pull reads T, s and x at one element index.
 constexpr int AOS = 64;
 __host__ __device__ __forceinline__ int64_t aos(int k, int64_t n) {
     return (n >> 6) * (2 * Q * AOS) + (int64_t)k * AOS + (n & (AOS - 1));
 }
-__host__ __device__ __forceinline__ int64_t aosx(int k, int64_t n) {
-    return (n >> 6) * (Q * AOS) + (int64_t)k * AOS + (n & (AOS - 1));
-}
+__host__ __device__ __forceinline__ int64_t aosx(int k, int64_t n) { return aos(k, n); }
 __host__ __device__ __forceinline__ int64_t aos_pad(int64_t nl) { return (nl + AOS - 1) / AOS * AOS; }
+// bytes of extension bytes per population buffer
+__host__ __device__ __forceinline__ int64_t xbytes(int64_t nl) { return 2 * Q * aos_pad(nl); }
 
 struct PullF32 {
     // the colour field stays FP64: calcFs differentiates it, and an FP32 grid
@@ -197,6 +199,18 @@ __device__ __forceinline__ void collide_node(double t1, double t2, double t3, do
     for (int q = 1; q < Q; q++) pcf[q] = f[q] - mb[q] + Fp[q];
 }
 
+// sqrt(x) for x >= 0: EXACT the correctly rounded one; else the hardware
+// reciprocal square root with one coupled Newton step (relative error near
+// 2^-50), exact 0 at 0
+template <bool EXACT>
+__device__ __forceinline__ double fs_sqrt(double x) {
+    if (EXACT) return sqrt(x);
+    const double rs = __builtin_amdgcn_rsq(x);
+    const double y = x * rs, h = 0.5 * rs;
+    const double y1 = fma(y, fma(-y, h, 0.5), y);
+    return x > 0.0 ? y1 : 0.0;
+}
+
 // calcFs (L1:2626-2796) on the 3x3x3 window of the colour field around a
 // voxel; phi3(i, j, k) returns the value at offset (i-1, j-1, k-1).
 template <bool EXACT, class Phi3>
@@ -244,24 +258,27 @@ __device__ __forceinline__ void calc_fs_window(const LbmConst &c, const Phi3 &ph
 #pragma unroll
     for (int d = 0; d < 3; d++)
         Cg[d] = nm[0][d] + nm[1][d] + nm[2][d] + nm[3][d] + nm[4][d] + nm[5][d] + nm[6][d] + nm[7][d];
-    double normc = sqrt(Cg[0] * Cg[0] + Cg[1] * Cg[1] + Cg[2] * Cg[2]);
+    const double norm2 = Cg[0] * Cg[0] + Cg[1] * Cg[1] + Cg[2] * Cg[2];
     double dn0 = nm[1][0] + nm[3][0] + nm[5][0] + nm[7][0] - nm[0][0] - nm[2][0] - nm[4][0] - nm[6][0];
     double dn1 = nm[3][1] + nm[7][1] + nm[2][1] + nm[6][1] - nm[0][1] - nm[1][1] - nm[4][1] - nm[5][1];
     double dn2 = nm[4][2] + nm[5][2] + nm[6][2] + nm[7][2] - nm[0][2] - nm[1][2] - nm[2][2] - nm[3][2];
     double k1 = dn0 + dn1 + dn2;
     double mm[8];
 #pragma unroll
-    for (int i = 0; i < 8; i++) mm[i] = sqrt(nm[i][0] * nm[i][0] + nm[i][1] * nm[i][1] + nm[i][2] * nm[i][2]);
+    for (int i = 0; i < 8; i++) mm[i] = fs_sqrt<EXACT>(nm[i][0] * nm[i][0] + nm[i][1] * nm[i][1] + nm[i][2] * nm[i][2]);
     double dm0 = mm[1] + mm[3] + mm[5] + mm[7] - mm[0] - mm[2] - mm[4] - mm[6];
     double dm1 = mm[3] + mm[7] + mm[2] + mm[6] - mm[0] - mm[1] - mm[4] - mm[5];
     double dm2 = mm[4] + mm[5] + mm[6] + mm[7] - mm[0] - mm[1] - mm[2] - mm[3];
     double k2 = Cg[0] * dm0 + Cg[1] * dm1 + Cg[2] * dm2;
-    if (normc != 0.0) {
+    if (norm2 != 0.0) {
         if (EXACT) {
+            const double normc = sqrt(norm2);
             k2 = k2 / normc;
             Cg[0] = Cg[0] / normc; Cg[1] = Cg[1] / normc; Cg[2] = Cg[2] / normc;
-        } else {
-            double in = 1.0 / normc;
+        } else {  // 1 / |Cg| from the reciprocal square root and two Newton steps
+            double in = __builtin_amdgcn_rsq(norm2);
+            in = in * fma(-0.5 * norm2 * in, in, 1.5);
+            in = in * fma(-0.5 * norm2 * in, in, 1.5);
             k2 *= in;
             Cg[0] *= in; Cg[1] *= in; Cg[2] *= in;
         }

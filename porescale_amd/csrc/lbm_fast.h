@@ -18,13 +18,15 @@ struct Brick {
 // 25-bit words).  The pull table is src16 (int16 residuals, -32768 =
 // bounce-back) with the per-plane bases pbase[3][nx] when src16 is not null,
 // else srca [nl][20] (node-major, two pad entries) when not null, else src
-// [18][nl] (-1 = bounce-back).
+// [18][nl] (-1 = bounce-back); etab, when not null, replaces them all:
+// [nl][20], the AoSoA element index of every pull (bounce-back resolved),
+// the node's voxel and its coordinates x | y << 10 | z << 20.
 // true when the collide kernel needs 64-bit pull offsets (the population
 // buffer of nl nodes holds 2^31 floats or more; PSX_LBM_WIDE=1 forces it)
 bool wide_index(int64_t nl);
 hipError_t collide(const psx_lbm_dev::LbmConst &c, const float *in, const uint8_t *inx, float *out, uint8_t *outx,
                    const int32_t *src, const int16_t *src16, const int32_t *pbase, const int32_t *srca,
-                   const int32_t *vox, const double *phi, int xcd, hipStream_t st);
+                   const uint32_t *etab, const int32_t *vox, const double *phi, int xcd, hipStream_t st);
 // Pulled densities -> colour field phi at every node's voxel.
 hipError_t density(const psx_lbm_dev::LbmConst &c, const float *in, const uint8_t *inx, const int32_t *src,
                    const int16_t *src16, const int32_t *pbase, const int32_t *vox, double *phi, int xcd,

@@ -42,7 +42,14 @@ struct PullIdx {
     const int16_t *__restrict__ src16;
     const int32_t *__restrict__ pbase;  // [3][nx]
     const int32_t *__restrict__ srca = nullptr;  // [nl][20] (M = 3)
+    // element table [nl][20] of the collide kernel (collide_pull_e), or null
+    const uint32_t *__restrict__ etab = nullptr;
 };
+
+// the 256-node block that workgroup blockIdx.x works on
+__device__ __forceinline__ int64_t work_block(int xcd) {
+    return xcd ? psx::xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+}
 
 template <int M>
 __device__ __forceinline__ void load_bases(const LbmConst &c, const PullIdx &P, int32_t v, int32_t (&pb)[3]) {
@@ -74,23 +81,48 @@ __device__ __forceinline__ void pull_from(const PullIdx &P, const int32_t (&pb)[
     }
 }
 
-// X: the colour word's extension byte goes to outx (33-bit fraction)
+// 1 / x for a normal, finite x: the hardware reciprocal and two Newton
+// steps (error below 2^-100 before the final rounding)
+__device__ __forceinline__ double frcp(double x) {
+    double rc = __builtin_amdgcn_rcp(x);
+    rc = fma(rc, fma(-x, rc, 1.0), rc);
+    return fma(rc, fma(-x, rc, 1.0), rc);
+}
+
+// Stores the population (fr, fb) of direction q of node j: T and the colour
+// word, and for X the word's extension byte (33-bit fraction).  The
+// minority's fraction m / f is the hardware reciprocal, one Newton step and
+// one residual correction: within an ulp of the correctly rounded quotient
+// (which costs twice the instructions), so the word's rounding to 25 or 33
+// bits is unbiased as before; f near zero or denormal takes the division.
 template <bool X>
 __device__ __forceinline__ void put(const LbmConst &c, float *__restrict__ out, uint8_t *__restrict__ outx, int q,
                                     int64_t j, double fr, double fb) {
-    out[aos(q, j)] = (float)((fr + fb) - c.w0[q]);
+    const double f = fr + fb;
+    out[aos(q, j)] = (float)(f - c.w0[q]);
+    const bool blue = fabs(fb) < fabs(fr);
+    const double mn = blue ? fb : fr;
+    double r;
+    if (fabs(f) >= 0x1p-1000) {
+        double rc = __builtin_amdgcn_rcp(f);
+        rc = fma(rc, fma(-f, rc, 1.0), rc);
+        const double q0 = mn * rc;
+        r = fma(fma(-q0, f, mn), rc, q0);
+    } else {
+        r = f == 0.0 ? 0.5 : mn / f;
+    }
     uint32_t x;
-    const float s = frac_encode_t<X ? 8 : 0>(fr, fb, x);
-    out[aos(Q + q, j)] = s;
+    const uint32_t w = frac_word_t<X ? 8 : 0>(r, x) | (blue ? 1u : 0u);
+    out[aos(Q + q, j)] = __uint_as_float(w);
     if (X) outx[aosx(q, j)] = (uint8_t)x;
 }
 
 // The collision of L1:2407-2620 with the relaxation vector selected per lane
 // (red: Sr, else Sb) and the force projection Fp_q = rk_q (c_q.bf) dt^2
 // formed in place, updated in place: f[0] becomes the post-collision rest
-// population and f[1..18] the post-collision populations.
+// population and f[1..18] the post-collision populations.  dr1 = 1 / r.
 __device__ __forceinline__ void collide_fast(const LbmConst &c, double t1, double t2, double t3, double (&f)[Q],
-                                             double r, const double (&bf)[3], bool red) {
+                                             double r, double dr1, const double (&bf)[3], bool red) {
     const double dt = c.dt, dti = c.dti, dt2 = c.dt2;
     const double c1d6 = 1.0 / 6.0, c1d12 = 1.0 / 12.0, c5d399 = 5.0 / 399.0,
                  c19d399 = 19.0 / 399.0, c11d2394 = 11.0 / 2394.0, c1d63 = 1.0 / 63.0,
@@ -99,7 +131,6 @@ __device__ __forceinline__ void collide_fast(const LbmConst &c, double t1, doubl
     double j0 = (f[1] - f[2] + f[7] - f[8] + f[9] - f[10] + f[11] - f[12] + f[13] - f[14]) * dti;
     double j1 = (f[3] - f[4] + f[7] + f[8] - f[9] - f[10] + f[15] - f[16] + f[17] - f[18]) * dti;
     double j2 = (f[5] - f[6] + f[11] + f[12] - f[13] - f[14] + f[15] + f[16] - f[17] - f[18]) * dti;
-    double dr1 = 1.0 / r;
     double u0 = (j0 + bf[0] * 0.5 * dt) * dr1;
     double u1 = (j1 + bf[1] * 0.5 * dt) * dr1;
     double u2v = (j2 + bf[2] * 0.5 * dt) * dr1;
@@ -195,6 +226,76 @@ __device__ __forceinline__ void collide_fast(const LbmConst &c, double t1, doubl
     f[17] = f[17] - mb17 - F12 * (-bf[1] + bf[2]);
 }
 
+// The node's collision, recolouring and store after its pull: f[q] holds the
+// pulled totals and rr, rb the colour densities.  resplit(blue) replaces
+// every f[q] by its red (false) or blue (true) part, for the single-colour
+// branches; fs(Cg, Fs) evaluates calcFs at the node's voxel.
+template <bool X, class Resplit, class CalcFs>
+__device__ __forceinline__ void collide_node_f32(const LbmConst &c, double (&f)[Q], double rr, double rb,
+                                                 float *__restrict__ out, uint8_t *__restrict__ outx, int64_t i,
+                                                 const Resplit &resplit, const CalcFs &fs) {
+    const double dt2 = c.dt2;
+    double bf[3];
+    double t1, t2, t3, r, rinv;
+    int mode;  // 0 pure red, 1 pure blue, 2 interface (L1:3701-3721)
+    bool red;
+    double Cg[3] = {0.0, 0.0, 0.0};
+    if (rb <= c.ee) {
+        mode = 0; red = true;
+        t1 = c.tr[0]; t2 = c.tr[1]; t3 = c.tr[7]; r = rr;
+        resplit(false);
+        bf[0] = c.bfr[0]; bf[1] = c.bfr[1]; bf[2] = c.bfr[2];
+        rinv = frcp(r);
+    } else if (rr <= c.ee) {
+        mode = 1; red = false;
+        t1 = c.tb[0]; t2 = c.tb[1]; t3 = c.tb[7]; r = rb;
+        resplit(true);
+        bf[0] = c.bfb[0]; bf[1] = c.bfb[1]; bf[2] = c.bfb[2];
+        rinv = frcp(r);
+    } else {
+        mode = 2;
+        double Fs[3];
+        fs(Cg, Fs);
+        r = rr + rb;
+        rinv = frcp(r);
+        double ci2 = (c.cr2 * rr + c.cb2 * rb) * rinv;
+        t1 = 1.0 - 2.0 * ci2 * dt2;
+        t2 = ci2 * (1.0 / 6.0) * dt2;
+        t3 = ci2 * (1.0 / 12.0) * dt2;
+        red = rr >= rb;
+#pragma unroll
+        for (int d = 0; d < 3; d++) bf[d] = (red ? c.bfr[d] : c.bfb[d]) + Fs[d];
+    }
+    collide_fast(c, t1, t2, t3, f, r, rinv, bf, red);
+    double ar = 0.0, ab = 0.0;
+    if (mode == 2) {
+        ar = rr * rinv;
+        ab = rb * rinv;
+    }
+    double o_r, o_b;
+    if (mode == 0) { o_r = f[0]; o_b = c.tb[0] * rb; }
+    else if (mode == 1) { o_b = f[0]; o_r = c.tr[0] * rr; }
+    else { o_r = f[0] * ar; o_b = f[0] * ab; }
+    put<X>(c, out, outx, 0, i, o_r, o_b);
+    const double cn = 0.70710678118654752440;
+    const double t2B = t2 * c.B, t3B = t3 * c.B;
+#pragma unroll
+    for (int q = 1; q < Q; q++) {
+        if (mode == 0) { o_r = f[q]; o_b = c.tb[q] * rb; }
+        else if (mode == 1) { o_b = f[q]; o_r = c.tr[q] * rr; }
+        else {
+            // cos of the angle between the colour gradient and c_q (L1:3786-3803)
+            const int cx = H_IV[q][0], cy = H_IV[q][1], cz = H_IV[q][2];
+            double cs = cx * Cg[0] + cy * Cg[1] + cz * Cg[2];
+            if (q >= 7) cs *= cn;
+            double tw = (q < 7 ? t2B : t3B) * cs;
+            o_r = (f[q] + tw * rb) * ar;
+            o_b = (f[q] - tw * rr) * ab;
+        }
+        put<X>(c, out, outx, q, i, o_r, o_b);
+    }
+}
+
 // Pull + collide + recolour of node i (voxel v) into its own slots of out.
 // fs(x, y, z, Cg, Fs) evaluates calcFs at the node's voxel.  X: extended
 // colour words, their bytes in inx / outx.
@@ -240,7 +341,7 @@ __device__ __forceinline__ void collide_pull(const LbmConst &c, const float *__r
         at[q] = (decltype(at[0] + 0))aos(k, j);
         double t = c.w0[q] + (double)in[at[q]];
         float s = in[at[q] + Q * AOS];
-        xs[q] = X ? (uint32_t)inx[aosx(k, j)] : 0u;
+        xs[q] = X ? (uint32_t)inx[at[q]] : 0u;
         double a, b;
         frac_split_fast(t, s, xs[q], a, b, esc);
         f[q] = t;
@@ -259,84 +360,98 @@ __device__ __forceinline__ void collide_pull(const LbmConst &c, const float *__r
             rb += b;
         }
     }
-    const double dt2 = c.dt2;
-    double bf[3];
-    double t1, t2, t3, r;
-    int mode;  // 0 pure red, 1 pure blue, 2 interface (L1:3701-3721)
-    bool red;
-    double Cg[3] = {0.0, 0.0, 0.0};
-    if (rb <= c.ee) {
-        mode = 0; red = true;
-        t1 = c.tr[0]; t2 = c.tr[1]; t3 = c.tr[7]; r = rr;
-        if (!allred) {  // the colour split of a node that is not single-coloured
+    auto resplit = [&](bool blue) {  // the colour split of a node that is not single-coloured
+        if (blue ? allblue : allred) return;
 #pragma unroll
-            for (int q = 0; q < Q; q++) {
-                double a, b;
-                frac_split(f[q], in[at[q] + Q * AOS], a, b, xs[q]);
-                f[q] = a;
-            }
+        for (int q = 0; q < Q; q++) {
+            double a, b;
+            frac_split(f[q], in[at[q] + Q * AOS], a, b, xs[q]);
+            f[q] = blue ? b : a;
         }
-        bf[0] = c.bfr[0]; bf[1] = c.bfr[1]; bf[2] = c.bfr[2];
-    } else if (rr <= c.ee) {
-        mode = 1; red = false;
-        t1 = c.tb[0]; t2 = c.tb[1]; t3 = c.tb[7]; r = rb;
-        if (!allblue) {
-#pragma unroll
-            for (int q = 0; q < Q; q++) {
-                double a, b;
-                frac_split(f[q], in[at[q] + Q * AOS], a, b, xs[q]);
-                f[q] = b;
-            }
-        }
-        bf[0] = c.bfb[0]; bf[1] = c.bfb[1]; bf[2] = c.bfb[2];
-    } else {
-        mode = 2;
+    };
+    auto fsv = [&](double *Cg, double *Fs) {
         int x, y, z;
         vox_xyz(c, v, x, y, z);
-        double Fs[3];
         fs(x, y, z, Cg, Fs);
-        r = rr + rb;
-        double ci2 = (c.cr2 * rr + c.cb2 * rb) / r;
-        t1 = 1.0 - 2.0 * ci2 * dt2;
-        t2 = ci2 * (1.0 / 6.0) * dt2;
-        t3 = ci2 * (1.0 / 12.0) * dt2;
-        red = rr >= rb;
+    };
+    collide_node_f32<X>(c, f, rr, rb, out, outx, i, resplit, fsv);
+}
+
+// calcFs at voxel v = z + nz (y + ny x) of the periodic FP64 colour grid,
+// the neighbours' element offsets formed from (x, y, z) without divisions.
+__device__ __forceinline__ void calc_fs_xyz(const LbmConst &c, const double *__restrict__ phi, uint32_t v, int x,
+                                           int y, int z, double *Cg, double *Fs) {
+    const int nz = c.nz, nyz = c.ny * c.nz;
+    const int ox[3] = {x == 0 ? (c.nx - 1) * nyz : -nyz, 0, x == c.nx - 1 ? -(c.nx - 1) * nyz : nyz};
+    const int oy[3] = {y == 0 ? (c.ny - 1) * nz : -nz, 0, y == c.ny - 1 ? -(c.ny - 1) * nz : nz};
+    const int oz[3] = {z == 0 ? nz - 1 : -1, 0, z == nz - 1 ? -(nz - 1) : 1};
+    auto phi3 = [&](int a, int b2, int e) -> double { return phi[v + (uint32_t)(ox[a] + oy[b2] + oz[e])]; };
+    calc_fs_window<false>(c, phi3, Cg, Fs);
+}
+
+// Pull + collide with the element table: etab[20 i + q - 1] (q = 1..18) is
+// the AoSoA element index node i pulls direction q from (the opposite slot
+// of its own node for a bounce-back link), etab[20 i + 18] its voxel and
+// etab[20 i + 19] its coordinates x | y << 10 | z << 20.  Five 16-byte loads
+// give every address; the rare re-splits load the table row again rather
+// than hold it in registers.
+template <bool X>
+__device__ __forceinline__ void collide_pull_e(const LbmConst &c, const float *__restrict__ in,
+                                               const uint8_t *__restrict__ inx, float *__restrict__ out,
+                                               uint8_t *__restrict__ outx, const uint32_t *__restrict__ etab,
+                                               const double *__restrict__ phi, int64_t i) {
+    uint32_t e[20];
+    const uint4 *p4 = reinterpret_cast<const uint4 *>(etab + 20 * i);
 #pragma unroll
-        for (int d = 0; d < 3; d++) bf[d] = (red ? c.bfr[d] : c.bfb[d]) + Fs[d];
+    for (int g = 0; g < 5; g++) {
+        const uint4 w = p4[g];
+        e[4 * g] = w.x; e[4 * g + 1] = w.y; e[4 * g + 2] = w.z; e[4 * g + 3] = w.w;
     }
-    collide_fast(c, t1, t2, t3, f, r, bf, red);
-    double ar = 0.0, ab = 0.0;
-    if (mode == 2) {
-        double inv = 1.0 / r;
-        ar = rr * inv;
-        ab = rb * inv;
-    }
-    double o_r, o_b;
-    if (mode == 0) { o_r = f[0]; o_b = c.tb[0] * rb; }
-    else if (mode == 1) { o_b = f[0]; o_r = c.tr[0] * rr; }
-    else { o_r = f[0] * ar; o_b = f[0] * ab; }
-    put<X>(c, out, outx, 0, i, o_r, o_b);
-    const double cn = 0.70710678118654752440;
+    const uint32_t self = (uint32_t)aos(0, i);
+    double f[Q];
+    double rr = 0.0, rb = 0.0;
+    bool esc = false;
 #pragma unroll
-    for (int q = 1; q < Q; q++) {
-        if (mode == 0) { o_r = f[q]; o_b = c.tb[q] * rb; }
-        else if (mode == 1) { o_b = f[q]; o_r = c.tr[q] * rr; }
-        else {
-            // cos of the angle between the colour gradient and c_q (L1:3786-3803)
-            const int cx = q == 1 ? 1 : q == 2 ? -1 : q == 7 || q == 9 || q == 11 || q == 13 ? 1
-                         : q == 8 || q == 10 || q == 12 || q == 14 ? -1 : 0;
-            const int cy = q == 3 ? 1 : q == 4 ? -1 : q == 7 || q == 8 || q == 15 || q == 17 ? 1
-                         : q == 9 || q == 10 || q == 16 || q == 18 ? -1 : 0;
-            const int cz = q == 5 ? 1 : q == 6 ? -1 : q == 11 || q == 12 || q == 15 || q == 16 ? 1
-                         : q == 13 || q == 14 || q == 17 || q == 18 ? -1 : 0;
-            double cs = cx * Cg[0] + cy * Cg[1] + cz * Cg[2];
-            if (q >= 7) cs *= cn;
-            double tw = (q < 7 ? t2 : t3) * c.B * cs;
-            o_r = (f[q] + tw * rb) * ar;
-            o_b = (f[q] - tw * rr) * ab;
+    for (int q = 0; q < Q; q++) {
+        const uint32_t at = q == 0 ? self : e[q - 1];
+        double t = c.w0[q] + (double)in[at];
+        const float s = in[at + Q * AOS];
+        const uint32_t xb = X ? (uint32_t)inx[at] : 0u;
+        double a, b;
+        frac_split_fast(t, s, xb, a, b, esc);
+        f[q] = t;
+        rr += a;
+        rb += b;
+    }
+    // the table row again, for the rare branches below
+    auto at_of = [&](int q) -> uint32_t {
+        return q == 0 ? self : __builtin_nontemporal_load(etab + 20 * i + q - 1);
+    };
+    if (esc) {  // a fraction outside [0, 1) (parts of opposite sign): rare
+        rr = 0.0;
+        rb = 0.0;
+        for (int q = 0; q < Q; q++) {
+            const uint32_t at = at_of(q);
+            double a, b;
+            frac_split(f[q], in[at + Q * AOS], a, b, X ? (uint32_t)inx[at] : 0u);
+            rr += a;
+            rb += b;
         }
-        put<X>(c, out, outx, q, i, o_r, o_b);
     }
+    auto resplit = [&](bool blue) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const uint32_t at = at_of(q);
+            double a, b;
+            frac_split(f[q], in[at + Q * AOS], a, b, X ? (uint32_t)inx[at] : 0u);
+            f[q] = blue ? b : a;
+        }
+    };
+    const uint32_t v = e[18], pxyz = e[19];
+    auto fsv = [&](double *Cg, double *Fs) {
+        calc_fs_xyz(c, phi, v, (int)(pxyz & 1023u), (int)((pxyz >> 10) & 1023u), (int)(pxyz >> 20), Cg, Fs);
+    };
+    collide_node_f32<X>(c, f, rr, rb, out, outx, i, resplit, fsv);
 }
 
 // Colour field of node i from its pulled populations.
@@ -368,11 +483,20 @@ __global__ __launch_bounds__(BLOCK) void k_collide_f32(const LbmConst c, const f
                                                        uint8_t *__restrict__ outx, const PullIdx P,
                                                        const int32_t *__restrict__ vox,
                                                        const double *__restrict__ phi, int xcd) {
-    int64_t b = xcd ? psx::xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
-    int64_t i = b * blockDim.x + threadIdx.x;
+    int64_t i = work_block(xcd) * blockDim.x + threadIdx.x;
     if (i >= c.nl) return;
     auto fs = [&](int x, int y, int z, double *Cg, double *Fs) { calc_fs<double, false>(c, phi, x, y, z, Cg, Fs); };
     collide_pull<M, decltype(fs), WIDE, X>(c, in, inx, out, outx, P, i, vox[i], fs);
+}
+
+template <bool X>
+__global__ __launch_bounds__(BLOCK) void k_collide_f32e(const LbmConst c, const float *__restrict__ in,
+                                                        const uint8_t *__restrict__ inx, float *__restrict__ out,
+                                                        uint8_t *__restrict__ outx, const PullIdx P,
+                                                        const double *__restrict__ phi, int xcd) {
+    int64_t i = work_block(xcd) * blockDim.x + threadIdx.x;
+    if (i >= c.nl) return;
+    collide_pull_e<X>(c, in, inx, out, outx, P.etab, phi, i);
 }
 
 template <int M, bool X>
@@ -380,8 +504,7 @@ __global__ __launch_bounds__(BLOCK) void k_density_f32(const LbmConst c, const f
                                                        const uint8_t *__restrict__ inx, const PullIdx P,
                                                        const int32_t *__restrict__ vox, double *__restrict__ phi,
                                                        int xcd) {
-    int64_t b = xcd ? psx::xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
-    int64_t i = b * blockDim.x + threadIdx.x;
+    int64_t i = work_block(xcd) * blockDim.x + threadIdx.x;
     if (i >= c.nl) return;
     const int32_t v = vox[i];
     phi[v] = colour_pull<M, X>(c, in, inx, P, i, v);
@@ -456,9 +579,20 @@ void launch_density(const LbmConst &c, const float *in, const uint8_t *inx, cons
 
 hipError_t collide(const LbmConst &c, const float *in, const uint8_t *inx, float *out, uint8_t *outx,
                    const int32_t *src, const int16_t *src16, const int32_t *pbase, const int32_t *srca,
-                   const int32_t *vox, const double *phi, int xcd, hipStream_t st) {
+                   const uint32_t *etab, const int32_t *vox, const double *phi, int xcd, hipStream_t st) {
     int64_t nb = psx::ceil_div(c.nl, BLOCK);
     if (nb == 0) return hipSuccess;
+    if (etab) {  // the element table: every address precomputed (see collide_pull_e)
+        PullIdx P{src, nullptr, nullptr, nullptr};
+        P.etab = etab;
+        if (inx)
+            hipLaunchKernelGGL((k_collide_f32e<true>), dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, inx, out, outx,
+                               P, phi, xcd);
+        else
+            hipLaunchKernelGGL((k_collide_f32e<false>), dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, inx, out,
+                               outx, P, phi, xcd);
+        return hipGetLastError();
+    }
     // the collide kernel is latency-bound (occupancy 3): it takes the direct
     // int16 offsets when they exist, else the node-major int32 table (five
     // loads a node, no dependent load of a plane base), else the
@@ -472,7 +606,7 @@ hipError_t collide(const LbmConst &c, const float *in, const uint8_t *inx, float
         if (inx) launch_collide<3, true>(c, in, inx, out, outx, P, vox, phi, xcd, st, nb);
         else launch_collide<3, false>(c, in, inx, out, outx, P, vox, phi, xcd, st, nb);
     } else {
-        const PullIdx P{src, nullptr, nullptr};
+        const PullIdx P{src, nullptr, nullptr, nullptr};
         if (inx) launch_collide<0, true>(c, in, inx, out, outx, P, vox, phi, xcd, st, nb);
         else launch_collide<0, false>(c, in, inx, out, outx, P, vox, phi, xcd, st, nb);
     }

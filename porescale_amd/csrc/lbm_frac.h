@@ -55,8 +55,10 @@ __host__ __device__ inline uint32_t frac_word_t(double r, uint32_t &x) {
     const uint32_t wn = (hi << 5) - (896u << 25) + ((lo >> 27) & 0x1Eu);
     const bool normal = hi - (897u << 20) < (126u << 20);  // 2^-126 <= r (rounded) < 1
     const bool small = hi < (897u << 20);                  // 0 <= r < 2^-126
-    const uint64_t n = (uint64_t)std::rint(r * frac_pow2(150 + XB));
-    const uint32_t wd = (uint32_t)(n >> XB) << 1;
+    // r 2^(150+XB) < 2^32 wherever small holds; the clamp keeps the
+    // conversion defined elsewhere (the value is then unused)
+    const uint32_t n = (uint32_t)std::fmin(std::fmax(std::rint(r * frac_pow2(150 + XB)), 0.0), 4294967295.0);
+    const uint32_t wd = (n >> XB) << 1;
     const float s = (float)r;
     uint32_t b;
     std::memcpy(&b, &s, 4);
@@ -64,7 +66,7 @@ __host__ __device__ inline uint32_t frac_word_t(double r, uint32_t &x) {
     const uint32_t wx = (127u << 25) | ((b >> 7) & ~1u);
     if (XB) {
         const uint32_t xn = (lo >> (28 - XB)) & ((1u << XB) - 1u);
-        const uint32_t xd = (uint32_t)n & ((1u << XB) - 1u);
+        const uint32_t xd = n & ((1u << XB) - 1u);
         x = normal ? xn : (small ? xd : 0u);
     } else {
         x = 0u;
