@@ -31,6 +31,7 @@
 // volume; a launch covers every tile of every volume and every active
 // system (volume, load case), so one launch serves the whole batch.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -863,31 +864,36 @@ int autotune_tiles(Batch &B, hipStream_t st) {
 }
 
 // Per-voxel spring constants k_a = alpha_c s_v, k_b = beta_c s_v of the
-// batch's components (c = 1: code 0, c = 2: code 3; pore voxels 0) from
-// (alf1, bet1, alf2, bet2), uploaded in FP64 and FP32 with the inverse
-// diagonal blocks of both block-Jacobi preconditioners.
+// batch's components (c = 1: code 0, c = 2: code 3; pore voxels 0), in
+// FP64 and FP32, from the component codes and stiffness on the device.
+__global__ __launch_bounds__(256) void k_springs(const uint8_t *__restrict__ comp, const double *__restrict__ stiff,
+                                                 int64_t n, double a1, double b1, double a2, double b2,
+                                                 double *__restrict__ ka, double *__restrict__ kb,
+                                                 float *__restrict__ ka32, float *__restrict__ kb32) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double s = stiff ? stiff[i] : 1.0;
+    const int c = comp[i];
+    const double a = c == 1 ? s * a1 : c == 2 ? s * a2 : 0.0;
+    const double b = c == 1 ? s * b1 : c == 2 ? s * b2 : 0.0;
+    ka[i] = a; kb[i] = b;
+    ka32[i] = (float)a; kb32[i] = (float)b;
+}
+
+// The spring constants of (alf1, bet1, alf2, bet2) and the inverse diagonal
+// blocks of both block-Jacobi preconditioners.
 int batch_springs(Batch &B, hipStream_t st, const psx_lsm_params &p) {
     const int64_t off = B.Ntot;
-    std::vector<float> ka32(off), kb32(off);
-    for (int64_t i = 0; i < off; i++) {
-        const double s = B.stiff_h[i];
-        const int c = B.comp_h[i];
-        B.ka_h[i] = c == 1 ? s * p.alf1 : c == 2 ? s * p.alf2 : 0.0;
-        B.kb_h[i] = c == 1 ? s * p.bet1 : c == 2 ? s * p.bet2 : 0.0;
-        ka32[i] = (float)B.ka_h[i];
-        kb32[i] = (float)B.kb_h[i];
-    }
-    PSX_HIP(hipMemcpyAsync(B.d_ka, B.ka_h.data(), sizeof(double) * off, hipMemcpyHostToDevice, st));
-    PSX_HIP(hipMemcpyAsync(B.d_kb, B.kb_h.data(), sizeof(double) * off, hipMemcpyHostToDevice, st));
-    PSX_HIP(hipMemcpyAsync(B.d_ka32, ka32.data(), sizeof(float) * off, hipMemcpyHostToDevice, st));
-    PSX_HIP(hipMemcpyAsync(B.d_kb32, kb32.data(), sizeof(float) * off, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_springs, dim3((unsigned)psx::ceil_div(off, 256)), dim3(256), 0, st, (const uint8_t *)B.d_comp,
+                       (const double *)B.d_stiff, off, p.alf1, p.bet1, p.alf2, p.bet2, B.d_ka, B.d_kb, B.d_ka32,
+                       B.d_kb32);
     int g = grid_for(B.ntiles);
     hipLaunchKernelGGL(k_diag_inv<double>, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles,
                        B.d_ka, B.d_kb, B.d_minv, B.Ntot);
     hipLaunchKernelGGL(k_diag_inv<float>, dim3(g), dim3(TILE), 0, st, B.d_vols, B.d_tvol, B.d_tstart, B.ntiles,
                        B.d_ka, B.d_kb, B.d_minv32, B.Ntot);
     PSX_HIP(hipGetLastError());
-    PSX_HIP(hipStreamSynchronize(st));  // the host copies above are the caller's stack
+    PSX_HIP(hipStreamSynchronize(st));
     return PSX_OK;
 }
 
@@ -915,14 +921,18 @@ int batch_build(Batch &B, hipStream_t st, const psx_lsm_params &p, int nvol, con
     }
     B.Ntot = off;
     B.ntiles = tile;
-    B.ka_h.assign(off, 0.0);
-    B.kb_h.assign(off, 0.0);
-    B.comp_h.assign(off, 0);
-    B.stiff_h.assign(off, 1.0);
+    B.comp_h.resize(off);
     for (int64_t i = 0; i < off; i++) {
         const int32_t c = ind[i];
-        B.stiff_h[i] = stiff ? stiff[i] : 1.0;
         B.comp_h[i] = c == 0 ? 1 : c == 3 ? 2 : 0;
+    }
+    if (stiff) B.stiff_h.assign(stiff, stiff + off);
+    else B.stiff_h.clear();
+    RC(dalloc(&B.d_comp, off));
+    PSX_HIP(hipMemcpyAsync(B.d_comp, B.comp_h.data(), off, hipMemcpyHostToDevice, st));
+    if (stiff) {
+        RC(dalloc(&B.d_stiff, off));
+        PSX_HIP(hipMemcpyAsync(B.d_stiff, B.stiff_h.data(), sizeof(double) * off, hipMemcpyHostToDevice, st));
     }
     RC(dalloc(&B.d_vols, B.vols.size()));
     RC(dalloc(&B.d_tvol, B.tvol.size()));
@@ -939,7 +949,8 @@ int batch_build(Batch &B, hipStream_t st, const psx_lsm_params &p, int nvol, con
 }
 
 void batch_free(Batch &B) {
-    void *bufs[] = {B.d_vols, B.d_tvol, B.d_tstart, B.d_ka, B.d_kb, B.d_ka32, B.d_kb32, B.d_minv, B.d_minv32};
+    void *bufs[] = {B.d_vols, B.d_tvol, B.d_tstart, B.d_ka, B.d_kb, B.d_ka32, B.d_kb32, B.d_minv, B.d_minv32,
+                    B.d_comp, B.d_stiff};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     B = Batch();
@@ -1935,7 +1946,7 @@ extern "C" int psx_lsm_adjoint(psx_lsm *h, double wK, double wG, double rtol, in
     double g4[4] = {0, 0, 0, 0};
     for (int64_t i = 0; i < N; i++) {
         int c = h->B.comp_h[i];
-        double s = h->B.stiff_h[i];
+        double s = h->B.stiff_at(i);
         if (c == 1) {
             g4[0] += s * tA[i]; g4[1] += s * tB[i];
             if (dstiff) dstiff[i] = h->p.alf1 * tA[i] + h->p.bet1 * tB[i];
@@ -2011,10 +2022,22 @@ extern "C" int psx_lsm_batch_moduli(const psx_lsm_params *p, int32_t nvol, const
     hipStream_t st = res.st;
     Batch &B = res.B;
     Solver &S = res.S;
+    const bool timing = std::getenv("PSX_LSM_TIMING") != nullptr;  // phase times to stderr
+    auto t0 = std::chrono::steady_clock::now();
+    auto lap = [&](const char *what) {
+        if (!timing) return;
+        (void)hipStreamSynchronize(st);
+        const auto t1 = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "psx_lsm_batch_moduli: %s %.1f ms\n", what,
+                     std::chrono::duration<double, std::milli>(t1 - t0).count());
+        t0 = t1;
+    };
     rc = batch_build(B, st, *p, nvol, dims, ind_concat, nullptr);
     if (rc) return rc;
+    lap("batch_build");
     rc = solver_alloc(S, B, MAXR, true);
     if (rc) return rc;
+    lap("solver_alloc");
     int cs[MAXR] = {0, 1, 2, 3, 4, 5};
     set_jumps(S, cs, MAXR, p->gstrain);
     PSX_HIP(hipMemcpyAsync(S.d_jump, &S.jump_h, sizeof(Jump), hipMemcpyHostToDevice, st));
@@ -2037,6 +2060,7 @@ extern "C" int psx_lsm_batch_moduli(const psx_lsm_params *p, int32_t nvol, const
     float ms = 0;
     PSX_HIP(hipEventElapsedTime(&ms, e0, e1));
     if (rc) return rc;
+    lap("rhs, solve, stress");
     for (int v = 0; v < nvol; v++)
         moduli_from_stress(&asig[(size_t)v * MAXR * 9], B.vols[v], p->gstrain, &C_out[36 * v], &KG_out[2 * v],
                            &KG_out[2 * v + 1]);

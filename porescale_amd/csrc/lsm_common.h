@@ -158,9 +158,11 @@ struct Batch {
     float *d_ka32 = nullptr, *d_kb32 = nullptr;
     double *d_minv = nullptr;
     float *d_minv32 = nullptr;
-    std::vector<double> ka_h, kb_h;
+    uint8_t *d_comp = nullptr;   // component per voxel (0 pore, 1, 2)
+    double *d_stiff = nullptr;   // stiffness multiplier per voxel, or null (all 1)
     std::vector<uint8_t> comp_h;
-    std::vector<double> stiff_h;
+    std::vector<double> stiff_h;  // empty: all 1
+    double stiff_at(int64_t i) const { return stiff_h.empty() ? 1.0 : stiff_h[(size_t)i]; }
 };
 
 // Rebuilds the tile lists of every volume for a tile shape (host side).

@@ -24,6 +24,10 @@ struct MGLevel {
     int ntiles = 0;
     int64_t N = 0;
     float *A = nullptr;     // [27*9][N] stencil (levels >= 1)
+    // the same stencil for the smoothing steps: each node's 243 entries in
+    // FP16 relative to the node's largest magnitude As[n] (null: use A)
+    _Float16 *Ah = nullptr;
+    float *As = nullptr;
     float *minv = nullptr;  // [6][N] inverse diagonal blocks
     float *lam = nullptr;   // [nvol] spectral bound of D^-1 A
     // work vectors [R][3][N]; level 0 uses the caller's r (b) and z (x)
@@ -32,7 +36,8 @@ struct MGLevel {
 };
 
 struct MGParams {
-    int levels = 4;        // at most this many levels (including level 0)
+    int levels = 6;        // at most this many levels (including level 0)
+    int coarse_max = 16;   // coarsen until the largest axis of every volume is at most this
     int nu = 3;            // Chebyshev degree of each smoothing pass (levels >= 1)
     int nu0 = 1;           // the same on the finest level (its matvec is the costliest)
     int coarse_deg = 4;    // Chebyshev degree on the coarsest level
@@ -74,8 +79,11 @@ void mg_apply(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, const
 // PSX_MG_UYZ), read at each call: a captured graph keeps those of its capture.
 int mg_launch_rall_tiles();
 int mg_launch_uyz();
-// Parameters from the environment (PSX_MG_LEVELS, PSX_MG_NU, PSX_MG_NU0, PSX_MG_CDEG,
-// PSX_MG_RATIO, PSX_MG_CRATIO, PSX_MG_POWER); defaults otherwise.
+// FP16 stencils for the smoothing steps of levels >= 1 (PSX_MG_HALF, default
+// 1), read when the hierarchy's operators are built
+int mg_half_stencils();
+// Parameters from the environment (PSX_MG_LEVELS, PSX_MG_CMAX, PSX_MG_NU, PSX_MG_NU0,
+// PSX_MG_CDEG, PSX_MG_RATIO, PSX_MG_CRATIO, PSX_MG_POWER); defaults otherwise.
 MGParams mg_params_from_env();
 
 }  // namespace lsmk

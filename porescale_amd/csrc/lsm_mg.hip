@@ -96,20 +96,10 @@ __global__ __launch_bounds__(256) void k_assemble_fine(const VolDesc *__restrict
     }
 }
 
-// Galerkin product on one level: Ac(I, slot) = sum P(i,I) Af(i, d) P(i+d, J).
-__global__ __launch_bounds__(256) void k_galerkin(const VolDesc *__restrict__ fvols, const VolDesc *__restrict__ cvols,
-                                                  const int32_t *__restrict__ ctvol,
-                                                  const int32_t *__restrict__ ctstart, int cntiles,
-                                                  const double *__restrict__ Af, int64_t Nf, double *__restrict__ Ac,
-                                                  int64_t Nc) {
-    for (int t = lsm_tile0(); t < cntiles; t += gridDim.x) {
-        const int v = ctvol[t];
-        const VolDesc fd = fvols[v], cd = cvols[v];
-        int X, Y, Z;
-        bool ok;
-        tile_node(cd, ctstart[t], X, Y, Z, ok);
-        if (!ok) continue;
-        const int64_t I = node_index(cd, X, Y, Z);
+// Galerkin product at coarse node (X, Y, Z) = I:
+// Ac(I, slot) = sum P(i,I) Af(i, d) P(i+d, J).
+__device__ void galerkin_node(const VolDesc &fd, const VolDesc &cd, int X, int Y, int Z, int64_t I,
+                              const double *__restrict__ Af, int64_t Nf, double *__restrict__ Ac, int64_t Nc) {
         for (int sz = -1; sz <= 1; sz++) {
             AxisCombo cz;
             axis_combos(Z, fd.nz, cd.nz, sz, cz);
@@ -137,7 +127,142 @@ __global__ __launch_bounds__(256) void k_galerkin(const VolDesc *__restrict__ fv
                 }
             }
         }
+}
+
+// Galerkin product on one level, from the assembled fine stencil.
+__global__ __launch_bounds__(256) void k_galerkin(const VolDesc *__restrict__ fvols, const VolDesc *__restrict__ cvols,
+                                                  const int32_t *__restrict__ ctvol,
+                                                  const int32_t *__restrict__ ctstart, int cntiles,
+                                                  const double *__restrict__ Af, int64_t Nf, double *__restrict__ Ac,
+                                                  int64_t Nc) {
+    for (int t = lsm_tile0(); t < cntiles; t += gridDim.x) {
+        const int v = ctvol[t];
+        const VolDesc fd = fvols[v], cd = cvols[v];
+        int X, Y, Z;
+        bool ok;
+        tile_node(cd, ctstart[t], X, Y, Z, ok);
+        if (!ok) continue;
+        galerkin_node(fd, cd, X, Y, Z, node_index(cd, X, Y, Z), Af, Nf, Ac, Nc);
     }
+}
+
+// The first coarse level's Galerkin product from the element matrices.
+// The fine operator is a sum of voxel (element) matrices a_v KA + b_v KB,
+// and trilinear interpolation maps fine element i of an axis onto coarse
+// element floor(i/2), its two fine corners taking the coarse corners'
+// values with the weights [[1, 0], [1/2, 1/2]] (i even), [[1/2, 1/2],
+// [0, 1]] (i odd), or [[1, 0], [0, 1]] (the last element of an odd-length
+// periodic axis, the only fine element of the coarse element that wraps).
+// So P^T A P = sum over fine voxels of a_v MA[p] + b_v MB[p] placed on the
+// coarse element, p the voxel's pattern (one of 3 per axis) and M = P_p^T K
+// P_p (mab: [27 p][2][24][24], exact dyadic values).  The coarse node sums,
+// per stencil slot, the rows of its corner over the coarse elements that
+// hold both ends: about 9k multiply-adds a node instead of the ~12k
+// stencil loads of the general product, and no assembled fine stencil.
+__global__ __launch_bounds__(256) void k_galerkin_elem(const VolDesc *__restrict__ fvols,
+                                                       const VolDesc *__restrict__ cvols,
+                                                       const int32_t *__restrict__ ctvol,
+                                                       const int32_t *__restrict__ ctstart, int cntiles,
+                                                       const double *__restrict__ ka, const double *__restrict__ kb,
+                                                       const double *__restrict__ mab, double *__restrict__ Ac,
+                                                       int64_t Nc) {
+    for (int t = lsm_tile0(); t < cntiles; t += gridDim.x) {
+        const int v = ctvol[t];
+        const VolDesc fd = fvols[v], cd = cvols[v];
+        int X, Y, Z;
+        bool ok;
+        tile_node(cd, ctstart[t], X, Y, Z, ok);
+        if (!ok) continue;
+        const int64_t I = node_index(cd, X, Y, Z);
+        for (int sz = -1; sz <= 1; sz++)
+            for (int sy = -1; sy <= 1; sy++)
+                for (int sx = -1; sx <= 1; sx++) {
+                    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                    // I sits at corner (cx, cy, cz) of a coarse element that
+                    // also holds I + (sx, sy, sz): c = 1 for -1, 0 for +1,
+                    // either for 0
+                    for (int cz = sz == -1 ? 1 : 0; cz <= (sz == 1 ? 0 : 1); cz++)
+                        for (int cy = sy == -1 ? 1 : 0; cy <= (sy == 1 ? 0 : 1); cy++)
+                            for (int cx = sx == -1 ? 1 : 0; cx <= (sx == 1 ? 0 : 1); cx++) {
+                                const int ex = wrap1(X - cx, cd.nx), ey = wrap1(Y - cy, cd.ny),
+                                          ez = wrap1(Z - cz, cd.nz);
+                                // the wrapping element of an odd fine axis holds one fine element
+                                const bool wx = (fd.nx & 1) && ex == cd.nx - 1, wy = (fd.ny & 1) && ey == cd.ny - 1,
+                                           wz = (fd.nz & 1) && ez == cd.nz - 1;
+                                const int ci = cx + 2 * cy + 4 * cz;
+                                const int cj = (cx + sx) + 2 * (cy + sy) + 4 * (cz + sz);
+                                for (int kz = 0; kz < (wz ? 1 : 2); kz++)
+                                    for (int ky = 0; ky < (wy ? 1 : 2); ky++)
+                                        for (int kx = 0; kx < (wx ? 1 : 2); kx++) {
+                                            const int64_t fv = node_index(fd, 2 * ex + kx, 2 * ey + ky, 2 * ez + kz);
+                                            const double a = ka[fv], b = kb[fv];
+                                            const int p = (wx ? 2 : kx) + 3 * (wy ? 2 : ky) + 9 * (wz ? 2 : kz);
+                                            const double *ma = mab + ((size_t)(2 * p) * 24 + 3 * ci) * 24 + 3 * cj;
+                                            const double *mb = ma + 24 * 24;
+#pragma unroll
+                                            for (int e = 0; e < 3; e++)
+#pragma unroll
+                                                for (int e2 = 0; e2 < 3; e2++)
+                                                    acc[3 * e + e2] += a * ma[24 * e + e2] + b * mb[24 * e + e2];
+                                        }
+                            }
+                    const int sc = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+#pragma unroll
+                    for (int k = 0; k < 9; k++) Ac[(int64_t)(9 * sc + k) * Nc + I] = acc[k];
+                }
+    }
+}
+
+// mab of k_galerkin_elem: for each voxel pattern p = px + 3 py + 9 pz (per
+// axis 0: even, 1: odd, 2: the wrapping element of an odd axis) the
+// projected element matrices P_p^T KA P_p and P_p^T KB P_p, rows and
+// columns 3 corner + component, corners x-bit first.
+std::vector<double> galerkin_element_tables() {
+    std::vector<double> m((size_t)27 * 2 * 24 * 24, 0.0);
+    // [pattern][fine corner bit][coarse corner bit]
+    const double P1[3][2][2] = {{{1.0, 0.0}, {0.5, 0.5}}, {{0.5, 0.5}, {0.0, 1.0}}, {{1.0, 0.0}, {0.0, 1.0}}};
+    for (int p = 0; p < 27; p++) {
+        const int pd[3] = {p % 3, (p / 3) % 3, p / 9};
+        double P[8][8];
+        for (int f = 0; f < 8; f++)
+            for (int C = 0; C < 8; C++) {
+                double w = 1.0;
+                for (int d = 0; d < 3; d++) w *= P1[pd[d]][(f >> d) & 1][(C >> d) & 1];
+                P[f][C] = w;
+            }
+        for (int k = 0; k < 2; k++) {
+            const auto &K = k == 0 ? lsm_el::KA : lsm_el::KB;
+            double *M = &m[((size_t)(2 * p + k)) * 24 * 24];
+            for (int C = 0; C < 8; C++)
+                for (int D = 0; D < 8; D++)
+                    for (int e = 0; e < 3; e++)
+                        for (int e2 = 0; e2 < 3; e2++) {
+                            double acc = 0.0;
+                            for (int f = 0; f < 8; f++)
+                                for (int g = 0; g < 8; g++)
+                                    acc += P[f][C] * K[3 * f + e][3 * g + e2] * P[g][D];
+                            M[(3 * C + e) * 24 + 3 * D + e2] = acc;
+                        }
+        }
+    }
+    return m;
+}
+
+// FP16 copy of a level's stencil for the smoothing steps: node n's 243
+// entries divided by their largest magnitude As[n] (1 for an all-zero
+// row), so every entry keeps FP16's 11-bit significand whatever the
+// springs' scale.  Halves the bytes the stencil steps stream; the power
+// iteration and the Galerkin products keep the FP32 / FP64 stencils.
+__global__ __launch_bounds__(256) void k_half_stencil(const float *__restrict__ A, int64_t N,
+                                                      _Float16 *__restrict__ Ah, float *__restrict__ As) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    float m = 0.f;
+    for (int k = 0; k < NSTEN; k++) m = fmaxf(m, fabsf(A[(int64_t)k * N + n]));
+    const float sc = m > 0.f ? m : 1.f;
+    const float inv = 1.f / sc;
+    for (int k = 0; k < NSTEN; k++) Ah[(int64_t)k * N + n] = (_Float16)(A[(int64_t)k * N + n] * inv);
+    As[n] = sc;
 }
 
 __device__ inline void inv_block(const double (&D)[3][3], double (&m)[6]) {
@@ -216,6 +341,69 @@ __global__ __launch_bounds__(256) void k_level_prep(const VolDesc *__restrict__ 
     }
 }
 
+// Level 0's Gershgorin bound max_i sum_j |(D^-1 A)_ij| per tile, with the
+// fine stencil's blocks formed from the voxel constants instead of read
+// from an assembled stencil: block (n, n + s) = sum over the voxels holding
+// both nodes of a_v KA + b_v KB at their corners (mab pattern 26, the
+// identity interpolation, holds KA and KB themselves).
+__global__ __launch_bounds__(256) void k_level_prep0_elem(const VolDesc *__restrict__ vols,
+                                                          const int32_t *__restrict__ tvol,
+                                                          const int32_t *__restrict__ tstart, int ntiles,
+                                                          const double *__restrict__ ka,
+                                                          const double *__restrict__ kb,
+                                                          const double *__restrict__ mab, int64_t N,
+                                                          const double *__restrict__ minv_in,
+                                                          double *__restrict__ pmax) {
+    __shared__ double sh[4];
+    const double *KA = mab + (size_t)(2 * 26) * 24 * 24, *KB = KA + 24 * 24;
+    for (int t = lsm_tile0(); t < ntiles; t += gridDim.x) {
+        const VolDesc vd = vols[tvol[t]];
+        int x, y, z;
+        bool ok;
+        tile_node(vd, tstart[t], x, y, z, ok);
+        double g = 0.0;
+        if (ok) {
+            const int64_t n = node_index(vd, x, y, z);
+            double m[6];
+            for (int k = 0; k < 6; k++) m[k] = minv_in[(int64_t)k * N + n];
+            const double M[3][3] = {{m[0], m[3], m[4]}, {m[3], m[1], m[5]}, {m[4], m[5], m[2]}};
+            double rs[3] = {0, 0, 0};
+            for (int sz = -1; sz <= 1; sz++)
+                for (int sy = -1; sy <= 1; sy++)
+                    for (int sx = -1; sx <= 1; sx++) {
+                        double B[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                        for (int cz = sz == -1 ? 1 : 0; cz <= (sz == 1 ? 0 : 1); cz++)
+                            for (int cy = sy == -1 ? 1 : 0; cy <= (sy == 1 ? 0 : 1); cy++)
+                                for (int cx = sx == -1 ? 1 : 0; cx <= (sx == 1 ? 0 : 1); cx++) {
+                                    const int64_t v = node_index(vd, wrap1(x - cx, vd.nx), wrap1(y - cy, vd.ny),
+                                                                 wrap1(z - cz, vd.nz));
+                                    const double a = ka[v], b = kb[v];
+                                    const int ci = cx + 2 * cy + 4 * cz;
+                                    const int cj = (cx + sx) + 2 * (cy + sy) + 4 * (cz + sz);
+                                    for (int e = 0; e < 3; e++)
+                                        for (int e2 = 0; e2 < 3; e2++)
+                                            B[e][e2] += a * KA[(3 * ci + e) * 24 + 3 * cj + e2] +
+                                                        b * KB[(3 * ci + e) * 24 + 3 * cj + e2];
+                                }
+                        for (int e = 0; e < 3; e++)
+                            for (int e2 = 0; e2 < 3; e2++)
+                                rs[e] += fabs(M[e][0] * B[0][e2] + M[e][1] * B[1][e2] + M[e][2] * B[2][e2]);
+                    }
+            g = fmax(rs[0], fmax(rs[1], rs[2]));
+        }
+        for (int off = 32; off > 0; off >>= 1) g = fmax(g, __shfl_down(g, off, 64));
+        int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        __syncthreads();
+        if (lane == 0) sh[w] = g;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double mx = 0.0;
+            for (int k = 0; k < (int)(blockDim.x >> 6); k++) mx = fmax(mx, sh[k]);
+            pmax[t] = mx;
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_vol_max(const VolDesc *__restrict__ vols, const double *__restrict__ pmax,
                                                  float *__restrict__ lam) {
     const VolDesc vd = vols[blockIdx.x];
@@ -240,6 +428,8 @@ struct LevelArgs {
     int ntiles;
     int64_t N;
     const float *A;     // stencil, or null for the element operator
+    const _Float16 *Ah;  // the stencil in FP16 scaled per node by As (smoothing steps), or null
+    const float *As;
     const float *ka, *kb;
     const float *minv;
     const float *lam;
@@ -275,7 +465,7 @@ __device__ inline void apply_minv(const float *__restrict__ minv, int64_t N, int
 // (A u)(node) for one right-hand side u ([3][N], offset to that rhs).  UYZ 3
 // unrolls the stencil's dy loop, putting nine neighbours' loads in flight
 // (1.5-2% faster smoothing at 64^3; unrolling dz as well is 55% slower).
-template <bool ELEM, int UYZ = 1>
+template <bool ELEM, int UYZ = 1, bool H = false>
 __device__ inline void op_node(const LevelArgs &L, const VolDesc &vd, int x, int y, int z, int64_t n,
                                const float *__restrict__ u, bool act, const float (&a)[8], const float (&b)[8],
                                float (&out)[3]) {
@@ -304,11 +494,20 @@ __device__ inline void op_node(const LevelArgs &L, const VolDesc &vd, int x, int
                     float u0 = u[nb], u1 = u[L.N + nb], u2 = u[2 * L.N + nb];
 #pragma unroll
                     for (int e = 0; e < 3; e++) {
-                        const float *Ar = L.A + (int64_t)(9 * s + 3 * e) * L.N + n;
-                        out[e] += Ar[0] * u0 + Ar[L.N] * u1 + Ar[2 * L.N] * u2;
+                        if (H) {
+                            const _Float16 *Ar = L.Ah + (int64_t)(9 * s + 3 * e) * L.N + n;
+                            out[e] += (float)Ar[0] * u0 + (float)Ar[L.N] * u1 + (float)Ar[2 * L.N] * u2;
+                        } else {
+                            const float *Ar = L.A + (int64_t)(9 * s + 3 * e) * L.N + n;
+                            out[e] += Ar[0] * u0 + Ar[L.N] * u1 + Ar[2 * L.N] * u2;
+                        }
                     }
                 }
             }
+        }
+        if (H) {
+            const float sc = L.As[n];
+            out[0] *= sc; out[1] *= sc; out[2] *= sc;
         }
     }
 }
@@ -402,7 +601,7 @@ __global__ __launch_bounds__(TILE) void k_cg_update_first(LevelArgs L, int R, co
 // STEP k: x += d (X0: x = d, the pass's first step from x = 0); r -= A d;
 // (NEWD) d' = c1 d + c2 D^-1 r.  k = -1 restarts the recurrence
 // (d' = D^-1 r / theta) after a coarse correction d.
-template <bool ELEM, bool NEWD, int UYZ = 1, bool X0 = false>
+template <bool ELEM, bool NEWD, int UYZ = 1, bool X0 = false, bool H = false>
 __global__ __launch_bounds__(TILE) void k_cheb_step(LevelArgs L, int R, const SysState *__restrict__ sys, int k,
                                                     float *__restrict__ xv, float *__restrict__ rv,
                                                     const float *__restrict__ dv, float *__restrict__ dout,
@@ -425,7 +624,7 @@ __global__ __launch_bounds__(TILE) void k_cheb_step(LevelArgs L, int R, const Sy
             if (!sys_active(sys, v, R, r)) continue;
             const int64_t base = (int64_t)r * 3 * L.N;
             float q[3];
-            op_node<ELEM, UYZ>(L, vd, x, y, z, n, dv + base, act, a, b, q);
+            op_node<ELEM, UYZ, H>(L, vd, x, y, z, n, dv + base, act, a, b, q);
             if (!ok) continue;
             float rr[3], dd[3];
             for (int e = 0; e < 3; e++) {
@@ -453,7 +652,7 @@ __global__ __launch_bounds__(TILE) void k_cheb_step(LevelArgs L, int R, const Sy
 // taking each backward block as the transpose of the neighbour's forward
 // block (the Galerkin operator is symmetric) was measured 4-8 % slower: the
 // neighbour's blocks miss the cache across tile edges.
-template <bool NEWD, bool X0 = false>
+template <bool NEWD, bool X0 = false, bool H = false>
 __global__ __launch_bounds__(TILE) void k_cheb_step_rall(LevelArgs L, int R, const SysState *__restrict__ sys,
                                                          int k, float *__restrict__ xv, float *__restrict__ rv,
                                                          const float *__restrict__ dv, float *__restrict__ dout,
@@ -491,7 +690,8 @@ __global__ __launch_bounds__(TILE) void k_cheb_step_rall(LevelArgs L, int R, con
                     const int64_t nb = node_index(vd, xx, yy, zz);
                     float Ab[9];
 #pragma unroll
-                    for (int j = 0; j < 9; j++) Ab[j] = L.A[(int64_t)(9 * s + j) * L.N + n];
+                    for (int j = 0; j < 9; j++)
+                        Ab[j] = H ? (float)L.Ah[(int64_t)(9 * s + j) * L.N + n] : L.A[(int64_t)(9 * s + j) * L.N + n];
 #pragma unroll
                     for (int r = 0; r < MAXR; r++) {
                         if (!act[r]) continue;
@@ -505,10 +705,12 @@ __global__ __launch_bounds__(TILE) void k_cheb_step_rall(LevelArgs L, int R, con
         }
         float c1 = 0.f, c2 = 0.f;
         if (NEWD) cheb_coef(L.lam[v], L.ratio, k, c1, c2);
+        const float sc = H ? L.As[n] : 1.f;
 #pragma unroll
         for (int r = 0; r < MAXR; r++) {
             if (!act[r]) continue;
             const int64_t base = (int64_t)r * 3 * L.N;
+            if (H) { q[r][0] *= sc; q[r][1] *= sc; q[r][2] *= sc; }
             float rr[3], dd[3];
             for (int e = 0; e < 3; e++) {
                 const int64_t i = base + e * L.N + n;
@@ -767,7 +969,7 @@ __global__ void k_pow_lam(int nvol, const double *__restrict__ nrm, float *__res
 LevelArgs args_of(const MGLevel &l, const Batch &B, float ratio) {
     LevelArgs a{};
     a.vols = l.d_vols; a.tvol = l.d_tvol; a.tstart = l.d_tstart; a.ntiles = l.ntiles; a.N = l.N;
-    a.A = l.A; a.ka = B.d_ka32; a.kb = B.d_kb32; a.minv = l.minv; a.lam = l.lam; a.ratio = ratio;
+    a.A = l.A; a.Ah = l.Ah; a.As = l.As; a.ka = B.d_ka32; a.kb = B.d_kb32; a.minv = l.minv; a.lam = l.lam; a.ratio = ratio;
     return a;
 }
 
@@ -784,15 +986,25 @@ void launch_step_t(hipStream_t st, const LevelArgs &L, bool elem, int R, const S
     // records the variant of its capture (the solver's graph key holds both)
     const int rall_tiles = mg_launch_rall_tiles();
     const int uyz = mg_launch_uyz();
+    const bool h = L.Ah != nullptr;
     if (elem)
         hipLaunchKernelGGL((k_cheb_step<true, NEWD, 1, X0>), dim3(g, R), dim3(TILE), 0, st, L, R, sys, k, x, r, d,
+                           dout, rin);
+    else if (rall_tiles > 0 && L.ntiles >= rall_tiles && h)
+        hipLaunchKernelGGL((k_cheb_step_rall<NEWD, X0, true>), dim3(g), dim3(TILE), 0, st, L, R, sys, k, x, r, d,
                            dout, rin);
     else if (rall_tiles > 0 && L.ntiles >= rall_tiles)
         hipLaunchKernelGGL((k_cheb_step_rall<NEWD, X0>), dim3(g), dim3(TILE), 0, st, L, R, sys, k, x, r, d, dout,
                            rin);
+    else if (uyz >= 3 && h)
+        hipLaunchKernelGGL((k_cheb_step<false, NEWD, 3, X0, true>), dim3(g, R), dim3(TILE), 0, st, L, R, sys, k, x,
+                           r, d, dout, rin);
     else if (uyz >= 3)
         hipLaunchKernelGGL((k_cheb_step<false, NEWD, 3, X0>), dim3(g, R), dim3(TILE), 0, st, L, R, sys, k, x, r, d,
                            dout, rin);
+    else if (h)
+        hipLaunchKernelGGL((k_cheb_step<false, NEWD, 1, X0, true>), dim3(g, R), dim3(TILE), 0, st, L, R, sys, k, x,
+                           r, d, dout, rin);
     else
         hipLaunchKernelGGL((k_cheb_step<false, NEWD, 1, X0>), dim3(g, R), dim3(TILE), 0, st, L, R, sys, k, x, r, d,
                            dout, rin);
@@ -903,6 +1115,7 @@ MGParams mg_params_from_env() {
     p.levels = std::max(1, env_int("PSX_MG_LEVELS", p.levels));
     p.nu = std::max(1, env_int("PSX_MG_NU", p.nu));
     p.nu0 = std::max(1, env_int("PSX_MG_NU0", p.nu0));
+    p.coarse_max = std::max(3, env_int("PSX_MG_CMAX", p.coarse_max));
     p.coarse_deg = std::max(1, env_int("PSX_MG_CDEG", p.coarse_deg));
     p.ratio = std::max(1.5f, env_float("PSX_MG_RATIO", p.ratio));
     p.coarse_ratio = std::max(1.5f, env_float("PSX_MG_CRATIO", p.coarse_ratio));
@@ -912,7 +1125,7 @@ MGParams mg_params_from_env() {
 
 void mg_free(MG &mg) {
     for (auto &l : mg.lv) {
-        void *bufs[] = {l.d_vols, l.d_tvol, l.d_tstart, l.A, l.lam, l.r, l.dA, l.dB};
+        void *bufs[] = {l.d_vols, l.d_tvol, l.d_tstart, l.A, l.Ah, l.As, l.lam, l.r, l.dA, l.dB};
         for (void *b : bufs)
             if (b) (void)hipFree(b);
         if (l.own_bx) { (void)hipFree(l.b); (void)hipFree(l.x); }
@@ -922,6 +1135,7 @@ void mg_free(MG &mg) {
 }
 
 int mg_launch_rall_tiles() { return env_int("PSX_MG_RALL_TILES", 1024); }
+int mg_half_stencils() { return env_int("PSX_MG_HALF", 1); }
 int mg_launch_uyz() { return env_int("PSX_MG_UYZ", 3); }
 
 int mg_build(hipStream_t st, const Batch &B, int R, const MGParams &prm, MG &mg) {
@@ -931,14 +1145,24 @@ int mg_build(hipStream_t st, const Batch &B, int R, const MGParams &prm, MG &mg)
     mg.prm = prm;
     mg.R = R;
     const int nvol = (int)B.vols.size();
-    // number of levels: coarsen while every axis of every volume keeps >= 3 coarse nodes
+    // number of levels: coarsen while every axis of every volume keeps >= 3
+    // coarse nodes and, past two levels, some axis is longer than
+    // coarse_max.  Measured
+    // (six-case moduli, rtol 1e-5 and 1e-10): a 16^3 coarsest level is best
+    // for the 64^3 pack (3 levels; 4 levels are 4-7 % slower) and the 128^3
+    // pack (4 levels; 3 are 35-43 % slower, 5 are 4 % slower); the uneven
+    // 32^3-96^3 batch keeps 4 (3 are 4 % slower)
     int L = 1;
     {
         std::vector<VolDesc> cur = B.vols;
         while (L < prm.levels) {
             bool can = true;
-            for (auto &d : cur) can &= d.nx >= 6 && d.ny >= 6 && d.nz >= 6;
-            if (!can) break;
+            int longest = 0;
+            for (auto &d : cur) {
+                can &= d.nx >= 6 && d.ny >= 6 && d.nz >= 6;
+                longest = std::max(longest, std::max(d.nx, std::max(d.ny, d.nz)));
+            }
+            if (!can || (L >= 2 && longest <= prm.coarse_max)) break;  // two levels where they fit
             for (auto &d : cur) { d.nx = (d.nx + 1) / 2; d.ny = (d.ny + 1) / 2; d.nz = (d.nz + 1) / 2; }
             L++;
         }
@@ -989,14 +1213,31 @@ int mg_operators(hipStream_t st, const Batch &B, MG &mg) {
     RC(dalloc(&tmp, maxtiles));
     pmax_buf.reset(tmp);
     double *const pmax = tmp;
-    RC(dalloc(&tmp, (size_t)NSTEN * l0.N));
-    Af.reset(tmp);
-    hipLaunchKernelGGL(k_assemble_fine, dim3(grid_for(l0.ntiles)), dim3(256), 0, st, l0.d_vols, l0.d_tvol,
-                       l0.d_tstart, l0.ntiles, B.d_ka, B.d_kb, Af.get(), l0.N);
+    // Level 0 and the first Galerkin product from the voxel constants and
+    // the projected element matrices (no assembled fine stencil: 1.9 kB a
+    // node), PSX_MG_GALERKIN_ELEM=0 the assembled stencil and the general
+    // product
+    const bool elem = env_int("PSX_MG_GALERKIN_ELEM", 1) != 0;
+    psx::DeviceBuf<double> d_mab;
+    if (elem) {
+        const std::vector<double> mab = galerkin_element_tables();
+        RC(dalloc(&tmp, mab.size()));
+        d_mab.reset(tmp);
+        PSX_HIP(hipMemcpyAsync(d_mab.get(), mab.data(), sizeof(double) * mab.size(), hipMemcpyHostToDevice, st));
+    } else {
+        RC(dalloc(&tmp, (size_t)NSTEN * l0.N));
+        Af.reset(tmp);
+        hipLaunchKernelGGL(k_assemble_fine, dim3(grid_for(l0.ntiles)), dim3(256), 0, st, l0.d_vols, l0.d_tvol,
+                           l0.d_tstart, l0.ntiles, B.d_ka, B.d_kb, Af.get(), l0.N);
+    }
     for (int l = 0; l < L; l++) {
         MGLevel &lv = mg.lv[l];
         if (!lv.lam) RC(dalloc(&lv.lam, nvol));
-        if (l == 0) {
+        if (l == 0 && elem) {
+            hipLaunchKernelGGL(k_level_prep0_elem, dim3(grid_for(lv.ntiles)), dim3(256), 0, st, lv.d_vols, lv.d_tvol,
+                               lv.d_tstart, lv.ntiles, B.d_ka, B.d_kb, (const double *)d_mab.get(), lv.N,
+                               (const double *)B.d_minv, pmax);
+        } else if (l == 0) {
             hipLaunchKernelGGL(k_level_prep, dim3(grid_for(lv.ntiles)), dim3(256), 0, st, lv.d_vols, lv.d_tvol,
                                lv.d_tstart, lv.ntiles, (const double *)Af.get(), lv.N, (const double *)B.d_minv,
                                (float *)nullptr, (float *)nullptr, pmax);
@@ -1009,6 +1250,12 @@ int mg_operators(hipStream_t st, const Batch &B, MG &mg) {
             hipLaunchKernelGGL(k_level_prep, dim3(grid_for(lv.ntiles)), dim3(256), 0, st, lv.d_vols, lv.d_tvol,
                                lv.d_tstart, lv.ntiles, (const double *)Af.get(), lv.N, (const double *)nullptr, lv.A,
                                lv.minv, pmax);
+            if (mg_half_stencils()) {
+                if (!lv.Ah) RC(dalloc(&lv.Ah, (size_t)NSTEN * lv.N));
+                if (!lv.As) RC(dalloc(&lv.As, (size_t)lv.N));
+                hipLaunchKernelGGL(k_half_stencil, dim3((unsigned)psx::ceil_div(lv.N, 256)), dim3(256), 0, st,
+                                   (const float *)lv.A, lv.N, lv.Ah, lv.As);
+            }
         }
         hipLaunchKernelGGL(k_vol_max, dim3(nvol), dim3(256), 0, st, lv.d_vols, (const double *)pmax, lv.lam);
         PSX_HIP(hipGetLastError());
@@ -1016,8 +1263,17 @@ int mg_operators(hipStream_t st, const Batch &B, MG &mg) {
             MGLevel &cv = mg.lv[l + 1];
             RC(dalloc(&tmp, (size_t)NSTEN * cv.N));
             psx::DeviceBuf<double> Ac(tmp);
-            hipLaunchKernelGGL(k_galerkin, dim3(grid_for(cv.ntiles)), dim3(256), 0, st, lv.d_vols, cv.d_vols,
-                               cv.d_tvol, cv.d_tstart, cv.ntiles, (const double *)Af.get(), lv.N, Ac.get(), cv.N);
+            if (l == 0 && elem) {
+                hipLaunchKernelGGL(k_galerkin_elem, dim3(grid_for(cv.ntiles)), dim3(256), 0, st, lv.d_vols,
+                                   cv.d_vols, cv.d_tvol, cv.d_tstart, cv.ntiles, B.d_ka, B.d_kb,
+                                   (const double *)d_mab.get(), Ac.get(), cv.N);
+                PSX_HIP(hipGetLastError());
+                PSX_HIP(hipStreamSynchronize(st));
+            } else {
+                hipLaunchKernelGGL(k_galerkin, dim3(grid_for(cv.ntiles)), dim3(256), 0, st, lv.d_vols, cv.d_vols,
+                                   cv.d_tvol, cv.d_tstart, cv.ntiles, (const double *)Af.get(), lv.N, Ac.get(),
+                                   cv.N);
+            }
             PSX_HIP(hipGetLastError());
             PSX_HIP(hipStreamSynchronize(st));
             Af = std::move(Ac);

@@ -182,3 +182,69 @@ def test_generated_node_stencil_equals_element_assembly(mats):
     # the rows of a homogeneous solid sum to zero (rigid translations)
     st = stencil(np.ones(8), np.full(8, 0.1))
     np.testing.assert_allclose(st.sum(axis=0), 0.0, atol=1e-13)
+
+
+def _periodic_interp(nf):
+    """1D trilinear interpolation of the multigrid: coarse node I on fine 2I,
+    odd fine nodes halfway between their two (periodic) even neighbours."""
+    P = np.zeros((nf, (nf + 1) // 2))
+    for j in range(nf):
+        if j % 2 == 0:
+            P[j, j // 2] = 1.0
+        else:
+            P[j, (j - 1) // 2] += 0.5
+            P[j, (0 if j + 1 == nf else j + 1) // 2] += 0.5
+    return P
+
+
+def _assemble(n, elems):
+    """Dense matrix of a periodic grid (x fastest) from per-element 24x24 blocks."""
+    nx, ny, nz = n
+    N = nx * ny * nz
+    A = np.zeros((3 * N, 3 * N))
+    for (x, y, z), K in elems:
+        nodes = [((x + c[0]) % nx) + nx * (((y + c[1]) % ny) + ny * ((z + c[2]) % nz)) for c in CORNER]
+        for f in range(8):
+            for h in range(8):
+                A[3 * nodes[f]:3 * nodes[f] + 3, 3 * nodes[h]:3 * nodes[h] + 3] += K[3 * f:3 * f + 3, 3 * h:3 * h + 3]
+    return A
+
+
+@pytest.mark.parametrize("n", [(6, 6, 6), (7, 6, 8), (7, 9, 5)])
+def test_first_galerkin_product_from_projected_element_matrices(n, mats):
+    """The first coarse operator P^T A P (trilinear P, periodic, odd or even
+    axes) equals the sum over fine voxels of a_v P_p^T KA P_p + b_v P_p^T KB
+    P_p placed on coarse element floor(v/2), with the per-axis patterns the
+    multigrid set-up tabulates (even voxel [[1,0],[1/2,1/2]], odd
+    [[1/2,1/2],[0,1]], the wrapping voxel of an odd axis the identity).  A
+    wrong pattern or a transposed corner order moves entries by O(1)."""
+    KA, KB = mats[0], mats[1]
+    rng = np.random.default_rng(3)
+    nx, ny, nz = n
+    N = nx * ny * nz
+    ka = rng.uniform(0.2, 1.5, N)
+    kb = rng.uniform(0.01, 0.2, N)
+    ka[rng.random(N) < 0.3] = 0.0
+    kb[ka == 0.0] = 0.0
+    vid = lambda x, y, z: x + nx * (y + ny * z)  # noqa: E731
+    fine = [((x, y, z), ka[vid(x, y, z)] * KA + kb[vid(x, y, z)] * KB)
+            for z in range(nz) for y in range(ny) for x in range(nx)]
+    P = np.kron(np.kron(_periodic_interp(nz), _periodic_interp(ny)), _periodic_interp(nx))
+    P3 = np.kron(P, np.eye(3))
+    want = P3.T @ _assemble(n, fine) @ P3
+    pat = [np.array([[1.0, 0.0], [0.5, 0.5]]), np.array([[0.5, 0.5], [0.0, 1.0]]), np.eye(2)]
+    nc = [(a + 1) // 2 for a in n]
+    coarse = []
+    for z in range(nz):
+        for y in range(ny):
+            for x in range(nx):
+                f = (x, y, z)
+                E = tuple(c // 2 for c in f)
+                p = [2 if (n[d] % 2 == 1 and f[d] == n[d] - 1) else f[d] % 2 for d in range(3)]
+                Pe = np.array([[np.prod([pat[p[d]][(a >> d) & 1, (b >> d) & 1] for d in range(3)])
+                                for b in range(8)] for a in range(8)])
+                Pe3 = np.kron(Pe, np.eye(3))
+                v = vid(x, y, z)
+                coarse.append((E, Pe3.T @ (ka[v] * KA + kb[v] * KB) @ Pe3))
+    got = _assemble(tuple(nc), coarse)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * np.abs(want).max())

@@ -174,6 +174,33 @@ def test_all_rhs_coarse_step_batch_equals_single(monkeypatch):
     assert max(its) - min(its) <= 2, its
 
 
+def test_multigrid_setup_variants_same_preconditioner(monkeypatch):
+    """The first Galerkin product from the projected element matrices (the
+    default) and from the assembled fine stencil (PSX_MG_GALERKIN_ELEM=0),
+    and the FP16 smoothing stencils (default) against the FP32 ones
+    (PSX_MG_HALF=0), on a batch with odd and even axes (the wrapping coarse
+    element of an odd axis holds one fine voxel): the same moduli and, the
+    preconditioners agreeing to rounding, the same iteration counts within
+    one."""
+    vols = [geo.two_component(geo.sphere_pack(sh, 0.35, 3.0, seed=80 + i), 0.4, seed=90 + i, radius=2.0)
+            for i, sh in enumerate([(20, 20, 20), (23, 18, 21), (17, 25, 19)])]
+    pars = (1.0, 0.1, 0.3, 0.02)
+    runs = {}
+    for label, env in (("default", {}), ("assembled-stencil Galerkin product", {"PSX_MG_GALERKIN_ELEM": "0"}),
+                       ("FP32 smoothing stencils", {"PSX_MG_HALF": "0"})):
+        for k, val in env.items():
+            monkeypatch.setenv(k, val)
+        _say(f"case multigrid set-up: {label}")
+        runs[label] = batch_moduli(vols, *pars, precision="f32", rtol=1e-10)
+        for k in env:
+            monkeypatch.delenv(k)
+    C0, KG0, _, it0 = runs["default"]
+    for label, (C1, KG1, _, it1) in runs.items():
+        np.testing.assert_allclose(C1, C0, rtol=0, atol=1e-8 * np.abs(C0).max())
+        np.testing.assert_allclose(KG1, KG0, rtol=1e-8)
+        assert abs(it1 - it0) <= 1, (label, it1, it0)
+
+
 @pytest.mark.parametrize("levels", ["1", "4"])
 def test_multigrid_coarsest_degree_one_same_moduli(monkeypatch, levels):
     """A coarsest level of Chebyshev degree 1 takes no smoothing step, so its
@@ -186,6 +213,7 @@ def test_multigrid_coarsest_degree_one_same_moduli(monkeypatch, levels):
         C6, K, G = s.moduli(rtol=1e-10)
     monkeypatch.setenv("PSX_MG_CDEG", "1")
     monkeypatch.setenv("PSX_MG_LEVELS", levels)
+    monkeypatch.setenv("PSX_MG_CMAX", "3")  # coarsen as deep as the levels allow
     _say(f"case coarsest degree 1, {levels} level(s)")
     with LSM(ind, *pars, precision="f32") as s:
         C1, K1, G1 = s.moduli(rtol=1e-10, maxit=200000)
