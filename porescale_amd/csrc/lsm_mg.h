@@ -45,6 +45,7 @@ struct MGParams {
     float coarse_ratio = 200.f;
     int power_steps = 30;        // power-iteration steps per level for the spectral bound
     float power_safety = 1.1f;   // bound = min(Gershgorin, safety * power estimate)
+    int rall0 = 1;               // finest-level smoothing with all right-hand sides per thread
 };
 
 struct MG {
@@ -83,7 +84,8 @@ int mg_launch_uyz();
 // 1), read when the hierarchy's operators are built
 int mg_half_stencils();
 // Parameters from the environment (PSX_MG_LEVELS, PSX_MG_CMAX, PSX_MG_NU, PSX_MG_NU0,
-// PSX_MG_CDEG, PSX_MG_RATIO, PSX_MG_CRATIO, PSX_MG_POWER); defaults otherwise.
+// PSX_MG_CDEG, PSX_MG_RATIO, PSX_MG_CRATIO, PSX_MG_POWER, PSX_MG_RALL0); defaults
+// otherwise.
 MGParams mg_params_from_env();
 
 }  // namespace lsmk

@@ -779,6 +779,71 @@ __global__ __launch_bounds__(TILE) void k_restart_final0(LevelArgs L, int R, con
     }
 }
 
+// The finest level's two element-operator smoothing kernels with every
+// right-hand side in one thread, the voxel scales gathered once per node
+// (the CG matvec's layout), instead of one grid row per right-hand side:
+//   FINAL = false: the first step of a degree-1 pass from x = 0:
+//     x = d, r = b - A d (k_cheb_step<true, false, 1, true>);
+//   FINAL = true: the restart that ends the V-cycle: z = x + e +
+//     D^-1 (r - A e) / theta written to x, with the (tile, rhs) partials of
+//     r_cg.z into prz (k_restart_final0).
+template <bool FINAL>
+__global__ __launch_bounds__(TILE) void k_step0_rall(LevelArgs L, int R, const SysState *__restrict__ sys,
+                                                     float *__restrict__ xv, float *__restrict__ rv,
+                                                     const float *__restrict__ dv, const float *__restrict__ rin,
+                                                     const float *__restrict__ rcg, double *__restrict__ prz) {
+    __shared__ double sh[TILE / 64];
+    for (int t = lsm_tile0(); t < L.ntiles; t += gridDim.x) {
+        const int v = L.tvol[t];
+        const VolDesc vd = L.vols[v];
+        int x, y, z;
+        bool ok;
+        tile_node(vd, L.tstart[t], x, y, z, ok);
+        const int64_t n = node_index(vd, x, y, z);
+        float a[8], b[8];
+        const bool act = ok && gather_k(vd, L.ka, L.kb, x, y, z, a, b);
+        float c1 = 0.f, c2 = 0.f;
+        if (FINAL) cheb_coef(L.lam[v], L.ratio, -1, c1, c2);
+        for (int r = 0; r < R; r++) {
+            if (!sys_active(sys, v, R, r)) {  // uniform over the tile (one volume)
+                if (FINAL && prz && threadIdx.x == 0) prz[(int64_t)t * R + r] = 0.0;
+                continue;
+            }
+            const int64_t base = (int64_t)r * 3 * L.N;
+            float q[3];
+            op_node<true>(L, vd, x, y, z, n, dv + base, act, a, b, q);
+            double dot = 0.0;
+            if (ok) {
+                if (!FINAL) {
+                    for (int e = 0; e < 3; e++) {
+                        const int64_t i = base + e * L.N + n;
+                        xv[i] = dv[i];
+                        rv[i] = rin[i] - q[e];
+                    }
+                } else {
+                    float rr[3], xx[3], zz[3];
+                    for (int e = 0; e < 3; e++) {
+                        const int64_t i = base + e * L.N + n;
+                        xx[e] = xv[i] + dv[i];
+                        rr[e] = rv[i] - q[e];
+                    }
+                    apply_minv(L.minv, L.N, n, rr, zz);
+                    for (int e = 0; e < 3; e++) {
+                        const int64_t i = base + e * L.N + n;
+                        const float zf = xx[e] + c2 * zz[e];
+                        xv[i] = zf;
+                        dot += (double)rcg[i] * (double)zf;
+                    }
+                }
+            }
+            if (FINAL && prz) {
+                double s = block_sum(dot, sh);
+                if (threadIdx.x == 0) prz[(int64_t)t * R + r] = s;
+            }
+        }
+    }
+}
+
 // coarse node I: r = P^T r_fine (the level's right-hand side; no copy is
 // kept); x = 0 when xc is given; d = D^-1 r / theta
 __global__ __launch_bounds__(TILE) void k_restrict_first(LevelArgs F, LevelArgs C, int R,
@@ -1054,7 +1119,11 @@ float *vcycle(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, int l
         x0 = false;
         std::swap(dA, dB);
     }
-    launch_step<false>(st, L, elem, mg.R, sys, 0, lv.x, lv.r, dA, nullptr, x0, x0 ? rin : nullptr);
+    if (elem && nu == 1 && p.rall0)  // x = d, r = b - A d with all right-hand sides per thread
+        hipLaunchKernelGGL(k_step0_rall<false>, dim3(grid_for(L.ntiles)), dim3(TILE), 0, st, L, mg.R, sys, lv.x,
+                           lv.r, (const float *)dA, (const float *)rin, (const float *)nullptr, (double *)nullptr);
+    else
+        launch_step<false>(st, L, elem, mg.R, sys, 0, lv.x, lv.r, dA, nullptr, x0, x0 ? rin : nullptr);
     // coarse correction
     MGLevel &cv = mg.lv[l + 1];
     const bool c_coarsest = l + 2 == (int)mg.lv.size();
@@ -1066,8 +1135,12 @@ float *vcycle(hipStream_t st, const Batch &B, MG &mg, const SysState *sys, int l
                        (const float *)pend, dA);
     // post-smoothing: restart from the corrected x, nu updates
     if (l == 0 && nu == 1 && finished) {  // the restart is the last update: it also forms z and r.z
-        hipLaunchKernelGGL(k_restart_final0, dim3(grid_for(L.ntiles), mg.R), dim3(TILE), 0, st, L, mg.R, sys, lv.x,
-                           lv.r, (const float *)dA, rcg, prz);
+        if (p.rall0)
+            hipLaunchKernelGGL(k_step0_rall<true>, dim3(grid_for(L.ntiles)), dim3(TILE), 0, st, L, mg.R, sys, lv.x,
+                               lv.r, (const float *)dA, (const float *)nullptr, rcg, prz);
+        else
+            hipLaunchKernelGGL(k_restart_final0, dim3(grid_for(L.ntiles), mg.R), dim3(TILE), 0, st, L, mg.R, sys,
+                               lv.x, lv.r, (const float *)dA, rcg, prz);
         *finished = true;
         return nullptr;
     }
@@ -1116,6 +1189,7 @@ MGParams mg_params_from_env() {
     p.nu = std::max(1, env_int("PSX_MG_NU", p.nu));
     p.nu0 = std::max(1, env_int("PSX_MG_NU0", p.nu0));
     p.coarse_max = std::max(3, env_int("PSX_MG_CMAX", p.coarse_max));
+    p.rall0 = env_int("PSX_MG_RALL0", p.rall0) != 0;
     p.coarse_deg = std::max(1, env_int("PSX_MG_CDEG", p.coarse_deg));
     p.ratio = std::max(1.5f, env_float("PSX_MG_RATIO", p.ratio));
     p.coarse_ratio = std::max(1.5f, env_float("PSX_MG_CRATIO", p.coarse_ratio));
