@@ -185,10 +185,12 @@ def test_fp32_state_round_trip_and_first_step(prec32):
 
 @pytest.mark.parametrize("prec32", ["f32", "f32x"])
 def test_fp32_compressed_pull_table_is_exact(monkeypatch, prec32):
-    """The int16 pull tables (direct offsets, flag 1; rank differences within
-    x-planes, flag 2) and the collide kernel's node-major int32 table (taken
-    with flags 0 and 2) address exactly the populations the direction-major
-    int32 table does (flag 0 without the node-major table): states
+    """Every pull-table form addresses exactly the populations the collide
+    kernel's element table does (precomputed element indices, voxel and
+    coordinates; the default): with the element table off, the
+    direction-major int32 table, the node-major one, and the int16 tables
+    (direct offsets, flag 1; rank differences within x-planes, flag 2, the
+    density kernel's form); with it on, each density-kernel form.  States
     bit-identical."""
     ind = geo.sphere_pack((20, 14, 16), 0.4, 3.0, seed=17)
     nz, ny, nx = ind.shape
@@ -197,16 +199,19 @@ def test_fp32_compressed_pull_table_is_exact(monkeypatch, prec32):
     state = ob.equilibrium_state(inp, ob.fluid_order_mask(ind, geo.phase_blobs(ind, 0.5, 5, radius=2.5)))
     out = {}
     monkeypatch.setenv("PSX_LBM_FUSED", "0")  # the int16 table serves the two-kernel step
-    for flag, aos in (("0", "0"), ("0", "1"), ("1", "1"), ("2", "1")):
+    forms = [(e, f, a) for e in ("0", "1") for f, a in (("0", "0"), ("0", "1"), ("1", "1"), ("2", "1"))]
+    for etab, flag, aos in forms:
+        monkeypatch.setenv("PSX_LBM_ETAB", etab)
         monkeypatch.setenv("PSX_LBM_IDX16", flag)
         monkeypatch.setenv("PSX_LBM_SRCAOS", aos)
-        _say(f"case {prec32} pull table int16={flag} node-major={aos}")
+        _say(f"case {prec32} element table={etab} pull table int16={flag} node-major={aos}")
         with LBM(inp, ind, precision=prec32) as lb:
             lb.set_state(*state)
             lb.step(40)
-            out[flag + aos] = lb.get_state()
-    for key in ("01", "11", "21"):
-        assert np.array_equal(out["00"][0], out[key][0]) and np.array_equal(out["00"][1], out[key][1])
+            out[etab + flag + aos] = lb.get_state()
+    ref = out["100"]
+    for key, (fr, fb) in out.items():
+        assert np.array_equal(ref[0], fr) and np.array_equal(ref[1], fb), key
 
 
 @pytest.mark.parametrize("prec32", ["f32", "f32x"])
