@@ -89,6 +89,22 @@ __device__ __forceinline__ double frcp(double x) {
     return fma(rc, fma(-x, rc, 1.0), rc);
 }
 
+// The colour word of fraction r (frac_word_t) with the common case first:
+// for 2^-126 <= r < 1 (after rounding) the word and extension byte are the
+// double's own rounded bits repacked, six integer operations; subnormal
+// fractions and escapes (parts of opposite sign) take the full conversion,
+// which a wavefront then runs only when one of its lanes needs it.
+template <int XB>
+__device__ __forceinline__ uint32_t frac_word_fast(double r, uint32_t &x) {
+    const uint64_t rb = frac_dbits(r) + (1ull << (27 - XB));
+    const uint32_t hi = (uint32_t)(rb >> 32), lo = (uint32_t)rb;
+    if (hi - (897u << 20) < (126u << 20)) {
+        x = XB ? (lo >> (28 - XB)) & ((1u << XB) - 1u) : 0u;
+        return (hi << 5) - (896u << 25) + ((lo >> 27) & 0x1Eu);
+    }
+    return frac_word_t<XB>(r, x);
+}
+
 // Stores the population (fr, fb) of direction q of node j: T and the colour
 // word, and for X the word's extension byte (33-bit fraction).  The
 // minority's fraction m / f is the hardware reciprocal, one Newton step and
@@ -112,7 +128,7 @@ __device__ __forceinline__ void put(const LbmConst &c, float *__restrict__ out, 
         r = f == 0.0 ? 0.5 : mn / f;
     }
     uint32_t x;
-    const uint32_t w = frac_word_t<X ? 8 : 0>(r, x) | (blue ? 1u : 0u);
+    const uint32_t w = frac_word_fast<X ? 8 : 0>(r, x) | (blue ? 1u : 0u);
     out[aos(Q + q, j)] = __uint_as_float(w);
     if (X) outx[aosx(q, j)] = (uint8_t)x;
 }
