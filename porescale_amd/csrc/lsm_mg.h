@@ -43,7 +43,10 @@ struct MGParams {
     int coarse_deg = 4;    // Chebyshev degree on the coarsest level
     float ratio = 30.f;    // smoothing interval [lam/ratio, lam]
     float coarse_ratio = 200.f;
-    int power_steps = 30;        // power-iteration steps per level for the spectral bound
+    // power-iteration steps per level for the spectral bound: 20 and 30 give
+    // the same iteration counts at 64^3, 128^3 and on the uneven batch
+    // (12 is 26-32 % slower at 128^3)
+    int power_steps = 20;
     float power_safety = 1.1f;   // bound = min(Gershgorin, safety * power estimate)
     int rall0 = 1;               // finest-level smoothing with all right-hand sides per thread
 };
