@@ -664,7 +664,7 @@ extern "C" int psx_lbm_set_state(psx_lbm *h, const double *fr, const double *fb)
                     else k = opp(q);
                 }
                 const int64_t r = h->ref_node(i);
-                f32_encode(h->c.w0[q], fr[Q * r + q], fb[Q * r + q], h->f32x, st[aos(k, j)], st[aos(Q + k, j)],
+                f32_encode(h->c.w0[q], fr[Q * r + q], fb[Q * r + q], h->f32x, st[2 * aos(k, j)], st[2 * aos(k, j) + 1],
                            sx[h->f32x ? aosx(k, j) : 0]);
             }
         PSX_HIP(hipMemcpyAsync(h->d_pop[0], st.data(), sizeof(float) * st.size(), hipMemcpyHostToDevice, h->stream));
@@ -709,9 +709,9 @@ extern "C" int psx_lbm_get_state(psx_lbm *h, double *fr, double *fb) {
                     if (sidx >= 0) j = sidx;
                     else k = opp(q);
                 }
-                double f = h->c.w0[q] + (double)st[aos(k, j)];
+                double f = h->c.w0[q] + (double)st[2 * aos(k, j)];
                 const int64_t o = h->ref_node(i);
-                frac_split(f, st[aos(Q + k, j)], fr[Q * o + q], fb[Q * o + q], h->f32x ? sx[aosx(k, j)] : 0u);
+                frac_split(f, st[2 * aos(k, j) + 1], fr[Q * o + q], fb[Q * o + q], h->f32x ? sx[aosx(k, j)] : 0u);
             }
     } else {
         std::vector<double> st((size_t)2 * Q * nl);

@@ -34,6 +34,11 @@ struct LbmConst {
     int32_t nx, ny, nz, pad_;
 };
 
+// The per-direction weight arrays tr, tb and w0 hold one value per lattice
+// shell: the rest direction, the six axis links and the twelve diagonals.
+// Reading them at wcls(q) keeps three values live instead of nineteen.
+__host__ __device__ constexpr int wcls(int q) { return q == 0 ? 0 : (q < 7 ? 1 : 7); }
+
 // ----------------------------------------------------------- storage ----
 struct StoreF64 {
     using phi_t = double;
@@ -50,22 +55,34 @@ struct StoreF64 {
 };
 
 // FP32 pull layout: post-collision populations in AoSoA blocks of 64 nodes,
-// block-major: element (slot k, node n) at (n/64)*38*64 + k*64 + n%64, slots
-// 0..18 hold T, 19..37 the colour word s (frac_encode).  A node pulls
-// direction q from the node src streams from, or from its own opposite slot
-// for a bounce-back link.  With the extended colour word the extension
-// bytes follow the float blocks, indexed like the floats: the byte of
-// (direction k, node n) is byte aosx(k, n) = aos(k, n) past the
-// 2*19*aos_pad(nl) floats (xbytes(nl) bytes, the bytes at colour-word
-// positions unused), so a pull reads T, s and x at one element index.
+// block-major: element (direction k, node n) is e = aos(k, n) =
+// (n/64)*19*64 + k*64 + n%64, and element e is the float pair at floats 2e
+// (T) and 2e + 1 (the colour word s, frac_encode), so one population is one
+// 8-byte access.  A node pulls direction q from the node src streams from,
+// or from its own opposite slot for a bounce-back link.  With the extended
+// colour word the extension bytes follow the 2*19*aos_pad(nl) floats,
+// byte e for element e (xbytes(nl) bytes), so a pull reads T, s and x at
+// one element index.
 constexpr int AOS = 64;
 __host__ __device__ __forceinline__ int64_t aos(int k, int64_t n) {
-    return (n >> 6) * (2 * Q * AOS) + (int64_t)k * AOS + (n & (AOS - 1));
+    return (n >> 6) * (Q * AOS) + (int64_t)k * AOS + (n & (AOS - 1));
 }
 __host__ __device__ __forceinline__ int64_t aosx(int k, int64_t n) { return aos(k, n); }
 __host__ __device__ __forceinline__ int64_t aos_pad(int64_t nl) { return (nl + AOS - 1) / AOS * AOS; }
 // bytes of extension bytes per population buffer
-__host__ __device__ __forceinline__ int64_t xbytes(int64_t nl) { return 2 * Q * aos_pad(nl); }
+__host__ __device__ __forceinline__ int64_t xbytes(int64_t nl) { return Q * aos_pad(nl); }
+// T and the colour word of element e, and both in one access
+template <class I>
+__device__ __forceinline__ float2 pop_ld(const float *__restrict__ p, I e) {
+    return reinterpret_cast<const float2 *>(p)[e];
+}
+template <class I>
+__device__ __forceinline__ float pop_word(const float *__restrict__ p, I e) {
+    return pop_ld(p, e).y;
+}
+__device__ __forceinline__ void pop_st(float *__restrict__ p, int64_t e, float t, float s) {
+    reinterpret_cast<float2 *>(p)[e] = make_float2(t, s);
+}
 
 struct PullF32 {
     // the colour field stays FP64: calcFs differentiates it, and an FP32 grid
@@ -83,8 +100,9 @@ struct PullF32 {
             if (s >= 0) j = s;
             else k = opp(q);
         }
-        double f = c.w0[q] + (double)p[aos(k, j)];  // w0 is the same for q and opp(q)
-        frac_split(f, p[aos(Q + k, j)], fr, fb, x ? (uint32_t)x[aosx(k, j)] : 0u);
+        const float2 ts = pop_ld(p, aos(k, j));
+        double f = c.w0[wcls(q)] + (double)ts.x;  // w0 is the same for q and opp(q)
+        frac_split(f, ts.y, fr, fb, x ? (uint32_t)x[aosx(k, j)] : 0u);
     }
 };
 

@@ -115,7 +115,6 @@ template <bool X>
 __device__ __forceinline__ void put(const LbmConst &c, float *__restrict__ out, uint8_t *__restrict__ outx, int q,
                                     int64_t j, double fr, double fb) {
     const double f = fr + fb;
-    out[aos(q, j)] = (float)(f - c.w0[q]);
     const bool blue = fabs(fb) < fabs(fr);
     const double mn = blue ? fb : fr;
     double r;
@@ -129,7 +128,7 @@ __device__ __forceinline__ void put(const LbmConst &c, float *__restrict__ out, 
     }
     uint32_t x;
     const uint32_t w = frac_word_fast<X ? 8 : 0>(r, x) | (blue ? 1u : 0u);
-    out[aos(Q + q, j)] = __uint_as_float(w);
+    pop_st(out, aos(q, j), (float)(f - c.w0[wcls(q)]), __uint_as_float(w));
     if (X) outx[aosx(q, j)] = (uint8_t)x;
 }
 
@@ -297,8 +296,8 @@ __device__ __forceinline__ void collide_node_f32(const LbmConst &c, double (&f)[
     const double t2B = t2 * c.B, t3B = t3 * c.B;
 #pragma unroll
     for (int q = 1; q < Q; q++) {
-        if (mode == 0) { o_r = f[q]; o_b = c.tb[q] * rb; }
-        else if (mode == 1) { o_b = f[q]; o_r = c.tr[q] * rr; }
+        if (mode == 0) { o_r = f[q]; o_b = c.tb[wcls(q)] * rb; }
+        else if (mode == 1) { o_b = f[q]; o_r = c.tr[wcls(q)] * rr; }
         else {
             // cos of the angle between the colour gradient and c_q (L1:3786-3803)
             const int cx = H_IV[q][0], cy = H_IV[q][1], cz = H_IV[q][2];
@@ -355,8 +354,9 @@ __device__ __forceinline__ void collide_pull(const LbmConst &c, const float *__r
             pull_from<M>(P, pb, nl, i, q, j, k);
         }
         at[q] = (decltype(at[0] + 0))aos(k, j);
-        double t = c.w0[q] + (double)in[at[q]];
-        float s = in[at[q] + Q * AOS];
+        const float2 ts = pop_ld(in, at[q]);
+        double t = c.w0[wcls(q)] + (double)ts.x;
+        const float s = ts.y;
         xs[q] = X ? (uint32_t)inx[at[q]] : 0u;
         double a, b;
         frac_split_fast(t, s, xs[q], a, b, esc);
@@ -371,7 +371,7 @@ __device__ __forceinline__ void collide_pull(const LbmConst &c, const float *__r
         rb = 0.0;
         for (int q = 0; q < Q; q++) {
             double a, b;
-            frac_split(f[q], in[at[q] + Q * AOS], a, b, xs[q]);
+            frac_split(f[q], pop_word(in, at[q]), a, b, xs[q]);
             rr += a;
             rb += b;
         }
@@ -381,7 +381,7 @@ __device__ __forceinline__ void collide_pull(const LbmConst &c, const float *__r
 #pragma unroll
         for (int q = 0; q < Q; q++) {
             double a, b;
-            frac_split(f[q], in[at[q] + Q * AOS], a, b, xs[q]);
+            frac_split(f[q], pop_word(in, at[q]), a, b, xs[q]);
             f[q] = blue ? b : a;
         }
     };
@@ -430,8 +430,9 @@ __device__ __forceinline__ void collide_pull_e(const LbmConst &c, const float *_
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         const uint32_t at = q == 0 ? self : e[q - 1];
-        double t = c.w0[q] + (double)in[at];
-        const float s = in[at + Q * AOS];
+        const float2 ts = pop_ld(in, at);
+        double t = c.w0[wcls(q)] + (double)ts.x;
+        const float s = ts.y;
         const uint32_t xb = X ? (uint32_t)inx[at] : 0u;
         double a, b;
         frac_split_fast(t, s, xb, a, b, esc);
@@ -449,7 +450,7 @@ __device__ __forceinline__ void collide_pull_e(const LbmConst &c, const float *_
         for (int q = 0; q < Q; q++) {
             const uint32_t at = at_of(q);
             double a, b;
-            frac_split(f[q], in[at + Q * AOS], a, b, X ? (uint32_t)inx[at] : 0u);
+            frac_split(f[q], pop_word(in, at), a, b, X ? (uint32_t)inx[at] : 0u);
             rr += a;
             rb += b;
         }
@@ -459,7 +460,7 @@ __device__ __forceinline__ void collide_pull_e(const LbmConst &c, const float *_
         for (int q = 0; q < Q; q++) {
             const uint32_t at = at_of(q);
             double a, b;
-            frac_split(f[q], in[at + Q * AOS], a, b, X ? (uint32_t)inx[at] : 0u);
+            frac_split(f[q], pop_word(in, at), a, b, X ? (uint32_t)inx[at] : 0u);
             f[q] = blue ? b : a;
         }
     };
@@ -483,10 +484,11 @@ __device__ __forceinline__ double colour_pull(const LbmConst &c, const float *__
         int64_t j;
         int k;
         pull_from<M>(P, pb, c.nl, i, q, j, k);
-        int64_t a = aos(k, j);
-        double t = c.w0[q] + (double)in[a];
+        const int64_t a = aos(k, j);
+        const float2 ts = pop_ld(in, a);
+        double t = c.w0[wcls(q)] + (double)ts.x;
         double fr, fb;
-        frac_split(t, in[a + Q * AOS], fr, fb, X ? (uint32_t)inx[aosx(k, j)] : 0u);
+        frac_split(t, ts.y, fr, fb, X ? (uint32_t)inx[a] : 0u);
         rr += fr;
         rb += fb;
     }
@@ -659,7 +661,7 @@ struct FOutF32 {
     double w0[Q];
     __device__ __forceinline__ double operator()(int64_t n, int qb) const {
         const int q = opp(qb);
-        return w0[q] + (double)p[aos(q, n)];
+        return w0[q] + (double)p[2 * aos(q, n)];
     }
 };
 }  // namespace
