@@ -488,12 +488,13 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
     // the packed coordinates, so the kernel forms no address and divides
     // nothing (collide 2-4 % faster at 256^3 than with the int32 source
     // table).  The density kernel keeps its int16 table: reading this one
-    // instead (80 B a node for 40) made it 29 % slower.  It needs 32-bit
-    // element indices and dimensions below 1025; PSX_LBM_ETAB=0 keeps the
-    // index tables above.
+    // instead (80 B a node for 40) made it 29 % slower.  It needs element
+    // byte offsets below 2^32 (a population buffer below 4 GiB, 28 M fluid
+    // nodes) and dimensions below 1025; PSX_LBM_ETAB=0 keeps the index
+    // tables above.
     const char *ee = std::getenv("PSX_LBM_ETAB");
     const bool etab = h->f32 && !h->fused && nl > 0 && (!ee || std::atoi(ee) != 0) && !psx_lbm_fast::wide_index(nl) &&
-                      nx <= 1024 && ny <= 1024 && nz <= 1024;
+                      (int64_t)Q * aos_pad(nl) * 8 <= (int64_t)UINT32_MAX && nx <= 1024 && ny <= 1024 && nz <= 1024;
     if (etab) {
         std::vector<uint32_t> et(20 * (size_t)nl);
         for (int64_t n = 0; n < nl; n++) {

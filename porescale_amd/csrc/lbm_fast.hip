@@ -424,13 +424,19 @@ __device__ __forceinline__ void collide_pull_e(const LbmConst &c, const float *_
         e[4 * g] = w.x; e[4 * g + 1] = w.y; e[4 * g + 2] = w.z; e[4 * g + 3] = w.w;
     }
     const uint32_t self = (uint32_t)aos(0, i);
+    // element at as a 32-bit byte offset (the table is built only for
+    // buffers below 4 GiB): the load takes the buffer base from scalar
+    // registers
+    auto ld = [&](uint32_t at) -> float2 {
+        return *reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(in) + (at << 3));
+    };
     double f[Q];
     double rr = 0.0, rb = 0.0;
     bool esc = false;
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         const uint32_t at = q == 0 ? self : e[q - 1];
-        const float2 ts = pop_ld(in, at);
+        const float2 ts = ld(at);
         double t = c.w0[wcls(q)] + (double)ts.x;
         const float s = ts.y;
         const uint32_t xb = X ? (uint32_t)inx[at] : 0u;
@@ -450,7 +456,7 @@ __device__ __forceinline__ void collide_pull_e(const LbmConst &c, const float *_
         for (int q = 0; q < Q; q++) {
             const uint32_t at = at_of(q);
             double a, b;
-            frac_split(f[q], pop_word(in, at), a, b, X ? (uint32_t)inx[at] : 0u);
+            frac_split(f[q], ld(at).y, a, b, X ? (uint32_t)inx[at] : 0u);
             rr += a;
             rb += b;
         }
@@ -460,7 +466,7 @@ __device__ __forceinline__ void collide_pull_e(const LbmConst &c, const float *_
         for (int q = 0; q < Q; q++) {
             const uint32_t at = at_of(q);
             double a, b;
-            frac_split(f[q], pop_word(in, at), a, b, X ? (uint32_t)inx[at] : 0u);
+            frac_split(f[q], ld(at).y, a, b, X ? (uint32_t)inx[at] : 0u);
             f[q] = blue ? b : a;
         }
     };
