@@ -70,6 +70,7 @@ struct psx_lbm {
     int32_t *d_srca = nullptr;   // FP32 path: node-major int32 pull table [nl][20] for collide
     int32_t *d_pbase = nullptr;  // [3][nx] per-plane bases of d_src16
     uint32_t *d_etab = nullptr;  // FP32 path: the collide kernel's element table [nl][20] (null: not used)
+    uint16_t *d_dtab = nullptr;  // FP32 path: the density kernel's node-major int16 pull table (null: not used)
     void *d_pop[2] = {nullptr, nullptr};
     void *d_phi = nullptr;
     double *d_partial = nullptr, *d_red = nullptr;
@@ -210,7 +211,7 @@ int launch_density(psx_lbm *h, int buf) {
     if (nb == 0) return PSX_OK;
     if (h->f32) {
         PSX_HIP(psx_lbm_fast::density(h->c, (const float *)h->d_pop[buf], popx(h, buf), h->d_dst, h->d_src16,
-                                      h->d_pbase, h->d_vox, (double *)h->d_phi, h->xcd, h->stream));
+                                      h->d_pbase, h->d_dtab, h->d_vox, (double *)h->d_phi, h->xcd, h->stream));
     } else {
         StoreF64 s{(double *)h->d_pop[buf], h->nl};
         hipLaunchKernelGGL(k_density<StoreF64>, dim3((unsigned)nb), dim3(256), 0, h->stream, h->c, s,
@@ -521,6 +522,24 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
             PSX_HIP(hipStreamSynchronize(h->stream));
         }
     }
+    // With the element table the int16 pull table serves the density kernel
+    // alone; it goes node-major there, the 18 offsets of a node in 32 + 4 B
+    // and its voxel beside the last two: three loads a node instead of 19.
+    const char *ed = std::getenv("PSX_LBM_DTAB");
+    if (use16 && etab && (!ed || std::atoi(ed) != 0)) {
+        std::vector<uint16_t> dt(20 * (size_t)nl);
+        for (int64_t n = 0; n < nl; n++) {
+            for (int q = 1; q <= 16; q++) dt[16 * (size_t)n + q - 1] = (uint16_t)s16[(size_t)(q - 1) * nl + n];
+            uint16_t *t = &dt[16 * (size_t)nl + 4 * (size_t)n];
+            t[0] = (uint16_t)s16[(size_t)16 * nl + n];
+            t[1] = (uint16_t)s16[(size_t)17 * nl + n];
+            const uint32_t v = (uint32_t)vox[n];
+            std::memcpy(&t[2], &v, 4);
+        }
+        PSX_HIP(hipMalloc(&h->d_dtab, sizeof(uint16_t) * dt.size()));
+        PSX_HIP(hipMemcpyAsync(h->d_dtab, dt.data(), sizeof(uint16_t) * dt.size(), hipMemcpyHostToDevice, h->stream));
+        PSX_HIP(hipStreamSynchronize(h->stream));
+    }
     if (use16) {
         PSX_HIP(hipMalloc(&h->d_src16, sizeof(int16_t) * s16.size()));
         PSX_HIP(hipMemcpyAsync(h->d_src16, s16.data(), sizeof(int16_t) * s16.size(), hipMemcpyHostToDevice, h->stream));
@@ -563,7 +582,7 @@ extern "C" void psx_lbm_destroy(psx_lbm *h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void *bufs[] = {h->d_pop[0], h->d_pop[1], h->d_dst, h->d_vox, h->d_phi, h->d_partial, h->d_red, h->d_src16,
-                    h->d_pbase, h->d_bricks, h->d_hnode, h->d_hcell, h->d_srca, h->d_etab};
+                    h->d_pbase, h->d_bricks, h->d_hnode, h->d_hcell, h->d_srca, h->d_etab, h->d_dtab};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (h->stream) (void)hipStreamDestroy(h->stream);

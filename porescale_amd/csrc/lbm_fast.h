@@ -28,9 +28,11 @@ hipError_t collide(const psx_lbm_dev::LbmConst &c, const float *in, const uint8_
                    const int32_t *src, const int16_t *src16, const int32_t *pbase, const int32_t *srca,
                    const uint32_t *etab, const int32_t *vox, const double *phi, int xcd, hipStream_t st);
 // Pulled densities -> colour field phi at every node's voxel.
+// dtab, when not null, is the int16 table node-major: [nl][16] offsets of
+// directions 1..16, then [nl] x {offset 17, offset 18, voxel (int32)}.
 hipError_t density(const psx_lbm_dev::LbmConst &c, const float *in, const uint8_t *inx, const int32_t *src,
-                   const int16_t *src16, const int32_t *pbase, const int32_t *vox, double *phi, int xcd,
-                   hipStream_t st);
+                   const int16_t *src16, const int32_t *pbase, const uint16_t *dtab, const int32_t *vox,
+                   double *phi, int xcd, hipStream_t st);
 // Fused pull + colour + collide step over bricks (k_step_f32): one launch
 // per time step.  box = the LDS box (brick + halo) dimensions (x, y, z).
 hipError_t step(const psx_lbm_dev::LbmConst &c, const float *in, float *out, const int32_t *src,

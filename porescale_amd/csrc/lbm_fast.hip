@@ -501,6 +501,48 @@ __device__ __forceinline__ double colour_pull(const LbmConst &c, const float *__
     return (rr - rb) / (rr + rb);
 }
 
+// Colour field of node i from the node-major int16 table (direct offsets,
+// or plane-based with PB): the same pulls and sums as colour_pull.
+template <bool PB, bool X>
+__global__ __launch_bounds__(BLOCK) void k_density_f32n(const LbmConst c, const float *__restrict__ in,
+                                                        const uint8_t *__restrict__ inx,
+                                                        const int32_t *__restrict__ pbase,
+                                                        const uint16_t *__restrict__ dtab, double *__restrict__ phi,
+                                                        int xcd) {
+    const int64_t i = work_block(xcd) * blockDim.x + threadIdx.x;
+    if (i >= c.nl) return;
+    const uint4 *pa = reinterpret_cast<const uint4 *>(dtab) + 2 * i;
+    const uint4 a0 = pa[0], a1 = pa[1];
+    const uint2 b = reinterpret_cast<const uint2 *>(dtab + 16 * c.nl)[i];
+    const uint32_t w[9] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w, b.x};
+    const int32_t v = (int32_t)b.y;
+    int32_t pb[3] = {0, 0, 0};
+    if (PB) {
+        const int Xp = v / (c.ny * c.nz);
+        pb[0] = pbase[Xp]; pb[1] = pbase[c.nx + Xp]; pb[2] = pbase[2 * c.nx + Xp];
+    }
+    double rr = 0.0, rb = 0.0;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        int64_t j = i;
+        int k = q;
+        if (q > 0) {
+            const uint32_t h = w[(q - 1) >> 1];
+            const int d = (int16_t)((q - 1) & 1 ? h >> 16 : h & 0xFFFFu);
+            if (d == -32768) k = opp(q);
+            else j = i + (PB ? pb[cxq(q) + 1] : 0) + d;
+        }
+        const int64_t a = aos(k, j);
+        const float2 ts = pop_ld(in, a);
+        double t = c.w0[wcls(q)] + (double)ts.x;
+        double fr, fb;
+        frac_split(t, ts.y, fr, fb, X ? (uint32_t)inx[a] : 0u);
+        rr += fr;
+        rb += fb;
+    }
+    phi[v] = (rr - rb) / (rr + rb);
+}
+
 template <int M, bool WIDE, bool X>
 __global__ __launch_bounds__(BLOCK) void k_collide_f32(const LbmConst c, const float *__restrict__ in,
                                                        const uint8_t *__restrict__ inx, float *__restrict__ out,
@@ -638,10 +680,25 @@ hipError_t collide(const LbmConst &c, const float *in, const uint8_t *inx, float
 }
 
 hipError_t density(const LbmConst &c, const float *in, const uint8_t *inx, const int32_t *src,
-                   const int16_t *src16, const int32_t *pbase, const int32_t *vox, double *phi, int xcd,
-                   hipStream_t st) {
+                   const int16_t *src16, const int32_t *pbase, const uint16_t *dtab, const int32_t *vox,
+                   double *phi, int xcd, hipStream_t st) {
     int64_t nb = psx::ceil_div(c.nl, BLOCK);
     if (nb == 0) return hipSuccess;
+    if (dtab) {
+        if (pbase && inx)
+            hipLaunchKernelGGL((k_density_f32n<true, true>), dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, inx,
+                               pbase, dtab, phi, xcd);
+        else if (pbase)
+            hipLaunchKernelGGL((k_density_f32n<true, false>), dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, inx,
+                               pbase, dtab, phi, xcd);
+        else if (inx)
+            hipLaunchKernelGGL((k_density_f32n<false, true>), dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, inx,
+                               pbase, dtab, phi, xcd);
+        else
+            hipLaunchKernelGGL((k_density_f32n<false, false>), dim3((unsigned)nb), dim3(BLOCK), 0, st, c, in, inx,
+                               pbase, dtab, phi, xcd);
+        return hipGetLastError();
+    }
     const PullIdx P{src, src16, pbase};
     if (src16 && pbase) launch_density<1>(c, in, inx, P, vox, phi, xcd, st, nb);
     else if (src16) launch_density<2>(c, in, inx, P, vox, phi, xcd, st, nb);

@@ -190,7 +190,8 @@ def test_fp32_compressed_pull_table_is_exact(monkeypatch, prec32):
     coordinates; the default): with the element table off, the
     direction-major int32 table, the node-major one, and the int16 tables
     (direct offsets, flag 1; rank differences within x-planes, flag 2, the
-    density kernel's form); with it on, each density-kernel form.  States
+    density kernel's form); with it on, each density-kernel form, the int16
+    ones direction-major and node-major (the default).  States
     bit-identical."""
     ind = geo.sphere_pack((20, 14, 16), 0.4, 3.0, seed=17)
     nz, ny, nx = ind.shape
@@ -199,17 +200,19 @@ def test_fp32_compressed_pull_table_is_exact(monkeypatch, prec32):
     state = ob.equilibrium_state(inp, ob.fluid_order_mask(ind, geo.phase_blobs(ind, 0.5, 5, radius=2.5)))
     out = {}
     monkeypatch.setenv("PSX_LBM_FUSED", "0")  # the int16 table serves the two-kernel step
-    forms = [(e, f, a) for e in ("0", "1") for f, a in (("0", "0"), ("0", "1"), ("1", "1"), ("2", "1"))]
-    for etab, flag, aos in forms:
+    forms = [(e, f, a, "1") for e in ("0", "1") for f, a in (("0", "0"), ("0", "1"), ("1", "1"), ("2", "1"))]
+    forms += [("1", f, "1", "0") for f in ("1", "2")]
+    for etab, flag, aos, dtab in forms:
         monkeypatch.setenv("PSX_LBM_ETAB", etab)
         monkeypatch.setenv("PSX_LBM_IDX16", flag)
         monkeypatch.setenv("PSX_LBM_SRCAOS", aos)
-        _say(f"case {prec32} element table={etab} pull table int16={flag} node-major={aos}")
+        monkeypatch.setenv("PSX_LBM_DTAB", dtab)
+        _say(f"case {prec32} element table={etab} pull table int16={flag} node-major={aos} density table={dtab}")
         with LBM(inp, ind, precision=prec32) as lb:
             lb.set_state(*state)
             lb.step(40)
-            out[etab + flag + aos] = lb.get_state()
-    ref = out["100"]
+            out[etab + flag + aos + dtab] = lb.get_state()
+    ref = out["1001"]
     for key, (fr, fb) in out.items():
         assert np.array_equal(ref[0], fr) and np.array_equal(ref[1], fb), key
 
