@@ -692,10 +692,14 @@ __global__ __launch_bounds__(TILE) void k_cheb_step_rall(LevelArgs L, int R, con
 #pragma unroll
                     for (int j = 0; j < 9; j++)
                         Ab[j] = H ? (float)L.Ah[(int64_t)(9 * s + j) * L.N + n] : L.A[(int64_t)(9 * s + j) * L.N + n];
+                    // every right-hand side's loads unconditionally (an
+                    // absent one, r >= R, reads right-hand side 0 and is
+                    // dropped at the store), so the 18 loads of a
+                    // neighbour are in flight together instead of three at
+                    // a time behind a branch per right-hand side
 #pragma unroll
                     for (int r = 0; r < MAXR; r++) {
-                        if (!act[r]) continue;
-                        const float *u = dv + (int64_t)r * 3 * L.N;
+                        const float *u = dv + (int64_t)(r < R ? r : 0) * 3 * L.N;
                         const float u0 = u[nb], u1 = u[L.N + nb], u2 = u[2 * L.N + nb];
 #pragma unroll
                         for (int e = 0; e < 3; e++) q[r][e] += Ab[3 * e] * u0 + Ab[3 * e + 1] * u1 + Ab[3 * e + 2] * u2;
