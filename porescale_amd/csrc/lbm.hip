@@ -541,8 +541,11 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
         PSX_HIP(hipStreamSynchronize(h->stream));
     }
     if (use16) {
-        PSX_HIP(hipMalloc(&h->d_src16, sizeof(int16_t) * s16.size()));
-        PSX_HIP(hipMemcpyAsync(h->d_src16, s16.data(), sizeof(int16_t) * s16.size(), hipMemcpyHostToDevice, h->stream));
+        if (!h->d_dtab) {  // else no kernel reads the direction-major form
+            PSX_HIP(hipMalloc(&h->d_src16, sizeof(int16_t) * s16.size()));
+            PSX_HIP(hipMemcpyAsync(h->d_src16, s16.data(), sizeof(int16_t) * s16.size(), hipMemcpyHostToDevice,
+                                   h->stream));
+        }
         if (!direct) {  // the plane-based form needs the bases
             PSX_HIP(hipMalloc(&h->d_pbase, sizeof(int32_t) * 3 * (size_t)nx));
             PSX_HIP(hipMemcpyAsync(h->d_pbase, pbase.data(), sizeof(int32_t) * 3 * nx, hipMemcpyHostToDevice, h->stream));
