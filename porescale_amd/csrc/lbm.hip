@@ -15,8 +15,10 @@
 //                the opposite direction).
 //                FP32 mode (lbm_fast.hip): per (q,node) T = (fr+fb) - tr_q*rr0
 //                and the minority colour's fraction of fr+fb, its sign naming
-//                the colour, as floats, post-collision, in AoSoA blocks
-//                of 64 nodes, streamed by a pull through src[18][nl] (source
+//                the colour, as a pair of 32-bit words side by side,
+//                post-collision, in AoSoA blocks of 64 nodes (with f32x one
+//                extension byte per population after them), streamed by a
+//                pull through src[18][nl] (source
 //                node of each incoming link, -1 for bounce-back), held on the
 //                device as int16 rank differences within x-planes plus
 //                per-plane bases (36 B per node instead of 72) when every
@@ -475,13 +477,13 @@ extern "C" int psx_lbm_create(const psx_lbm_params *p, int32_t nx, int32_t ny, i
     PSX_HIP(hipMemsetAsync(h->d_pop[0], 0, popb, h->stream));
     PSX_HIP(hipMalloc(&h->d_dst, sizeof(int32_t) * 18 * (size_t)(nl > 0 ? nl : 1)));
     PSX_HIP(hipMalloc(&h->d_vox, sizeof(int32_t) * (size_t)(nl > 0 ? nl : 1)));
-    // With the extended colour words (three loads per population) a collide
-    // wavefront would hold more loads than it can keep in flight (63) with
-    // 18 table loads per node: its int32 table, taken when the direct int16
-    // form does not fit, goes node-major, five 16-byte loads a node
-    // (measured at 256^3: collide 0.61 ms instead of 0.92; without the
-    // extension it gains nothing, 0.567 vs 0.562 ms; PSX_LBM_SRCAOS=0/1
-    // forces it off/on):
+    // With the extended colour words a collide wavefront would hold more
+    // loads than it can keep in flight (63) with 18 table loads per node:
+    // its int32 table, taken when the direct int16 form does not fit, goes
+    // node-major, five 16-byte loads a node (measured at 256^3 with T and the
+    // word in separate slots, three loads per population: collide 0.61 ms
+    // instead of 0.92; without the extension it gains nothing, 0.567 vs
+    // 0.562 ms; PSX_LBM_SRCAOS=0/1 forces it off/on):
     const char *ea = std::getenv("PSX_LBM_SRCAOS");
     const bool node_major = ea ? std::atoi(ea) != 0 : h->f32x;
     // The collide kernel's element table (collide_pull_e): per node the 18

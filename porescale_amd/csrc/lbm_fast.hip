@@ -2,14 +2,15 @@
 // per (direction, node) as
 //   T = (fr + fb) - tr_q*rr0   (FP32)   and   s = the colour word of the
 //   minority colour's fraction of fr + fb (25-bit significand, lbm_frac.h)
-// in 32 bits each (arithmetic in FP64 registers), post-collision and not yet
-// streamed, in AoSoA blocks of 64 nodes (lbm_device.h: aos()).  Streaming is
-// a pull: each node gathers direction q from its source node (src table) or,
-// for a bounce-back link, from its own opposite slot, and writes its own
-// slots contiguously.  Measured on MI355X, gathered reads with contiguous
-// writes move the step's populations 1.6x faster than the scattered writes
-// of a push, and the 64-node blocks keep each wavefront's 38 streams inside
-// one 9.7 KB span.  This translation unit allows FMA contraction and
+// in 32 bits each, side by side as one 8-byte element (arithmetic in FP64
+// registers), post-collision and not yet streamed, in AoSoA blocks of 64
+// nodes (lbm_device.h: aos()).  Streaming is a pull: each node gathers
+// direction q from its source node (src table) or, for a bounce-back link,
+// from its own opposite slot, and writes its own slots contiguously.
+// Measured on MI355X, gathered reads with contiguous writes move the step's
+// populations 1.6x faster than the scattered writes of a push, and the
+// 64-node blocks keep each wavefront's 19 element streams inside one 9.7 KB
+// span.  This translation unit allows FMA contraction and
 // replaces the per-node divisions of the recolouring step (L1:3798-3816) and
 // of calcFs by one reciprocal each.
 #include <cstdlib>
