@@ -14,8 +14,8 @@
 //                bounce-back link: the population returns to the same node in
 //                the opposite direction).
 //                FP32 mode (lbm_fast.hip): per (q,node) T = (fr+fb) - tr_q*rr0
-//                and the minority colour's fraction of fr+fb, its sign naming
-//                the colour, as a pair of 32-bit words side by side,
+//                and the colour word (the minority colour's fraction of
+//                fr+fb, bit 0 naming the colour), a pair of 32-bit words,
 //                post-collision, in AoSoA blocks of 64 nodes (with f32x one
 //                extension byte per population after them), streamed by a
 //                pull through src[18][nl] (source
