@@ -626,19 +626,20 @@ __global__ __launch_bounds__(TILE) void k_cheb_step(LevelArgs L, int R, const Sy
             float q[3];
             op_node<ELEM, UYZ, H>(L, vd, x, y, z, n, dv + base, act, a, b, q);
             if (!ok) continue;
-            float rr[3], dd[3];
+            // every load before the first store (rs may alias r)
+            float rr[3], dd[3], xo[3], zz[3];
             for (int e = 0; e < 3; e++) {
                 const int64_t i = base + e * L.N + n;
                 dd[e] = dv[i];
-                if (X0) xv[i] = dd[e];
-                else xv[i] += dd[e];
+                xo[e] = X0 ? 0.f : xv[i];
                 rr[e] = rs[i] - q[e];
-                rv[i] = rr[e];
             }
-            if (NEWD) {
-                float zz[3];
-                apply_minv(L.minv, L.N, n, rr, zz);
-                for (int e = 0; e < 3; e++) dout[base + e * L.N + n] = c1 * dd[e] + c2 * zz[e];
+            if (NEWD) apply_minv(L.minv, L.N, n, rr, zz);
+            for (int e = 0; e < 3; e++) {
+                const int64_t i = base + e * L.N + n;
+                xv[i] = X0 ? dd[e] : xo[e] + dd[e];
+                rv[i] = rr[e];
+                if (NEWD) dout[i] = c1 * dd[e] + c2 * zz[e];
             }
         }
     }
@@ -710,24 +711,43 @@ __global__ __launch_bounds__(TILE) void k_cheb_step_rall(LevelArgs L, int R, con
         float c1 = 0.f, c2 = 0.f;
         if (NEWD) cheb_coef(L.lam[v], L.ratio, k, c1, c2);
         const float sc = H ? L.As[n] : 1.f;
+        // every load of the update before its first store (r may alias rs,
+        // which would otherwise hold each right-hand side's loads behind the
+        // previous one's stores); absent right-hand sides read the first
+        float dd[MAXR][3], xo[MAXR][3], rr[MAXR][3], mv[6];
+#pragma unroll
+        for (int r = 0; r < MAXR; r++) {
+            const int64_t base = (int64_t)(r < R ? r : 0) * 3 * L.N;
+#pragma unroll
+            for (int e = 0; e < 3; e++) {
+                const int64_t i = base + e * L.N + n;
+                dd[r][e] = dv[i];
+                xo[r][e] = X0 ? 0.f : xv[i];
+                rr[r][e] = rs[i];
+            }
+        }
+        if (NEWD)
+#pragma unroll
+            for (int j = 0; j < 6; j++) mv[j] = L.minv[j * L.N + n];
 #pragma unroll
         for (int r = 0; r < MAXR; r++) {
             if (!act[r]) continue;
             const int64_t base = (int64_t)r * 3 * L.N;
             if (H) { q[r][0] *= sc; q[r][1] *= sc; q[r][2] *= sc; }
-            float rr[3], dd[3];
+#pragma unroll
             for (int e = 0; e < 3; e++) {
                 const int64_t i = base + e * L.N + n;
-                dd[e] = dv[i];
-                if (X0) xv[i] = dd[e];
-                else xv[i] += dd[e];
-                rr[e] = rs[i] - q[r][e];
-                rv[i] = rr[e];
+                xv[i] = X0 ? dd[r][e] : xo[r][e] + dd[r][e];
+                rr[r][e] -= q[r][e];
+                rv[i] = rr[r][e];
             }
             if (NEWD) {
-                float zz[3];
-                apply_minv(L.minv, L.N, n, rr, zz);
-                for (int e = 0; e < 3; e++) dout[base + e * L.N + n] = c1 * dd[e] + c2 * zz[e];
+                const float z0 = mv[0] * rr[r][0] + mv[3] * rr[r][1] + mv[4] * rr[r][2];
+                const float z1 = mv[3] * rr[r][0] + mv[1] * rr[r][1] + mv[5] * rr[r][2];
+                const float z2 = mv[4] * rr[r][0] + mv[5] * rr[r][1] + mv[2] * rr[r][2];
+                dout[base + n] = c1 * dd[r][0] + c2 * z0;
+                dout[base + L.N + n] = c1 * dd[r][1] + c2 * z1;
+                dout[base + 2 * L.N + n] = c1 * dd[r][2] + c2 * z2;
             }
         }
     }
